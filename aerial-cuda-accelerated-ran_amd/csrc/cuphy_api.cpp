@@ -1,0 +1,392 @@
+// C ABI of the LDPC decode path: context, tensor descriptors, error strings
+// and the LDPC decoder entry points.  Argument validation and return codes
+// follow the reference's cuPHY/src/cuphy/cuphy_ldpc.cpp:26-222 and
+// error_correction/ldpc.cpp:709-913.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <array>
+#include <cstring>
+#include <new>
+
+#include "cuphy_hip.h"
+#include "ldpc_decode.hpp"
+
+// ---------------------------------------------------------------------------
+// context (cuphy.h:1156-1208): records the device it was created on
+// ---------------------------------------------------------------------------
+struct cuphyContext {
+    int device = 0;
+    int cu_count = 0;
+    size_t lds_per_block = 0;
+};
+
+// ---------------------------------------------------------------------------
+// tensor descriptor (cuphy.h:1225-1470): type, rank, dims, strides
+// ---------------------------------------------------------------------------
+struct cuphyTensorDescriptor {
+    cuphyDataType_t type = CUPHY_VOID;
+    int rank = 0;
+    int dims[CUPHY_DIM_MAX] = {0};
+    int strides[CUPHY_DIM_MAX] = {0};
+};
+
+// ---------------------------------------------------------------------------
+// LDPC decoder object: immutable after creation (decodes from several host
+// threads on different streams are safe), owns no device memory.
+// ---------------------------------------------------------------------------
+struct cuphyLDPCDecoder {
+    int device = 0;
+};
+
+namespace {
+
+size_t type_bytes(cuphyDataType_t t)
+{
+    switch (t) {
+    case CUPHY_R_32F: case CUPHY_R_32I: case CUPHY_R_32U: return 4;
+    case CUPHY_R_64F: return 8;
+    case CUPHY_R_16F: case CUPHY_R_16I: case CUPHY_R_16U: return 2;
+    case CUPHY_C_16F: return 4;
+    case CUPHY_R_8I: case CUPHY_R_8U: return 1;
+    case CUPHY_BIT: return 0;  // sub-byte: stored in 32-bit words
+    default: return 0;
+    }
+}
+
+bool valid_type(cuphyDataType_t t)
+{
+    return t == CUPHY_BIT || type_bytes(t) > 0;
+}
+
+// Normalisation tables g_min_sum_norm_BG{1,2}_Z384 indexed by mb
+// (error_correction/ldpc.cpp:139-235; identical to LDPC_decode.m:101-208).
+const float kNormBG1[47] = {0.0f,  0.0f,  0.0f,  0.0f,  0.79f, 0.77f, 0.75f, 0.73f, 0.75f, 0.70f, 0.67f, 0.68f,
+                            0.67f, 0.67f, 0.68f, 0.66f, 0.65f, 0.66f, 0.64f, 0.65f, 0.65f, 0.65f, 0.65f, 0.66f,
+                            0.66f, 0.66f, 0.66f, 0.66f, 0.66f, 0.67f, 0.66f, 0.65f, 0.64f, 0.63f, 0.63f, 0.63f,
+                            0.63f, 0.63f, 0.62f, 0.63f, 0.63f, 0.64f, 0.63f, 0.63f, 0.63f, 0.62f, 0.63f};
+const float kNormBG2[43] = {0.0f,  0.0f,  0.0f,  0.0f,  0.86f, 0.84f, 0.80f, 0.77f, 0.75f, 0.75f, 0.74f,
+                            0.74f, 0.74f, 0.73f, 0.73f, 0.73f, 0.73f, 0.72f, 0.70f, 0.71f, 0.71f, 0.71f,
+                            0.71f, 0.70f, 0.69f, 0.70f, 0.70f, 0.70f, 0.70f, 0.70f, 0.70f, 0.70f, 0.70f,
+                            0.68f, 0.67f, 0.67f, 0.68f, 0.69f, 0.69f, 0.69f, 0.69f, 0.69f, 0.69f};
+
+// IEEE binary16 bits of a float, round to nearest even.
+uint16_t float_to_half_bits(float f)
+{
+    const _Float16 h = (_Float16)f;
+    uint16_t b;
+    std::memcpy(&b, &h, 2);
+    return b;
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+const char* CUPHYWINAPI cuphyGetErrorName(cuphyStatus_t status)
+{
+    switch (status) {
+    case CUPHY_STATUS_SUCCESS: return "CUPHY_STATUS_SUCCESS";
+    case CUPHY_STATUS_INTERNAL_ERROR: return "CUPHY_STATUS_INTERNAL_ERROR";
+    case CUPHY_STATUS_NOT_SUPPORTED: return "CUPHY_STATUS_NOT_SUPPORTED";
+    case CUPHY_STATUS_INVALID_ARGUMENT: return "CUPHY_STATUS_INVALID_ARGUMENT";
+    case CUPHY_STATUS_ARCH_MISMATCH: return "CUPHY_STATUS_ARCH_MISMATCH";
+    case CUPHY_STATUS_ALLOC_FAILED: return "CUPHY_STATUS_ALLOC_FAILED";
+    case CUPHY_STATUS_SIZE_MISMATCH: return "CUPHY_STATUS_SIZE_MISMATCH";
+    case CUPHY_STATUS_MEMCPY_ERROR: return "CUPHY_STATUS_MEMCPY_ERROR";
+    case CUPHY_STATUS_INVALID_CONVERSION: return "CUPHY_STATUS_INVALID_CONVERSION";
+    case CUPHY_STATUS_UNSUPPORTED_TYPE: return "CUPHY_STATUS_UNSUPPORTED_TYPE";
+    case CUPHY_STATUS_UNSUPPORTED_LAYOUT: return "CUPHY_STATUS_UNSUPPORTED_LAYOUT";
+    case CUPHY_STATUS_UNSUPPORTED_RANK: return "CUPHY_STATUS_UNSUPPORTED_RANK";
+    case CUPHY_STATUS_UNSUPPORTED_CONFIG: return "CUPHY_STATUS_UNSUPPORTED_CONFIG";
+    case CUPHY_STATUS_UNSUPPORTED_ALIGNMENT: return "CUPHY_STATUS_UNSUPPORTED_ALIGNMENT";
+    case CUPHY_STATUS_VALUE_OUT_OF_RANGE: return "CUPHY_STATUS_VALUE_OUT_OF_RANGE";
+    case CUPHY_STATUS_REF_MISMATCH: return "CUPHY_STATUS_REF_MISMATCH";
+    default: return "CUPHY_UNKNOWN_STATUS";
+    }
+}
+
+const char* CUPHYWINAPI cuphyGetErrorString(cuphyStatus_t status)
+{
+    switch (status) {
+    case CUPHY_STATUS_SUCCESS: return "Success";
+    case CUPHY_STATUS_INTERNAL_ERROR: return "Internal error";
+    case CUPHY_STATUS_NOT_SUPPORTED: return "An operation was requested that is not currently supported";
+    case CUPHY_STATUS_INVALID_ARGUMENT: return "An invalid argument was provided";
+    case CUPHY_STATUS_ARCH_MISMATCH: return "Requested operation not supported on current architecture";
+    case CUPHY_STATUS_ALLOC_FAILED: return "Memory allocation failed";
+    case CUPHY_STATUS_SIZE_MISMATCH: return "The size of the operands provided to the function do not match";
+    case CUPHY_STATUS_MEMCPY_ERROR: return "An error occurred during a memcpy operation";
+    case CUPHY_STATUS_INVALID_CONVERSION: return "An invalid conversion operation was requested";
+    case CUPHY_STATUS_UNSUPPORTED_TYPE: return "An operation was requested on an unsupported type";
+    case CUPHY_STATUS_UNSUPPORTED_LAYOUT: return "An operation was requested on an unsupported layout";
+    case CUPHY_STATUS_UNSUPPORTED_RANK: return "An operation was requested on an unsupported rank";
+    case CUPHY_STATUS_UNSUPPORTED_CONFIG: return "An operation was requested on an unsupported configuration";
+    case CUPHY_STATUS_UNSUPPORTED_ALIGNMENT: return "One or more API arguments don't have the required alignment";
+    case CUPHY_STATUS_VALUE_OUT_OF_RANGE: return "Data conversion could not occur because an input value was out of range";
+    case CUPHY_STATUS_REF_MISMATCH: return "Mismatch found when comparing to reference";
+    default: return "Unknown status value";
+    }
+}
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+cuphyStatus_t CUPHYWINAPI cuphyCreateContext(cuphyContext_t* pcontext, unsigned int flags)
+{
+    (void)flags;
+    if (!pcontext) return CUPHY_STATUS_INVALID_ARGUMENT;
+    *pcontext = nullptr;
+    cuphyContext* c = new (std::nothrow) cuphyContext;
+    if (!c) return CUPHY_STATUS_ALLOC_FAILED;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        delete c;
+        return CUPHY_STATUS_INTERNAL_ERROR;
+    }
+    c->device = dev;
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess) c->cu_count = v;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess) c->lds_per_block = (size_t)v;
+    *pcontext = c;
+    return CUPHY_STATUS_SUCCESS;
+}
+
+cuphyStatus_t CUPHYWINAPI cuphyDestroyContext(cuphyContext_t ctx)
+{
+    if (!ctx) return CUPHY_STATUS_INVALID_ARGUMENT;
+    delete ctx;
+    return CUPHY_STATUS_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
+// tensor descriptors
+// ---------------------------------------------------------------------------
+cuphyStatus_t CUPHYWINAPI cuphyCreateTensorDescriptor(cuphyTensorDescriptor_t* ptensorDesc)
+{
+    if (!ptensorDesc) return CUPHY_STATUS_INVALID_ARGUMENT;
+    *ptensorDesc = new (std::nothrow) cuphyTensorDescriptor;
+    return *ptensorDesc ? CUPHY_STATUS_SUCCESS : CUPHY_STATUS_ALLOC_FAILED;
+}
+
+cuphyStatus_t CUPHYWINAPI cuphyDestroyTensorDescriptor(cuphyTensorDescriptor_t tensorDesc)
+{
+    if (!tensorDesc) return CUPHY_STATUS_INVALID_ARGUMENT;
+    delete tensorDesc;
+    return CUPHY_STATUS_SUCCESS;
+}
+
+// Validation as tensor_layout_any::validate (tensor_desc.cpp:150-213);
+// default strides tight, CUPHY_BIT first dimension padded to whole words,
+// COALESCE pads the second-dimension stride to a multiple of 128 bytes.
+cuphyStatus_t CUPHYWINAPI cuphySetTensorDescriptor(cuphyTensorDescriptor_t tensorDesc, cuphyDataType_t type,
+                                                   int numDimensions, const int dimensions[], const int strides[],
+                                                   unsigned int flags)
+{
+    if (!tensorDesc || !dimensions || !valid_type(type)) return CUPHY_STATUS_INVALID_ARGUMENT;
+    if (numDimensions <= 0 || numDimensions > CUPHY_DIM_MAX) return CUPHY_STATUS_INVALID_ARGUMENT;
+    for (int i = 0; i < numDimensions; ++i)
+        if (dimensions[i] < 1) return CUPHY_STATUS_INVALID_ARGUMENT;
+    int st[CUPHY_DIM_MAX] = {0};
+    const bool use_given = strides && !(flags & (CUPHY_TENSOR_ALIGN_TIGHT | CUPHY_TENSOR_ALIGN_COALESCE));
+    if (use_given) {
+        for (int i = 0; i < numDimensions; ++i) st[i] = strides[i];
+        if (type == CUPHY_BIT && (st[0] != 1 || (numDimensions > 1 && st[1] % 32))) return CUPHY_STATUS_INVALID_ARGUMENT;
+        // non-overlap: sorted by stride, each stride covers the previous extent
+        std::array<int, CUPHY_DIM_MAX> idx{};
+        for (int i = 0; i < numDimensions; ++i) idx[i] = i;
+        std::sort(idx.begin(), idx.begin() + numDimensions, [&](int a, int b) {
+            return st[a] != st[b] ? st[a] < st[b] : dimensions[a] < dimensions[b];
+        });
+        for (int i = 1; i < numDimensions; ++i)
+            if (dimensions[idx[i]] > 1 && (long long)st[idx[i]] < (long long)st[idx[i - 1]] * dimensions[idx[i - 1]])
+                return CUPHY_STATUS_INVALID_ARGUMENT;
+    } else {
+        st[0] = 1;
+        for (int i = 1; i < numDimensions; ++i) {
+            int prev = (i == 1 && type == CUPHY_BIT) ? ((dimensions[0] + 31) / 32) * 32 : dimensions[i - 1];
+            if (i == 1 && (flags & CUPHY_TENSOR_ALIGN_COALESCE) && type != CUPHY_BIT) {
+                const int mult = (int)(128 / type_bytes(type));
+                prev = (prev + mult - 1) / mult * mult;
+            }
+            st[i] = st[i - 1] * prev;
+            if (i > 1) st[i] = st[i - 1] * dimensions[i - 1];
+        }
+    }
+    tensorDesc->type = type;
+    tensorDesc->rank = numDimensions;
+    for (int i = 0; i < CUPHY_DIM_MAX; ++i) {
+        tensorDesc->dims[i] = i < numDimensions ? dimensions[i] : 1;
+        tensorDesc->strides[i] = i < numDimensions ? st[i] : 0;
+    }
+    return CUPHY_STATUS_SUCCESS;
+}
+
+cuphyStatus_t CUPHYWINAPI cuphyGetTensorDescriptor(cuphyTensorDescriptor_t tensorDesc, int numDimsRequested,
+                                                   cuphyDataType_t* dataType, int* numDims, int dimensions[],
+                                                   int strides[])
+{
+    if (!tensorDesc || (numDimsRequested > 0 && !dimensions)) return CUPHY_STATUS_INVALID_ARGUMENT;
+    if (dataType) *dataType = tensorDesc->type;
+    if (numDims) *numDims = tensorDesc->rank;
+    for (int i = 0; i < numDimsRequested && i < CUPHY_DIM_MAX; ++i) {
+        dimensions[i] = tensorDesc->dims[i];
+        if (strides) strides[i] = tensorDesc->strides[i];
+    }
+    return CUPHY_STATUS_SUCCESS;
+}
+
+cuphyStatus_t CUPHYWINAPI cuphyGetTensorSizeInBytes(cuphyTensorDescriptor_t tensorDesc, size_t* psz)
+{
+    if (!tensorDesc || !psz) return CUPHY_STATUS_INVALID_ARGUMENT;
+    size_t last = 0;
+    for (int i = 0; i < tensorDesc->rank; ++i)
+        last = std::max(last, (size_t)tensorDesc->strides[i] * (size_t)tensorDesc->dims[i]);
+    *psz = (tensorDesc->type == CUPHY_BIT) ? (last + 31) / 32 * 4 : last * type_bytes(tensorDesc->type);
+    return CUPHY_STATUS_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
+// LDPC decoder
+// ---------------------------------------------------------------------------
+cuphyStatus_t CUPHYWINAPI cuphyCreateLDPCDecoder(cuphyContext_t context, cuphyLDPCDecoder_t* pdecoder,
+                                                 unsigned int flags)
+{
+    (void)flags;
+    if (!pdecoder || !context) return CUPHY_STATUS_INVALID_ARGUMENT;
+    *pdecoder = nullptr;
+    cuphyLDPCDecoder* d = new (std::nothrow) cuphyLDPCDecoder;
+    if (!d) return CUPHY_STATUS_ALLOC_FAILED;
+    d->device = context->device;
+    *pdecoder = d;
+    return CUPHY_STATUS_SUCCESS;
+}
+
+cuphyStatus_t CUPHYWINAPI cuphyDestroyLDPCDecoder(cuphyLDPCDecoder_t decoder)
+{
+    if (!decoder) return CUPHY_STATUS_INVALID_ARGUMENT;
+    delete decoder;
+    return CUPHY_STATUS_SUCCESS;
+}
+
+// Tensor interface: LLR tensor [nLLR x nCW] (stride[1] = per-codeword
+// stride in elements), output CUPHY_BIT tensor [K x nCW] (stride[1] in
+// bits, multiple of 32), optional binary16 soft-output tensor.  Forwarded
+// to the transport-block kernels as a one-TB descriptor.
+cuphyStatus_t CUPHYWINAPI cuphyErrorCorrectionLDPCDecode(cuphyLDPCDecoder_t decoder, cuphyTensorDescriptor_t tensorDescDst,
+                                                         void* dstAddr, cuphyTensorDescriptor_t tensorDescLLR,
+                                                         const void* LLRAddr, cuphyTensorDescriptor_t tensorDescSoftOutputs,
+                                                         void* softOutputsAddr, const cuphyLDPCDecodeConfigDesc_t* config,
+                                                         hipStream_t strm)
+{
+    // cuphy_ldpc.cpp:36-55
+    if (!decoder || !tensorDescDst || !dstAddr || !tensorDescLLR || !LLRAddr || !config ||
+        config->max_iterations < 0 || config->BG < 1 || config->BG > 2 ||
+        (config->BG == 1 ? config->Kb != 22
+                         : (config->Kb != 6 && config->Kb != 8 && config->Kb != 9 && config->Kb != 10)) ||
+        config->Z < 2 || config->Z > 384 || config->num_parity_nodes < 4 ||
+        (config->BG == 1 ? config->num_parity_nodes > 46 : config->num_parity_nodes > 42) ||
+        ((tensorDescSoftOutputs == nullptr) != (softOutputsAddr == nullptr)))
+        return CUPHY_STATUS_INVALID_ARGUMENT;
+    if (config->llr_type != tensorDescLLR->type) return CUPHY_STATUS_INVALID_ARGUMENT;  // :63-66
+    if (tensorDescSoftOutputs) {                                                        // :69-91
+        if (tensorDescSoftOutputs->type != CUPHY_R_16F) return CUPHY_STATUS_INVALID_ARGUMENT;
+        if (reinterpret_cast<uintptr_t>(softOutputsAddr) % 4) return CUPHY_STATUS_INVALID_ARGUMENT;
+        if (tensorDescSoftOutputs->rank > 1 && tensorDescSoftOutputs->dims[1] > 1 &&
+            tensorDescSoftOutputs->strides[1] % 2)
+            return CUPHY_STATUS_INVALID_ARGUMENT;
+    }
+    // ldpc.cpp:733-745
+    if (tensorDescLLR->rank > 2) return CUPHY_STATUS_UNSUPPORTED_RANK;
+    if (tensorDescDst->type != CUPHY_BIT || tensorDescDst->rank > 2) return CUPHY_STATUS_UNSUPPORTED_TYPE;
+    if (!nrldpc::config_supported(*config)) return CUPHY_STATUS_UNSUPPORTED_CONFIG;
+
+    cuphyLDPCDecodeDesc_t d;
+    std::memset(&d, 0, sizeof(d));
+    d.config = *config;
+    d.num_tbs = 1;
+    const int ncw = tensorDescLLR->rank > 1 ? tensorDescLLR->dims[1] : 1;
+    d.llr_input[0].addr = const_cast<void*>(LLRAddr);
+    d.llr_input[0].stride_elements = tensorDescLLR->rank > 1 ? tensorDescLLR->strides[1] : tensorDescLLR->dims[0];
+    d.llr_input[0].num_codewords = ncw;
+    d.tb_output[0].addr = static_cast<uint32_t*>(dstAddr);
+    d.tb_output[0].stride_words =
+        tensorDescDst->rank > 1 ? tensorDescDst->strides[1] / 32 : (tensorDescDst->dims[0] + 31) / 32;
+    d.tb_output[0].num_codewords = ncw;
+    if (softOutputsAddr) {
+        d.config.flags |= CUPHY_LDPC_DECODE_WRITE_SOFT_OUTPUTS;
+        d.llr_output[0].addr = softOutputsAddr;
+        d.llr_output[0].stride_elements =
+            tensorDescSoftOutputs->rank > 1 ? tensorDescSoftOutputs->strides[1] : tensorDescSoftOutputs->dims[0];
+        d.llr_output[0].num_codewords = ncw;
+    } else {
+        d.config.flags &= ~(uint32_t)CUPHY_LDPC_DECODE_WRITE_SOFT_OUTPUTS;
+    }
+    return nrldpc::decode_tb(d, strm);
+}
+
+cuphyStatus_t CUPHYWINAPI cuphyErrorCorrectionLDPCTransportBlockDecode(cuphyLDPCDecoder_t decoder,
+                                                                       const cuphyLDPCDecodeDesc_t* decodeDesc,
+                                                                       hipStream_t strm)
+{
+    if (!decoder || !decodeDesc) return CUPHY_STATUS_INVALID_ARGUMENT;  // cuphy_ldpc.cpp:105-109
+    const cuphyLDPCDecodeConfigDesc_t& c = decodeDesc->config;
+    if (c.max_iterations < 0 || c.BG < 1 || c.BG > 2 || c.Z < 2 || c.Z > 384 || c.num_parity_nodes < 4 ||
+        (c.BG == 1 ? c.num_parity_nodes > 46 : c.num_parity_nodes > 42) || decodeDesc->num_tbs < 0 ||
+        decodeDesc->num_tbs > CUPHY_LDPC_DECODE_DESC_MAX_TB)
+        return CUPHY_STATUS_INVALID_ARGUMENT;
+    for (int i = 0; i < decodeDesc->num_tbs; ++i)
+        if (decodeDesc->llr_input[i].num_codewords < 0 ||
+            (decodeDesc->llr_input[i].num_codewords > 0 &&
+             (!decodeDesc->llr_input[i].addr || !decodeDesc->tb_output[i].addr)))
+            return CUPHY_STATUS_INVALID_ARGUMENT;
+    if (decodeDesc->num_tbs == 0) return CUPHY_STATUS_SUCCESS;
+    if (!nrldpc::config_supported(c)) return CUPHY_STATUS_UNSUPPORTED_CONFIG;
+    return nrldpc::decode_tb(*decodeDesc, strm);
+}
+
+// cuphy_ldpc.cpp:116-155: the kernels need no workspace.
+cuphyStatus_t CUPHYWINAPI cuphyErrorCorrectionLDPCDecodeGetWorkspaceSize(cuphyLDPCDecoder_t decoder,
+                                                                         const cuphyLDPCDecodeConfigDesc_t* config,
+                                                                         int numCodeWords, size_t* sizeInBytes)
+{
+    if (!decoder || !config || (config->BG != 1 && config->BG != 2) ||
+        (config->Kb != 22 && config->Kb != 10 && config->Kb != 9 && config->Kb != 8 && config->Kb != 6) ||
+        !nrldpc::is_lifting_size(config->Z) || numCodeWords <= 0 || !sizeInBytes ||
+        config->num_parity_nodes < 4 || (config->BG == 1 ? config->num_parity_nodes > 46 : config->num_parity_nodes > 42))
+        return CUPHY_STATUS_INVALID_ARGUMENT;
+    if (config->llr_type != CUPHY_R_32F && config->llr_type != CUPHY_R_16F) return CUPHY_STATUS_UNSUPPORTED_CONFIG;
+    *sizeInBytes = 0;
+    return CUPHY_STATUS_SUCCESS;
+}
+
+// ldpc.cpp:851-879
+cuphyStatus_t CUPHYWINAPI cuphyErrorCorrectionLDPCDecodeSetNormalization(cuphyLDPCDecoder_t decoder,
+                                                                         cuphyLDPCDecodeConfigDesc_t* decodeDesc)
+{
+    if (!decoder || !decodeDesc) return CUPHY_STATUS_INVALID_ARGUMENT;
+    cuphyLDPCDecodeConfigDesc_t& c = *decodeDesc;
+    if ((c.llr_type != CUPHY_R_16F && c.llr_type != CUPHY_R_32F) || c.num_parity_nodes < 4 ||
+        (c.BG == 1 && c.num_parity_nodes > 46) || (c.BG == 2 && c.num_parity_nodes > 42))
+        return CUPHY_STATUS_INVALID_ARGUMENT;
+    const float norm = (c.BG == 1) ? kNormBG1[c.num_parity_nodes] : kNormBG2[c.num_parity_nodes];
+    if (c.llr_type == CUPHY_R_16F) {
+        const uint32_t h = float_to_half_bits(norm);
+        c.norm.f16x2 = h | (h << 16);
+    } else {
+        c.norm.f32 = norm;
+    }
+    return CUPHY_STATUS_SUCCESS;
+}
+
+cuphyStatus_t CUPHYWINAPI cuphyErrorCorrectionLDPCDecodeGetLaunchDescriptor(cuphyLDPCDecoder_t decoder,
+                                                                            cuphyLDPCDecodeLaunchConfig_t* launchConfig)
+{
+    if (!decoder || !launchConfig) return CUPHY_STATUS_INVALID_ARGUMENT;
+    return nrldpc::get_launch_config(*launchConfig);
+}
+
+}  // extern "C"

@@ -1,0 +1,31 @@
+// Host-side interface of the HIP layered LDPC decoder (ldpc_decode.hip).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include "cuphy_hip.h"
+
+namespace nrldpc {
+
+struct launch_geometry {
+    void* func = nullptr;
+    dim3 grid, block;
+    size_t lds_bytes = 0;
+};
+
+// Z is one of the 51 TS 38.212 lifting sizes.
+bool is_lifting_size(int Z);
+
+// (BG, Z, mb, Kb, LLR type) is decodable by the kernels of this library.
+bool config_supported(const cuphyLDPCDecodeConfigDesc_t& c);
+
+// Kernel and launch shape for a transport-block descriptor.
+cuphyStatus_t fill_launch(const cuphyLDPCDecodeDesc_t& desc, launch_geometry& g);
+
+// Enqueue the decode of every code word of `desc` on `strm`.
+cuphyStatus_t decode_tb(const cuphyLDPCDecodeDesc_t& desc, hipStream_t strm);
+
+// Populate a kernel-node launch configuration (graph capture / driver launch).
+cuphyStatus_t get_launch_config(cuphyLDPCDecodeLaunchConfig_t& lc);
+
+}  // namespace nrldpc

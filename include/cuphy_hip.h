@@ -1,0 +1,337 @@
+/*
+ * cuphy_hip.h -- C ABI of the MI355X (gfx950) uplink LDPC decode path.
+ *
+ * Drop-in for the LDPC decoder, PUSCH de-rate-match and PUSCH CRC-decode
+ * sections of the reference's cuPHY C API.  Type names, field order, enum
+ * values and entry-point names are those of cuPHY/src/cuphy/cuphy.h so that
+ * a caller written against that header (PuschRx, the pyAerial bindings,
+ * cuphy_ex_ldpc) compiles against this one; the only change a caller sees is
+ * the stream type (hipStream_t instead of cudaStream_t) and the launch
+ * descriptor (hipKernelNodeParams instead of CUDA_KERNEL_NODE_PARAMS).
+ *
+ * Every entry point is asynchronous on the given stream, never synchronises
+ * the host, and returns a cuphyStatus_t.
+ */
+#ifndef CUPHY_HIP_H
+#define CUPHY_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CUPHYWINAPI
+
+/* ------------------------------------------------------------------------
+ * Status codes -- cuphy.h:346-363 (values identical)
+ * ---------------------------------------------------------------------- */
+typedef enum {
+    CUPHY_STATUS_SUCCESS = 0,
+    CUPHY_STATUS_INTERNAL_ERROR = 1,
+    CUPHY_STATUS_NOT_SUPPORTED = 2,
+    CUPHY_STATUS_INVALID_ARGUMENT = 3,
+    CUPHY_STATUS_ARCH_MISMATCH = 4,
+    CUPHY_STATUS_ALLOC_FAILED = 5,
+    CUPHY_STATUS_SIZE_MISMATCH = 6,
+    CUPHY_STATUS_MEMCPY_ERROR = 7,
+    CUPHY_STATUS_INVALID_CONVERSION = 8,
+    CUPHY_STATUS_UNSUPPORTED_TYPE = 9,
+    CUPHY_STATUS_UNSUPPORTED_LAYOUT = 10,
+    CUPHY_STATUS_UNSUPPORTED_RANK = 11,
+    CUPHY_STATUS_UNSUPPORTED_CONFIG = 12,
+    CUPHY_STATUS_UNSUPPORTED_ALIGNMENT = 13,
+    CUPHY_STATUS_VALUE_OUT_OF_RANGE = 14,
+    CUPHY_STATUS_REF_MISMATCH = 15,
+    CUPHY_N_STATUS_CONFIGS = 16
+} cuphyStatus_t;
+
+/* ------------------------------------------------------------------------
+ * Data types -- cuphy.h:960-983 (values of the CUDA/HIP R types kept)
+ * ---------------------------------------------------------------------- */
+typedef enum {
+    CUPHY_VOID = -1,
+    CUPHY_R_32F = 0,
+    CUPHY_R_64F = 1,
+    CUPHY_R_16F = 2,
+    CUPHY_R_8I = 3,
+    CUPHY_C_16F = 6,
+    CUPHY_R_8U = 8,
+    CUPHY_R_32I = 10,
+    CUPHY_R_32U = 12,
+    CUPHY_BIT = 20,
+    CUPHY_R_16I = 21,
+    CUPHY_R_16U = 23
+} cuphyDataType_t;
+
+/* cuphy.h:43-62,118,143-145 */
+#define MAX_DECODED_CODE_BLOCK_BIT_SIZE 8448
+#define MAX_ENCODED_CODE_BLOCK_BIT_SIZE 25344
+#define MAX_N_LAYERS_PUSCH 8
+#define MAX_N_TBS_PER_CELL_GROUP_SUPPORTED 256
+#define MAX_N_TBS_SUPPORTED (MAX_N_TBS_PER_CELL_GROUP_SUPPORTED)
+#define MAX_N_CBS_PER_TB_SUPPORTED 152
+#define CUPHY_LDPC_DECODE_DESC_MAX_TB (32)
+#define QAM_STRIDE 8
+#define CUPHY_MAX_N_CSI1_PRMS (4)
+#define CUPHY_MAX_N_CSI2_REPORTS_PER_UE (16)
+
+/* LDPC decoder flags -- cuphy.h:1128-1144.  EARLY_TERM is implemented
+ * here (syndrome check over all mb*Z parity equations after each
+ * iteration); in the reference it is documented as unsupported. */
+#define CUPHY_LDPC_DECODE_DEFAULT (0)
+#define CUPHY_LDPC_DECODE_EARLY_TERM (0x01)
+#define CUPHY_LDPC_DECODE_CHOOSE_THROUGHPUT (0x2)
+#define CUPHY_LDPC_DECODE_WRITE_SOFT_OUTPUTS (0x04)
+
+/* cuphySetTensorDescriptor() flags -- cuphy.h:1091-1097 */
+#define CUPHY_TENSOR_N_DIM_1 (1)
+#define CUPHY_TENSOR_N_DIM_2 (2)
+#define CUPHY_TENSOR_N_DIM_3 (3)
+#define CUPHY_TENSOR_N_DIM_4 (4)
+#define CUPHY_TENSOR_N_DIM_5 (5)
+#define CUPHY_DIM_MAX CUPHY_TENSOR_N_DIM_5
+#define CUPHY_TENSOR_ALIGN_DEFAULT 0x00
+#define CUPHY_TENSOR_ALIGN_TIGHT 0x01
+#define CUPHY_TENSOR_ALIGN_COALESCE 0x02
+
+/* ------------------------------------------------------------------------
+ * Context and errors -- cuphy.h:1056-1075, 1156-1208
+ * ---------------------------------------------------------------------- */
+struct cuphyContext;
+typedef struct cuphyContext* cuphyContext_t;
+
+const char* CUPHYWINAPI cuphyGetErrorString(cuphyStatus_t status);   /* cuphy.h:1060 */
+const char* CUPHYWINAPI cuphyGetErrorName(cuphyStatus_t status);     /* cuphy.h:1075 */
+cuphyStatus_t CUPHYWINAPI cuphyCreateContext(cuphyContext_t* pcontext, unsigned int flags); /* cuphy.h:1186 */
+cuphyStatus_t CUPHYWINAPI cuphyDestroyContext(cuphyContext_t ctx);   /* cuphy.h:1208 */
+
+/* ------------------------------------------------------------------------
+ * Tensor descriptors -- cuphy.h:1225-1470 (only the 2-D semantics that the
+ * LDPC tensor interface uses are meaningful)
+ * ---------------------------------------------------------------------- */
+struct cuphyTensorDescriptor;
+typedef struct cuphyTensorDescriptor* cuphyTensorDescriptor_t;
+
+typedef struct _cuphyTensorPrm {
+    cuphyTensorDescriptor_t desc;
+    void* pAddr;
+} cuphyTensorPrm_t;
+
+cuphyStatus_t CUPHYWINAPI cuphyCreateTensorDescriptor(cuphyTensorDescriptor_t* ptensorDesc);      /* cuphy.h:1318 */
+cuphyStatus_t CUPHYWINAPI cuphyDestroyTensorDescriptor(cuphyTensorDescriptor_t tensorDesc);       /* cuphy.h:1338 */
+cuphyStatus_t CUPHYWINAPI cuphySetTensorDescriptor(cuphyTensorDescriptor_t tensorDesc,             /* cuphy.h:1403 */
+                                                   cuphyDataType_t type, int numDimensions,
+                                                   const int dimensions[], const int strides[],
+                                                   unsigned int flags);
+cuphyStatus_t CUPHYWINAPI cuphyGetTensorDescriptor(cuphyTensorDescriptor_t tensorDesc,             /* cuphy.h:1432 */
+                                                   int numDimsRequested, cuphyDataType_t* dataType,
+                                                   int* numDims, int dimensions[], int strides[]);
+cuphyStatus_t CUPHYWINAPI cuphyGetTensorSizeInBytes(cuphyTensorDescriptor_t tensorDesc, size_t* psz); /* cuphy.h:1457 */
+
+/* ------------------------------------------------------------------------
+ * LDPC decoder -- cuphy.h:3858-4258
+ * ---------------------------------------------------------------------- */
+struct cuphyLDPCDecoder;
+typedef struct cuphyLDPCDecoder* cuphyLDPCDecoder_t;
+
+typedef struct {                 /* cuphy.h:3916-3923 */
+    void* addr;
+    int32_t stride_elements;
+    int32_t num_codewords;
+} cuphyTransportBlockLLRDesc_t;
+
+typedef struct {                 /* cuphy.h:3926-3933 */
+    uint32_t* addr;
+    int32_t stride_words;
+    int32_t num_codewords;
+} cuphyTransportBlockDataDesc_t;
+
+typedef union {                  /* cuphy.h:3936-3942; f16x2 = two IEEE binary16 bit patterns */
+    float f32;
+    uint32_t f16x2;
+} cuphyLDPCNormalization_t;
+
+typedef struct {                 /* cuphy.h:3945-3960 */
+    cuphyDataType_t llr_type;    /* CUPHY_R_16F or CUPHY_R_32F (CUPHY_R_8I: int8 LLRs, scale in clamp_value) */
+    int16_t num_parity_nodes;    /* mb */
+    int16_t Z;
+    int16_t max_iterations;
+    int16_t Kb;
+    cuphyLDPCNormalization_t norm;
+    uint32_t flags;
+    int16_t BG;
+    int16_t algo;                /* 0 = automatic */
+    float clamp_value;
+    void* workspace;
+} cuphyLDPCDecodeConfigDesc_t;
+
+typedef struct {                 /* cuphy.h:3963-3972 */
+    cuphyLDPCDecodeConfigDesc_t config;
+    int32_t num_tbs;
+    cuphyTransportBlockLLRDesc_t llr_input[CUPHY_LDPC_DECODE_DESC_MAX_TB];
+    cuphyTransportBlockDataDesc_t tb_output[CUPHY_LDPC_DECODE_DESC_MAX_TB];
+    cuphyTransportBlockLLRDesc_t llr_output[CUPHY_LDPC_DECODE_DESC_MAX_TB];
+} cuphyLDPCDecodeDesc_t;
+
+typedef struct {                 /* cuphy.h:3975-3982; hipKernelNodeParams replaces CUDA_KERNEL_NODE_PARAMS */
+    hipKernelNodeParams kernel_node_params_driver;
+    void* kernel_args[2];
+    cuphyLDPCDecodeDesc_t decode_desc;
+} cuphyLDPCDecodeLaunchConfig_t;
+
+cuphyStatus_t CUPHYWINAPI cuphyCreateLDPCDecoder(cuphyContext_t context, cuphyLDPCDecoder_t* pdecoder, /* cuphy.h:3889 */
+                                                 unsigned int flags);
+cuphyStatus_t CUPHYWINAPI cuphyDestroyLDPCDecoder(cuphyLDPCDecoder_t decoder);                         /* cuphy.h:3913 */
+
+cuphyStatus_t CUPHYWINAPI cuphyErrorCorrectionLDPCDecode(cuphyLDPCDecoder_t decoder,                   /* cuphy.h:4077 */
+                                                         cuphyTensorDescriptor_t tensorDescDst, void* dstAddr,
+                                                         cuphyTensorDescriptor_t tensorDescLLR, const void* LLRAddr,
+                                                         cuphyTensorDescriptor_t tensorDescSoftOutputs,
+                                                         void* softOutputsAddr,
+                                                         const cuphyLDPCDecodeConfigDesc_t* config,
+                                                         hipStream_t strm);
+
+cuphyStatus_t CUPHYWINAPI cuphyErrorCorrectionLDPCTransportBlockDecode(cuphyLDPCDecoder_t decoder,     /* cuphy.h:4147 */
+                                                                       const cuphyLDPCDecodeDesc_t* decodeDesc,
+                                                                       hipStream_t strm);
+
+cuphyStatus_t CUPHYWINAPI cuphyErrorCorrectionLDPCDecodeGetWorkspaceSize(cuphyLDPCDecoder_t decoder,  /* cuphy.h:4193 */
+                                                                         const cuphyLDPCDecodeConfigDesc_t* config,
+                                                                         int numCodeWords, size_t* sizeInBytes);
+
+cuphyStatus_t CUPHYWINAPI cuphyErrorCorrectionLDPCDecodeSetNormalization(cuphyLDPCDecoder_t decoder,  /* cuphy.h:4228 */
+                                                                         cuphyLDPCDecodeConfigDesc_t* decodeDesc);
+
+cuphyStatus_t CUPHYWINAPI cuphyErrorCorrectionLDPCDecodeGetLaunchDescriptor(cuphyLDPCDecoder_t decoder, /* cuphy.h:4257 */
+                                                                            cuphyLDPCDecodeLaunchConfig_t* launchConfig);
+
+/* ------------------------------------------------------------------------
+ * Per-TB parameters -- cuphy.h:3506-3575 (layout identical; the three
+ * __half* members are declared void*)
+ * ---------------------------------------------------------------------- */
+typedef struct _cuphyCalcCsi2SizePrm {   /* cuphy.h:811-818 */
+    uint8_t nPart1Prms;
+    uint8_t prmSizes[CUPHY_MAX_N_CSI1_PRMS];
+    uint16_t prmOffsets[CUPHY_MAX_N_CSI1_PRMS];
+    uint16_t csi2sizeMapIdx;
+} cuphyCalcCsi2SizePrm_t;
+
+typedef struct PerTbParams {
+    uint32_t ndi;
+    uint32_t rv;
+    uint32_t Qm;
+    uint32_t bg;
+    uint32_t Nl;
+    uint32_t num_CBs;
+    uint32_t Zc;
+    uint32_t N;
+    uint32_t Ncb;
+    uint32_t Ncb_padded;
+    uint32_t G;
+    uint32_t K;
+    uint32_t F;
+    uint32_t cinit;
+    uint32_t nDataBytes;
+    uint32_t nZpBitsPerCb;
+    uint32_t k0;
+    uint8_t ldpcEarlyTerminationPerUe;
+    uint8_t ldpcMaxNumItrPerUe;
+    uint32_t firstCodeBlockIndex;
+    uint32_t encodedSize;
+    uint32_t layer_map_array[MAX_N_LAYERS_PUSCH];
+    uint32_t userGroupIndex;
+    uint32_t nBBULayers;
+    uint32_t startLLR;
+    uint32_t mScUciSum;
+    uint32_t codedBitsSum;
+    uint8_t isDataPresent;
+    uint8_t betaOffsetCsi2;
+    uint32_t qPrimeAck;
+    float codeRate;
+    float alpha;
+    uint32_t qPrimeCsi1;
+    uint8_t isEarlyHarq;
+    uint8_t uciOnPuschFlag;
+    uint8_t csi2Flag;
+    uint32_t G_schAndCsi2;
+    uint32_t G_harq;
+    uint32_t G_csi1;
+    uint32_t G_csi2;
+    uint32_t G_harq_rvd;
+    uint32_t nBitsHarq;
+    uint16_t nBitsCsi2;
+    uint8_t nCsiReports;
+    uint8_t rankBitOffset;
+    uint8_t nRanksBits;
+    void* d_schAndCsi2LLRs;
+    void* d_csi1LLRs;
+    void* d_harqLLrs;
+    uint32_t tbSize;
+    uint16_t nCsi2Reports;
+    cuphyCalcCsi2SizePrm_t calcCsi2SizePrms[CUPHY_MAX_N_CSI2_REPORTS_PER_UE];
+    uint8_t enableTfPrcd;
+    uint8_t nDmrsCdmGrpsNoData;
+    uint32_t* debug_d_derateCbsIndices;
+} PerTbParams;
+
+/* ------------------------------------------------------------------------
+ * PUSCH de-rate-matching + HARQ combining -- cuphy.h:3349-3490
+ * ---------------------------------------------------------------------- */
+struct cuphyPuschRxRateMatch { uint8_t reserved; };
+typedef struct cuphyPuschRxRateMatch* cuphyPuschRxRateMatchHndl_t;
+
+typedef struct {                 /* cuphy.h:3359-3366 */
+    hipKernelNodeParams kernelNodeParamsDriver;       /* main de-rate-match kernel      */
+    hipKernelNodeParams resetKernelNodeParamsDriver;  /* unused here (no atomics): gridDim 0 */
+    hipKernelNodeParams clampKernelNodeParamsDriver;  /* unused here (no atomics): gridDim 0 */
+    void* desc;
+    void* kernelArgs[1];
+} cuphyPuschRxRateMatchLaunchCfg_t;
+
+cuphyStatus_t CUPHYWINAPI cuphyPuschRxRateMatchGetDescrInfo(size_t* pDescrSizeBytes, size_t* pDescrAlignBytes); /* cuphy.h:3392 */
+cuphyStatus_t CUPHYWINAPI cuphyCreatePuschRxRateMatch(cuphyPuschRxRateMatchHndl_t* pPuschRxRateMatchHndl,      /* cuphy.h:3419 */
+                                                      int FPconfig, int descramblingOn);
+cuphyStatus_t CUPHYWINAPI cuphySetupPuschRxRateMatch(cuphyPuschRxRateMatchHndl_t puschRxRateMatchHndl,         /* cuphy.h:3460 */
+                                                     uint16_t nSchUes, uint16_t* pSchUserIdxsCpu,
+                                                     const PerTbParams* pTbPrmsCpu, const PerTbParams* pTbPrmsGpu,
+                                                     cuphyTensorPrm_t* pTPrmRmIn, cuphyTensorPrm_t* pTPrmCdm1RmIn,
+                                                     void** ppRmOut, void* pCpuDesc, void* pGpuDesc,
+                                                     uint8_t enableCpuToGpuDescrAsyncCpy,
+                                                     cuphyPuschRxRateMatchLaunchCfg_t* pLaunchCfg, hipStream_t strm);
+cuphyStatus_t CUPHYWINAPI cuphyDestroyPuschRxRateMatch(cuphyPuschRxRateMatchHndl_t puschRxRateMatchHndl);     /* cuphy.h:3490 */
+
+/* ------------------------------------------------------------------------
+ * PUSCH CB/TB CRC decode -- cuphy.h:3154-3293
+ * ---------------------------------------------------------------------- */
+struct cuphyPuschRxCrcDecode { uint8_t reserved; };
+typedef struct cuphyPuschRxCrcDecode* cuphyPuschRxCrcDecodeHndl_t;
+
+typedef struct {                 /* cuphy.h:3168-3173 */
+    hipKernelNodeParams kernelNodeParamsDriver;
+    void* desc;
+    void* kernelArgs[1];
+} cuphyPuschRxCrcDecodeLaunchCfg_t;
+
+cuphyStatus_t CUPHYWINAPI cuphyPuschRxCrcDecodeGetDescrInfo(size_t* pDescrSizeBytes, size_t* pDescrAlignBytes); /* cuphy.h:3195 */
+cuphyStatus_t CUPHYWINAPI cuphyCreatePuschRxCrcDecode(cuphyPuschRxCrcDecodeHndl_t* puschRxCrcDecodeHndl,        /* cuphy.h:3217 */
+                                                      int reverseBytes);
+cuphyStatus_t CUPHYWINAPI cuphySetupPuschRxCrcDecode(cuphyPuschRxCrcDecodeHndl_t puschRxCrcDecodeHndl,          /* cuphy.h:3254 */
+                                                     uint16_t nSchUes, uint16_t* pSchUserIdxsCpu,
+                                                     uint32_t* pOutputCBCRCs, uint8_t* pOutputTBs,
+                                                     const uint32_t* pInputCodeBlocks, uint32_t* pOutputTBCRCs,
+                                                     const PerTbParams* pTbPrmsCpu, const PerTbParams* pTbPrmsGpu,
+                                                     void* pCpuDesc, void* pGpuDesc,
+                                                     uint8_t enableCpuToGpuDescrAsyncCpy,
+                                                     cuphyPuschRxCrcDecodeLaunchCfg_t* pCbCrcLaunchCfg,
+                                                     cuphyPuschRxCrcDecodeLaunchCfg_t* pTbCrcLaunchCfg,
+                                                     hipStream_t strm);
+cuphyStatus_t CUPHYWINAPI cuphyDestroyPuschRxCrcDecode(cuphyPuschRxCrcDecodeHndl_t puschRxCrcDecodeHndl);        /* cuphy.h:3293 */
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CUPHY_HIP_H */

@@ -1,0 +1,583 @@
+/*
+ * CPU oracle for the 5G NR uplink LDPC decode path -- TEST INFRASTRUCTURE.
+ * Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg load
+ * this library; the product never does.  See nr_oracle.h for the reference
+ * files each function restates.
+ *
+ * Signed zero: the decoder canonicalises -0 inputs to +0.  In MATLAB
+ * (compute_V2C.m:46-49, update_APP.m:40) the first update of a column
+ * computes |v2c|*sgn, which maps a -0 input to +0 as well, so no result of
+ * MSA_layering.m changes; doing it at load keeps the GPU kernel free of a
+ * per-edge fix-up.
+ *
+ * Build: see Makefile (gcc -O2 -ffp-contract=off, no fast-math: every
+ * float operation is one IEEE-754 RN operation).
+ */
+#include "nr_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../aerial-cuda-accelerated-ran_amd/csrc/nr_bg_tables.h"
+
+/* ======================================================================
+ * lifting sizes, TS 38.212 Table 5.3.2-1
+ * ==================================================================== */
+int nr_orc_ils(int Z)
+{
+    static const int a[8] = {2, 3, 5, 7, 9, 11, 13, 15};
+    for (int i = 0; i < 8; ++i)
+        for (int j = 0; a[i] << j <= 384; ++j)
+            if ((a[i] << j) == Z) return i;
+    return -1;
+}
+
+int nr_orc_num_rows(int bg) { return bg == 1 ? NR_BG1_NUM_ROWS : NR_BG2_NUM_ROWS; }
+int nr_orc_num_sys_cols(int bg) { return bg == 1 ? 22 : 10; }
+
+int nr_orc_row_degree(int bg, int r)
+{
+    const uint16_t* rs = (bg == 1) ? nr_bg1_row_start : nr_bg2_row_start;
+    return rs[r + 1] - rs[r];
+}
+
+/* load_Tanner.m:93-98: shift reduced mod Zc */
+int nr_orc_edge(int bg, int Z, int r, int j, int* col, int* shift)
+{
+    const int ils = nr_orc_ils(Z);
+    if (ils < 0) return -1;
+    const uint16_t* rs = (bg == 1) ? nr_bg1_row_start : nr_bg2_row_start;
+    const int e = rs[r] + j;
+    if (bg == 1) {
+        *col = nr_bg1_col[e];
+        *shift = nr_bg1_shift[ils][e] % Z;
+    } else {
+        *col = nr_bg2_col[e];
+        *shift = nr_bg2_shift[ils][e] % Z;
+    }
+    return 0;
+}
+
+/* ======================================================================
+ * random numbers: splitmix64 seeding + xoshiro256** (public-domain
+ * algorithms by Vigna / Blackman); Box-Muller for normals.
+ * ==================================================================== */
+static uint64_t splitmix64(uint64_t* x)
+{
+    uint64_t z = (*x += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+void nr_orc_rng_seed(nr_orc_rng* g, uint64_t seed)
+{
+    uint64_t x = seed;
+    for (int i = 0; i < 4; ++i) g->s[i] = splitmix64(&x);
+}
+
+static inline uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+
+uint64_t nr_orc_rng_next(nr_orc_rng* g)
+{
+    uint64_t* s = g->s;
+    const uint64_t result = rotl(s[1] * 5, 7) * 9;
+    const uint64_t t = s[1] << 17;
+    s[2] ^= s[0];
+    s[3] ^= s[1];
+    s[1] ^= s[2];
+    s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = rotl(s[3], 45);
+    return result;
+}
+
+double nr_orc_rng_uniform(nr_orc_rng* g)
+{
+    return (double)((nr_orc_rng_next(g) >> 11) + 1) * (1.0 / 9007199254740992.0);
+}
+
+double nr_orc_rng_normal(nr_orc_rng* g)
+{
+    const double u1 = nr_orc_rng_uniform(g);
+    const double u2 = nr_orc_rng_uniform(g);
+    return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+void nr_orc_random_bits(uint64_t seed, uint8_t* bits, int n)
+{
+    nr_orc_rng g;
+    nr_orc_rng_seed(&g, seed);
+    for (int i = 0; i < n; ++i) bits[i] = (uint8_t)(nr_orc_rng_next(&g) >> 63);
+}
+
+/* ======================================================================
+ * channel models
+ * ==================================================================== */
+void nr_orc_bpsk_awgn_llr(const uint8_t* bits, int n, double sigma2, uint64_t seed, float* llr)
+{
+    nr_orc_rng g;
+    nr_orc_rng_seed(&g, seed);
+    const double sigma = sqrt(sigma2);
+    for (int i = 0; i < n; ++i) {
+        const double x = bits[i] ? -1.0 : 1.0;
+        const double y = x + sigma * nr_orc_rng_normal(&g);
+        llr[i] = (float)(2.0 * y / sigma2);
+    }
+}
+
+/* noise convention of cuPHY/examples/error_correction/ldpc_decode_test_vec_gen.cpp:183-188:
+ * total noise variance 10^(-SNR/10), half per real component */
+void nr_orc_qpsk_awgn_llr(const uint8_t* bits, int n, double snr_db, uint64_t seed, float* llr)
+{
+    nr_orc_rng g;
+    nr_orc_rng_seed(&g, seed);
+    const double n0 = pow(10.0, -snr_db / 10.0);
+    const double var = n0 / 2.0;
+    const double a = sqrt(0.5);
+    const double sd = sqrt(var);
+    for (int i = 0; i < n; ++i) {
+        const double x = bits[i] ? -1.0 : 1.0;
+        const double y = a * x + sd * nr_orc_rng_normal(&g);
+        llr[i] = (float)(2.0 * a * y / var);
+    }
+}
+
+void nr_orc_consistent_gaussian_llr(const uint8_t* bits, int n, double mu, uint64_t seed, float* llr)
+{
+    nr_orc_rng g;
+    nr_orc_rng_seed(&g, seed);
+    const double sd = sqrt(2.0 * mu);
+    for (int i = 0; i < n; ++i) {
+        const double x = bits[i] ? -1.0 : 1.0;
+        llr[i] = (float)(mu * x + sd * nr_orc_rng_normal(&g));
+    }
+}
+
+/* ======================================================================
+ * IEEE binary16 rounding (round to nearest, ties to even)
+ * ==================================================================== */
+float nr_orc_round_half(float x)
+{
+    if (isnan(x)) return x;
+    const float ax = fabsf(x);
+    if (ax >= 65520.0f) return copysignf(INFINITY, x);
+    if (ax < 6.103515625e-05f) { /* subnormal: quantum 2^-24 */
+        const float q = nearbyintf(ax * 16777216.0f);
+        return copysignf(q / 16777216.0f, x);
+    }
+    int e;
+    frexpf(ax, &e);                              /* ax in [2^(e-1), 2^e) */
+    const float quantum = ldexpf(1.0f, e - 11);  /* 10 fraction bits    */
+    return copysignf(nearbyintf(ax / quantum) * quantum, x);
+}
+
+/* ======================================================================
+ * layered normalized min-sum, instantiated per arithmetic mode
+ * ==================================================================== */
+#define NMS_NAME nms_fp32
+#define NMS_T float
+#define NMS_OP(x) (x)
+#define NMS_ABS fabsf
+#include "nr_nms_body.inc"
+#undef NMS_NAME
+#undef NMS_T
+#undef NMS_OP
+#undef NMS_ABS
+
+#define NMS_NAME nms_fp64
+#define NMS_T double
+#define NMS_OP(x) (x)
+#define NMS_ABS fabs
+#include "nr_nms_body.inc"
+#undef NMS_NAME
+#undef NMS_T
+#undef NMS_OP
+#undef NMS_ABS
+
+/* binary16: every product and sum of two binary16 values is formed in
+ * binary32 and rounded once to binary16.  Since 24 >= 2*11 + 2 the double
+ * rounding is innocuous, so each op equals the correctly rounded binary16
+ * result. */
+#define NMS_NAME nms_fp16
+#define NMS_T float
+#define NMS_OP(x) nr_orc_round_half(x)
+#define NMS_ABS fabsf
+#include "nr_nms_body.inc"
+#undef NMS_NAME
+#undef NMS_T
+#undef NMS_OP
+#undef NMS_ABS
+
+int nr_orc_decode_cb(const nr_orc_dec_cfg* cfg, const float* llr, uint32_t* hard, float* app_out)
+{
+    if (!cfg || !llr || nr_orc_ils(cfg->Z) < 0 || (cfg->bg != 1 && cfg->bg != 2)) return -1;
+    if (cfg->mb < 4 || cfg->mb > nr_orc_num_rows(cfg->bg)) return -1;
+    switch (cfg->mode) {
+    case NR_ORC_FP32: return nms_fp32(cfg, llr, hard, app_out);
+    case NR_ORC_FP64: return nms_fp64(cfg, llr, hard, app_out);
+    case NR_ORC_FP16: return nms_fp16(cfg, llr, hard, app_out);
+    default: return -1;
+    }
+}
+
+/* compute_cC2V.m:42-54 + update_C2V.m:39-45 for one lane */
+void nr_orc_check_node(int d, const float* t, float* c2v, int mode)
+{
+    double min1 = INFINITY, min2 = INFINITY;
+    int idx = -1, par = 0;
+    for (int j = 0; j < d; ++j) {
+        const double v = (mode == NR_ORC_FP16) ? nr_orc_round_half(t[j]) : t[j];
+        const double a = fabs(v);
+        if (a < min1) {
+            min2 = min1;
+            min1 = a;
+            idx = j;
+        } else if (a < min2) {
+            min2 = a;
+        }
+        par ^= (v < 0);
+    }
+    for (int j = 0; j < d; ++j) {
+        const double v = (mode == NR_ORC_FP16) ? nr_orc_round_half(t[j]) : t[j];
+        const double mag = (j == idx) ? min2 : min1;
+        c2v[j] = (float)((par ^ (v < 0)) ? -mag : mag);
+    }
+}
+
+/* ======================================================================
+ * encoder: H*c = 0 with H from the base graph (TS 38.212 5.3.2):
+ * block (r, col) with shift s is the Z x Z identity cyclically shifted
+ * right s times, i.e. check z of row r involves bit (z + s) mod Z of col.
+ * Core parity (4 columns after the systematic ones) is solved with a
+ * cached GF(2) inverse of the 4Z x 4Z core block; each extension parity
+ * column is determined by its single row.
+ * ==================================================================== */
+typedef struct {
+    int n;          /* 4Z */
+    int w;          /* words per row */
+    uint64_t* inv;  /* n rows x w words */
+} core_inv_t;
+
+static core_inv_t* g_core_inv[3][385];
+static pthread_mutex_t g_core_mutex = PTHREAD_MUTEX_INITIALIZER;
+
+static core_inv_t* core_inverse(int bg, int Z)
+{
+    pthread_mutex_lock(&g_core_mutex);
+    core_inv_t* ci = g_core_inv[bg][Z];
+    if (ci) {
+        pthread_mutex_unlock(&g_core_mutex);
+        return ci;
+    }
+    const int nsys = nr_orc_num_sys_cols(bg);
+    const int n = 4 * Z, w = (n + 63) / 64;
+    uint64_t* a = (uint64_t*)calloc((size_t)n * 2 * w, sizeof(uint64_t)); /* [A | I] */
+    for (int r = 0; r < 4; ++r) {
+        const int d = nr_orc_row_degree(bg, r);
+        for (int j = 0; j < d; ++j) {
+            int col, s;
+            nr_orc_edge(bg, Z, r, j, &col, &s);
+            if (col < nsys || col >= nsys + 4) continue;
+            for (int z = 0; z < Z; ++z) {
+                const int row = r * Z + z;
+                const int c = (col - nsys) * Z + (z + s) % Z;
+                a[(size_t)row * 2 * w + c / 64] ^= 1ull << (c % 64);
+            }
+        }
+    }
+    for (int i = 0; i < n; ++i) a[(size_t)i * 2 * w + w + i / 64] |= 1ull << (i % 64);
+    int ok = 1;
+    for (int c = 0; c < n && ok; ++c) {
+        int p = -1;
+        for (int i = c; i < n; ++i)
+            if ((a[(size_t)i * 2 * w + c / 64] >> (c % 64)) & 1) {
+                p = i;
+                break;
+            }
+        if (p < 0) {
+            ok = 0;
+            break;
+        }
+        if (p != c)
+            for (int k = 0; k < 2 * w; ++k) {
+                uint64_t t = a[(size_t)p * 2 * w + k];
+                a[(size_t)p * 2 * w + k] = a[(size_t)c * 2 * w + k];
+                a[(size_t)c * 2 * w + k] = t;
+            }
+        for (int i = 0; i < n; ++i)
+            if (i != c && ((a[(size_t)i * 2 * w + c / 64] >> (c % 64)) & 1))
+                for (int k = 0; k < 2 * w; ++k) a[(size_t)i * 2 * w + k] ^= a[(size_t)c * 2 * w + k];
+    }
+    if (!ok) {
+        free(a);
+        pthread_mutex_unlock(&g_core_mutex);
+        return NULL;
+    }
+    ci = (core_inv_t*)malloc(sizeof(core_inv_t));
+    ci->n = n;
+    ci->w = w;
+    ci->inv = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)n * w);
+    for (int i = 0; i < n; ++i) memcpy(ci->inv + (size_t)i * w, a + (size_t)i * 2 * w + w, sizeof(uint64_t) * w);
+    free(a);
+    g_core_inv[bg][Z] = ci;
+    pthread_mutex_unlock(&g_core_mutex);
+    return ci;
+}
+
+int nr_orc_encode(int bg, int Z, const uint8_t* info, uint8_t* cw)
+{
+    if ((bg != 1 && bg != 2) || nr_orc_ils(Z) < 0) return -1;
+    const int nsys = nr_orc_num_sys_cols(bg), nrows = nr_orc_num_rows(bg);
+    const int ncol = nsys + nrows;
+    core_inv_t* ci = core_inverse(bg, Z);
+    if (!ci) return -2;
+    memset(cw, 0, (size_t)ncol * Z);
+    memcpy(cw, info, (size_t)nsys * Z);
+    /* right-hand side of the 4 core rows from the systematic columns */
+    uint64_t* b = (uint64_t*)calloc((size_t)ci->w, sizeof(uint64_t));
+    for (int r = 0; r < 4; ++r) {
+        const int d = nr_orc_row_degree(bg, r);
+        for (int z = 0; z < Z; ++z) {
+            int acc = 0;
+            for (int j = 0; j < d; ++j) {
+                int col, s;
+                nr_orc_edge(bg, Z, r, j, &col, &s);
+                if (col < nsys) acc ^= cw[col * Z + (z + s) % Z];
+            }
+            if (acc) b[(r * Z + z) / 64] |= 1ull << ((r * Z + z) % 64);
+        }
+    }
+    for (int i = 0; i < ci->n; ++i) {
+        int acc = 0;
+        for (int k = 0; k < ci->w; ++k) acc ^= __builtin_parityll(ci->inv[(size_t)i * ci->w + k] & b[k]);
+        cw[nsys * Z + i] = (uint8_t)acc;
+    }
+    free(b);
+    /* extension rows: row r >= 4 owns column nsys + r (degree-1 column) */
+    for (int r = 4; r < nrows; ++r) {
+        const int d = nr_orc_row_degree(bg, r);
+        int ext_s = -1;
+        for (int j = 0; j < d; ++j) {
+            int col, s;
+            nr_orc_edge(bg, Z, r, j, &col, &s);
+            if (col == nsys + r) ext_s = s;
+        }
+        if (ext_s < 0) return -3;
+        for (int z = 0; z < Z; ++z) {
+            int acc = 0;
+            for (int j = 0; j < d; ++j) {
+                int col, s;
+                nr_orc_edge(bg, Z, r, j, &col, &s);
+                if (col != nsys + r) acc ^= cw[col * Z + (z + s) % Z];
+            }
+            cw[(nsys + r) * Z + (z + ext_s) % Z] = (uint8_t)acc;
+        }
+    }
+    return 0;
+}
+
+int nr_orc_syndrome(int bg, int Z, int mb, const uint8_t* x)
+{
+    int fails = 0;
+    for (int r = 0; r < mb; ++r) {
+        const int d = nr_orc_row_degree(bg, r);
+        for (int z = 0; z < Z; ++z) {
+            int acc = 0;
+            for (int j = 0; j < d; ++j) {
+                int col, s;
+                nr_orc_edge(bg, Z, r, j, &col, &s);
+                acc ^= x[col * Z + (z + s) % Z];
+            }
+            fails += acc;
+        }
+    }
+    return fails;
+}
+
+/* ======================================================================
+ * CRC, TS 38.212 5.1 (bit-serial long division, zero initial value)
+ * ==================================================================== */
+uint32_t nr_orc_crc(const uint8_t* bits, int n, int type)
+{
+    uint32_t poly, L;
+    switch (type) {
+    case NR_CRC24A: poly = 0x864CFBu; L = 24; break;
+    case NR_CRC24B: poly = 0x800063u; L = 24; break;
+    case NR_CRC16: poly = 0x1021u; L = 16; break;
+    default: return 0xFFFFFFFFu;
+    }
+    const uint32_t top = 1u << (L - 1), mask = (L == 32) ? 0xFFFFFFFFu : ((1u << L) - 1u);
+    uint32_t reg = 0;
+    for (int i = 0; i < n; ++i) {
+        const uint32_t fb = ((reg & top) ? 1u : 0u) ^ (bits[i] & 1u);
+        reg = (reg << 1) & mask;
+        if (fb) reg ^= poly;
+    }
+    return reg;
+}
+
+/* ======================================================================
+ * de-rate-matching with HARQ combining
+ * ==================================================================== */
+int nr_orc_k0(int bg, int rv, int Ncb, int Zc)
+{
+    static const int n1[4] = {0, 17, 33, 56}, n2[4] = {0, 13, 25, 43};
+    if (rv < 0 || rv > 3) return -1;
+    if (bg == 1) return (int)(((long long)n1[rv] * Ncb) / (66LL * Zc)) * Zc;
+    return (int)(((long long)n2[rv] * Ncb) / (50LL * Zc)) * Zc;
+}
+
+/* derate_match.m:54-63 */
+int nr_orc_rm_E(const nr_orc_rm_cfg* c, int r)
+{
+    const int nq = c->Nl * c->Qm;
+    const int q = c->G / nq; /* G / (Nl*Qm), exact */
+    if (r <= c->C - (q % c->C) - 1) return nq * (q / c->C);
+    return nq * ((q + c->C - 1) / c->C);
+}
+
+#define LLR_MAX_ABS 10000.0f
+
+static void derate_match_impl(const nr_orc_rm_cfg* c, const float* llr_demap, float* harq, int fp16)
+{
+    const int twoZ = 2 * c->Zc;
+    const int Kp = c->K - c->F;
+    const int fill_lo = Kp - twoZ, fill_hi = c->K - twoZ; /* filler [lo, hi) in circular-buffer coords */
+    size_t start = 0;
+    for (int r = 0; r < c->C; ++r) {
+        float* out = harq + (size_t)r * c->Ncb_padded;
+        const int E = nr_orc_rm_E(c, r);
+        const int EQ = E / c->Qm;
+        if (c->ndi)
+            for (int n = 0; n < c->Ncb_padded; ++n) out[n] = 0.0f;
+        /* bit selection after de-interleaving (derate_match.m:138-170):
+         * the k-th selected LLR is e_k = LLR_c[(k % EQ)*Qm + k / EQ] */
+        int k = 0, j = 0;
+        while (k < E) {
+            const int idx = (c->k0 + j) % c->Ncb;
+            if (idx < fill_lo || idx >= fill_hi) {
+                const int kIdx = k / EQ, jIdx = k % EQ;
+                float v = llr_demap[start + (size_t)jIdx * c->Qm + kIdx];
+                float* o = &out[twoZ + idx];
+                if (fp16) {
+                    v = nr_orc_round_half(v);
+                    if (v > LLR_MAX_ABS) v = LLR_MAX_ABS;
+                    if (v < -LLR_MAX_ABS) v = -LLR_MAX_ABS;
+                    float s = nr_orc_round_half(*o + v);
+                    if (s > LLR_MAX_ABS) s = LLR_MAX_ABS;
+                    if (s < -LLR_MAX_ABS) s = -LLR_MAX_ABS;
+                    *o = s;
+                } else {
+                    if (v > LLR_MAX_ABS) v = LLR_MAX_ABS;
+                    if (v < -LLR_MAX_ABS) v = -LLR_MAX_ABS;
+                    float s = *o + v;
+                    if (s > LLR_MAX_ABS) s = LLR_MAX_ABS;
+                    if (s < -LLR_MAX_ABS) s = -LLR_MAX_ABS;
+                    *o = s;
+                }
+                ++k;
+            }
+            ++j;
+        }
+        for (int n = fill_lo; n < fill_hi; ++n) out[twoZ + n] = LLR_MAX_ABS; /* :179 */
+        for (int n = 0; n < twoZ; ++n) out[n] = 0.0f;                        /* punctured */
+        start += (size_t)E;
+    }
+}
+
+void nr_orc_derate_match(const nr_orc_rm_cfg* c, const float* llr_demap, float* harq)
+{
+    derate_match_impl(c, llr_demap, harq, 0);
+}
+
+void nr_orc_derate_match_fp16(const nr_orc_rm_cfg* c, const float* llr_demap, float* harq)
+{
+    derate_match_impl(c, llr_demap, harq, 1);
+}
+
+/* ======================================================================
+ * CB CRC / TB assembly / TB CRC (cuPHY crc.cu:55-237 conventions)
+ * ==================================================================== */
+#define WORDS_PER_CB 264
+
+int nr_orc_crc_desegment(const uint32_t* cb_words, int C, int Kp,
+                         uint32_t* cb_crc, uint8_t* tb_bytes, uint32_t* tb_crc)
+{
+    /* the CB CRC covers the decoded bits up to the next 32-bit boundary */
+    const int nbits = ((Kp + 31) / 32) * 32;
+    uint8_t* bits = (uint8_t*)malloc((size_t)nbits);
+    int crc_bits;
+    int type;
+    if (C > 1) {
+        crc_bits = 24;
+        type = NR_CRC24B;
+    } else {
+        crc_bits = 0;
+        type = ((Kp + 7) / 8 <= 480) ? NR_CRC16 : NR_CRC24A;
+    }
+    const int data_bytes = (Kp - crc_bits + 7) / 8;
+    for (int r = 0; r < C; ++r) {
+        const uint32_t* w = cb_words + (size_t)r * WORDS_PER_CB;
+        for (int n = 0; n < nbits; ++n) bits[n] = (uint8_t)((w[n >> 5] >> (n & 31)) & 1u);
+        cb_crc[r] = nr_orc_crc(bits, nbits, type);
+        for (int b = 0; b < data_bytes; ++b) {
+            uint8_t v = 0;
+            for (int i = 0; i < 8; ++i) v = (uint8_t)((v << 1) | bits[8 * b + i]);
+            tb_bytes[(size_t)r * data_bytes + b] = v;
+        }
+    }
+    int total = C * data_bytes;
+    while (total % 4) tb_bytes[total++] = 0;
+    if (C == 1) {
+        *tb_crc = cb_crc[0];
+    } else {
+        uint8_t* tbits = (uint8_t*)malloc((size_t)total * 8);
+        for (int b = 0; b < total; ++b)
+            for (int i = 0; i < 8; ++i) tbits[8 * b + i] = (uint8_t)((tb_bytes[b] >> (7 - i)) & 1u);
+        *tb_crc = nr_orc_crc(tbits, total * 8, NR_CRC24A);
+        free(tbits);
+    }
+    free(bits);
+    return total;
+}
+
+/* ======================================================================
+ * many code blocks on several host threads (CPU baseline): code block i
+ * reads llr + i*llr_stride and writes hard + i*hard_stride; block i goes to
+ * thread i % nthreads.
+ * ==================================================================== */
+typedef struct {
+    const nr_orc_dec_cfg* cfg;
+    const float* llr;
+    size_t llr_stride;
+    uint32_t* hard;
+    size_t hard_stride;
+    int ncb, tid, nthreads;
+} par_job_t;
+
+static void* par_worker(void* p)
+{
+    par_job_t* j = (par_job_t*)p;
+    for (int i = j->tid; i < j->ncb; i += j->nthreads)
+        nr_orc_decode_cb(j->cfg, j->llr + (size_t)i * j->llr_stride,
+                         j->hard ? j->hard + (size_t)i * j->hard_stride : NULL, NULL);
+    return NULL;
+}
+
+int nr_orc_decode_cbs(const nr_orc_dec_cfg* cfg, const float* llr, size_t llr_stride, int ncb,
+                      uint32_t* hard, size_t hard_stride, int nthreads)
+{
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    pthread_t th[256];
+    par_job_t jobs[256];
+    for (int t = 0; t < nthreads; ++t) {
+        jobs[t] = (par_job_t){cfg, llr, llr_stride, hard, hard_stride, ncb, t, nthreads};
+        if (pthread_create(&th[t], NULL, par_worker, &jobs[t]) != 0) return -1;
+    }
+    for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+    return 0;
+}

@@ -1,0 +1,119 @@
+"""C-ABI checks that need no GPU: the library loads, exports every entry point
+declared in include/cuphy_hip.h, and rejects bad arguments the way the
+reference's cuphy_ldpc.cpp does (no compute call is made)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_entry_points():
+    names = []
+    for fn in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if not fn.endswith(".h"):
+            continue
+        text = open(os.path.join(ROOT, "include", fn)).read()
+        names += re.findall(r"CUPHYWINAPI\s+(\w+)\s*\(", text)
+    return sorted(set(names))
+
+
+@pytest.fixture(scope="module")
+def L():
+    from aerial_amd import _lib
+    _lib.lib()
+    return _lib
+
+
+def test_header_declares_the_reference_surface():
+    names = header_entry_points()
+    for required in ("cuphyCreateLDPCDecoder", "cuphyDestroyLDPCDecoder", "cuphyErrorCorrectionLDPCDecode",
+                     "cuphyErrorCorrectionLDPCTransportBlockDecode", "cuphyErrorCorrectionLDPCDecodeGetWorkspaceSize",
+                     "cuphyErrorCorrectionLDPCDecodeSetNormalization",
+                     "cuphyErrorCorrectionLDPCDecodeGetLaunchDescriptor", "cuphyCreatePuschRxRateMatch",
+                     "cuphySetupPuschRxRateMatch", "cuphyDestroyPuschRxRateMatch", "cuphyPuschRxRateMatchGetDescrInfo",
+                     "cuphyCreatePuschRxCrcDecode", "cuphySetupPuschRxCrcDecode", "cuphyDestroyPuschRxCrcDecode",
+                     "cuphyPuschRxCrcDecodeGetDescrInfo"):
+        assert required in names
+
+
+def test_library_exports_every_declared_symbol(L):
+    lib = L.lib()
+    missing = [n for n in header_entry_points() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_every_declared_symbol_has_a_python_signature():
+    from aerial_amd import _lib
+    missing = [n for n in header_entry_points() if n not in _lib.SIGNATURES]
+    assert not missing, missing
+
+
+def test_struct_sizes_match_header(L):
+    # sizes of the C structs (x86-64): checked against the header layout
+    assert C.sizeof(L.LDPCDecodeConfigDesc) == 40
+    assert C.sizeof(L.LDPCDecodeDesc) == 40 + 8 + 3 * 32 * 16
+    assert C.sizeof(L.PerTbParams) == 488
+
+
+def test_error_strings(L):
+    lib = L.lib()
+    assert lib.cuphyGetErrorName(0) == b"CUPHY_STATUS_SUCCESS"
+    assert lib.cuphyGetErrorName(12) == b"CUPHY_STATUS_UNSUPPORTED_CONFIG"
+    assert lib.cuphyGetErrorName(99) == b"CUPHY_UNKNOWN_STATUS"
+    assert lib.cuphyGetErrorString(3).startswith(b"An invalid argument")
+
+
+def test_null_arguments_rejected(L):
+    lib = L.lib()
+    INV = L.CUPHY_STATUS_INVALID_ARGUMENT
+    assert lib.cuphyCreateContext(None, 0) == INV
+    assert lib.cuphyDestroyContext(None) == INV
+    assert lib.cuphyCreateLDPCDecoder(None, None, 0) == INV
+    assert lib.cuphyDestroyLDPCDecoder(None) == INV
+    assert lib.cuphyErrorCorrectionLDPCTransportBlockDecode(None, None, None) == INV
+    assert lib.cuphyErrorCorrectionLDPCDecodeSetNormalization(None, None) == INV
+    assert lib.cuphyErrorCorrectionLDPCDecodeGetLaunchDescriptor(None, None) == INV
+    cfg = L.LDPCDecodeConfigDesc()
+    sz = C.c_size_t()
+    assert lib.cuphyErrorCorrectionLDPCDecodeGetWorkspaceSize(None, C.byref(cfg), 1, C.byref(sz)) == INV
+    assert lib.cuphyErrorCorrectionLDPCDecode(None, None, None, None, None, None, None, None, None) == INV
+
+
+def test_tensor_descriptor_semantics(L):
+    lib = L.lib()
+    t = C.c_void_p()
+    assert lib.cuphyCreateTensorDescriptor(C.byref(t)) == 0
+    dims = (C.c_int * 2)(25344 + 768, 8)
+    assert lib.cuphySetTensorDescriptor(t, L.CUPHY_R_32F, 2, dims, None, 0) == 0
+    dt, nd = C.c_int(), C.c_int()
+    od, os_ = (C.c_int * 2)(), (C.c_int * 2)()
+    assert lib.cuphyGetTensorDescriptor(t, 2, C.byref(dt), C.byref(nd), od, os_) == 0
+    assert (dt.value, nd.value, list(od), list(os_)) == (L.CUPHY_R_32F, 2, [26112, 8], [1, 26112])
+    sz = C.c_size_t()
+    assert lib.cuphyGetTensorSizeInBytes(t, C.byref(sz)) == 0 and sz.value == 26112 * 8 * 4
+    # bit tensor: first dimension padded to whole 32-bit words
+    bdims = (C.c_int * 2)(8448 + 5, 3)
+    assert lib.cuphySetTensorDescriptor(t, L.CUPHY_BIT, 2, bdims, None, 0) == 0
+    assert lib.cuphyGetTensorDescriptor(t, 2, None, None, od, os_) == 0
+    assert list(os_) == [1, 8480]
+    # overlapping strides and bad bit strides are rejected
+    bad = (C.c_int * 2)(1, 10)
+    assert lib.cuphySetTensorDescriptor(t, L.CUPHY_R_16F, 2, (C.c_int * 2)(20, 4), bad, 0) == 3
+    assert lib.cuphySetTensorDescriptor(t, L.CUPHY_BIT, 2, (C.c_int * 2)(40, 4), (C.c_int * 2)(1, 40), 0) == 3
+    assert lib.cuphySetTensorDescriptor(t, L.CUPHY_R_32F, 0, dims, None, 0) == 3
+    assert lib.cuphyDestroyTensorDescriptor(t) == 0
+
+
+def test_descriptor_info_queries(L):
+    lib = L.lib()
+    a, b = C.c_size_t(), C.c_size_t()
+    if hasattr(lib, "cuphyPuschRxRateMatchGetDescrInfo"):
+        assert lib.cuphyPuschRxRateMatchGetDescrInfo(C.byref(a), C.byref(b)) == 0
+        assert a.value > 0 and b.value > 0
+        assert lib.cuphyPuschRxRateMatchGetDescrInfo(None, C.byref(b)) == 3
+    if hasattr(lib, "cuphyPuschRxCrcDecodeGetDescrInfo"):
+        assert lib.cuphyPuschRxCrcDecodeGetDescrInfo(C.byref(a), C.byref(b)) == 0
+        assert a.value > 0 and b.value > 0
