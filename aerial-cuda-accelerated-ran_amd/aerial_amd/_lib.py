@@ -142,6 +142,7 @@ SIGNATURES = {
     "cuphySetupPuschRxCrcDecode": ([_VP, C.c_uint16, _P(C.c_uint16), _VP, _VP, _VP, _VP, _P(PerTbParams), _VP,
                                     _VP, _VP, C.c_uint8, _VP, _VP, _VP], _I),
     "cuphyDestroyPuschRxCrcDecode": ([_VP], _I),
+    "cuphyHipLDPCEncodeCodeBlocks": ([_I, _I, _VP, _I, _I, _VP, _I, _VP], _I),
 }
 
 _lib = None

@@ -20,6 +20,7 @@ ARCH = os.environ.get("NRLDPC_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = [
     "ldpc_decode.hip",
+    "ldpc_encode.hip",
     "cuphy_api.cpp",
 ]
 

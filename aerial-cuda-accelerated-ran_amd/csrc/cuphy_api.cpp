@@ -11,6 +11,7 @@
 
 #include "cuphy_hip.h"
 #include "ldpc_decode.hpp"
+#include "ldpc_encode.hpp"
 
 // ---------------------------------------------------------------------------
 // context (cuphy.h:1156-1208): records the device it was created on
@@ -387,6 +388,16 @@ cuphyStatus_t CUPHYWINAPI cuphyErrorCorrectionLDPCDecodeGetLaunchDescriptor(cuph
 {
     if (!decoder || !launchConfig) return CUPHY_STATUS_INVALID_ARGUMENT;
     return nrldpc::get_launch_config(*launchConfig);
+}
+
+// ---------------------------------------------------------------------------
+// extensions
+// ---------------------------------------------------------------------------
+cuphyStatus_t CUPHYWINAPI cuphyHipLDPCEncodeCodeBlocks(int BG, int Z, const uint8_t* info, int info_stride,
+                                                       int num_code_blocks, uint8_t* cw, int cw_stride,
+                                                       hipStream_t strm)
+{
+    return nrldpc::encode_cbs(BG, Z, info, info_stride, num_code_blocks, cw, cw_stride, strm);
 }
 
 }  // extern "C"

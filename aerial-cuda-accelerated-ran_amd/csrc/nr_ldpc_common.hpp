@@ -71,6 +71,22 @@ constexpr reduced_shift_tables make_reduced_shift_tables()
     return t;
 }
 
+// Row starts and columns of both base graphs as one constant-initialised
+// table (runtime-indexed device copies: g_graph_tables in the .hip files).
+struct graph_tables {
+    uint16_t row_start[2][48];
+    uint8_t col[2][320];
+};
+constexpr graph_tables make_graph_tables()
+{
+    graph_tables t{};
+    for (int r = 0; r <= NR_BG1_NUM_ROWS; ++r) t.row_start[0][r] = nr_bg1_row_start[r];
+    for (int r = 0; r <= NR_BG2_NUM_ROWS; ++r) t.row_start[1][r] = nr_bg2_row_start[r];
+    for (int e = 0; e < NR_BG1_NUM_EDGES; ++e) t.col[0][e] = nr_bg1_col[e];
+    for (int e = 0; e < NR_BG2_NUM_EDGES; ++e) t.col[1][e] = nr_bg2_col[e];
+    return t;
+}
+
 // Compile-time loop: f(std::integral_constant<int, I>) for I = 0..N-1.
 template <typename F, int... I>
 __host__ __device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>)

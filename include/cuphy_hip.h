@@ -330,6 +330,20 @@ cuphyStatus_t CUPHYWINAPI cuphySetupPuschRxCrcDecode(cuphyPuschRxCrcDecodeHndl_t
                                                      hipStream_t strm);
 cuphyStatus_t CUPHYWINAPI cuphyDestroyPuschRxCrcDecode(cuphyPuschRxCrcDecodeHndl_t puschRxCrcDecodeHndl);        /* cuphy.h:3293 */
 
+/* ------------------------------------------------------------------------
+ * Extensions of this library (no cuphy.h counterpart with this signature)
+ * ---------------------------------------------------------------------- */
+
+/* Systematic TS 38.212 LDPC encoding of ncb code blocks on the device: the
+ * test-signal source of the decode path (the reference's DL encoder API,
+ * cuphy.h:3781-3850, is a different, tensor-based interface).  info: ncb rows
+ * of nsys*Z bytes (0/1, filler bits 0), row stride info_stride bytes;
+ * cw: ncb rows of (nsys + nrows)*Z bytes, row stride cw_stride bytes, first
+ * 2Z (punctured) columns included. */
+cuphyStatus_t CUPHYWINAPI cuphyHipLDPCEncodeCodeBlocks(int BG, int Z, const uint8_t* info, int info_stride,
+                                                       int num_code_blocks, uint8_t* cw, int cw_stride,
+                                                       hipStream_t strm);
+
 #ifdef __cplusplus
 }
 #endif
