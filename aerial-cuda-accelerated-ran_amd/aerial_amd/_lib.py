@@ -107,6 +107,13 @@ class PerTbParams(C.Structure):
                 ("debug_d_derateCbsIndices", C.c_void_p)]
 
 
+class LDPCWorkItem(C.Structure):
+    _fields_ = [("llr", C.c_void_p), ("out", C.c_void_p), ("soft", C.c_void_p), ("BG", C.c_int16),
+                ("Z", C.c_int16), ("num_parity_nodes", C.c_int16), ("Kb", C.c_int16),
+                ("max_iterations", C.c_int16), ("flags", C.c_uint16), ("norm", Norm),
+                ("clamp_value", C.c_float), ("reserved", C.c_uint32)]
+
+
 class TensorPrm(C.Structure):
     _fields_ = [("desc", C.c_void_p), ("pAddr", C.c_void_p)]
 
@@ -143,6 +150,8 @@ SIGNATURES = {
                                     _VP, _VP, C.c_uint8, _VP, _VP, _VP], _I),
     "cuphyDestroyPuschRxCrcDecode": ([_VP], _I),
     "cuphyHipLDPCEncodeCodeBlocks": ([_I, _I, _VP, _I, _I, _VP, _I, _VP], _I),
+    "cuphyHipLDPCBatchDecodeGetWorkspaceSize": ([_P(C.c_size_t)], _I),
+    "cuphyHipLDPCBatchDecode": ([_VP, _VP, _I, _I, _I, _I, _VP, _VP], _I),
 }
 
 _lib = None

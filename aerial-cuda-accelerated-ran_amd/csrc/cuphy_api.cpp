@@ -38,6 +38,7 @@ struct cuphyTensorDescriptor {
 // ---------------------------------------------------------------------------
 struct cuphyLDPCDecoder {
     int device = 0;
+    int cu_count = 0;
 };
 
 namespace {
@@ -262,6 +263,7 @@ cuphyStatus_t CUPHYWINAPI cuphyCreateLDPCDecoder(cuphyContext_t context, cuphyLD
     cuphyLDPCDecoder* d = new (std::nothrow) cuphyLDPCDecoder;
     if (!d) return CUPHY_STATUS_ALLOC_FAILED;
     d->device = context->device;
+    d->cu_count = context->cu_count;
     *pdecoder = d;
     return CUPHY_STATUS_SUCCESS;
 }
@@ -398,6 +400,22 @@ cuphyStatus_t CUPHYWINAPI cuphyHipLDPCEncodeCodeBlocks(int BG, int Z, const uint
                                                        hipStream_t strm)
 {
     return nrldpc::encode_cbs(BG, Z, info, info_stride, num_code_blocks, cw, cw_stride, strm);
+}
+
+cuphyStatus_t CUPHYWINAPI cuphyHipLDPCBatchDecodeGetWorkspaceSize(size_t* sizeInBytes)
+{
+    if (!sizeInBytes) return CUPHY_STATUS_INVALID_ARGUMENT;
+    *sizeInBytes = 256;
+    return CUPHY_STATUS_SUCCESS;
+}
+
+cuphyStatus_t CUPHYWINAPI cuphyHipLDPCBatchDecode(cuphyLDPCDecoder_t decoder, const cuphyHipLDPCWorkItem_t* d_items,
+                                                  int num_items, cuphyDataType_t llr_type, int fp16_math,
+                                                  int max_values_per_cb, void* d_workspace, hipStream_t strm)
+{
+    if (!decoder || !d_workspace) return CUPHY_STATUS_INVALID_ARGUMENT;
+    return nrldpc::decode_batch(d_items, num_items, llr_type, fp16_math != 0, max_values_per_cb,
+                                static_cast<uint32_t*>(d_workspace), decoder->cu_count, strm);
 }
 
 }  // extern "C"

@@ -33,8 +33,11 @@
 #include "nr_ldpc_common.hpp"
 #include "ldpc_decode.hpp"
 
+#include <algorithm>
 #include <cstdio>
 #include <mutex>
+#include <utility>
+#include <vector>
 
 namespace nrldpc {
 
@@ -154,28 +157,37 @@ __device__ __forceinline__ uint32_t row_syndrome(const T* __restrict__ app, shif
     return par;
 }
 
-template <int BG, typename T, typename TIn, bool ET>
-__global__ __launch_bounds__(384) void ldpc_layered_tb(cuphyLDPCDecodeDesc_t desc)
+// Everything one code block's decode needs, resolved from a TB descriptor
+// (transport-block kernel) or from a batch work item (persistent kernel).
+struct cb_job {
+    const void* llr;
+    uint32_t* out;
+    _Float16* soft;  // nullptr: no soft output
+    int Z, mb, Kb, max_iter;
+    bool early_term;
+    float clamp;
+    float alpha_f32;
+    uint16_t alpha_f16;
+};
+
+// Decode one code block with the whole workgroup; LDS `app` holds
+// (NSYS + mb) * Z values of type T.  Every thread of the workgroup calls it.
+template <int BG, typename T, typename TIn>
+__device__ __forceinline__ void decode_one(T* __restrict__ app, const cb_job& job)
 {
     using BGT = bg_traits<BG>;
     constexpr int NR = BGT::NROWS;
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-    T* app = reinterpret_cast<T*>(lds_raw);
-
-    const cuphyLDPCDecodeConfigDesc_t& cfg = desc.config;
-    const uint32_t Z = (uint32_t)cfg.Z;
-    const int mb = cfg.num_parity_nodes;
+    const uint32_t Z = (uint32_t)job.Z;
+    const int mb = job.mb;
     const int nval = (BGT::NSYS + mb) * (int)Z;
     const uint32_t tid = threadIdx.x;
     const uint32_t nthr = blockDim.x;
-    const cw_location loc = locate_cw(desc, blockIdx.x);
 
     // ---- LLRs -> LDS: convert, clamp, signed zero -> +0 ----------------
     {
-        const TIn* src = static_cast<const TIn*>(desc.llr_input[loc.tb].addr) +
-                         (size_t)loc.idx * (size_t)desc.llr_input[loc.tb].stride_elements;
-        const T clampv = (T)cfg.clamp_value;
-        const bool do_clamp = cfg.clamp_value > 0.0f;
+        const TIn* src = static_cast<const TIn*>(job.llr);
+        const T clampv = (T)job.clamp;
+        const bool do_clamp = job.clamp > 0.0f;
         for (int n = tid; n < nval; n += nthr) {
             T v = llr_in<TIn, T>::cvt(src[n]);
             if (do_clamp) v = num<T>::vmax(num<T>::vmin(v, clampv), -clampv);
@@ -185,12 +197,10 @@ __global__ __launch_bounds__(384) void ldpc_layered_tb(cuphyLDPCDecodeDesc_t des
     __syncthreads();
 
     T alpha;
-    if constexpr (std::is_same<T, float>::value) {
-        alpha = cfg.norm.f32;
-    } else {
-        const uint16_t bits = (uint16_t)(cfg.norm.f16x2 & 0xFFFFu);
-        alpha = __builtin_bit_cast(_Float16, bits);
-    }
+    if constexpr (std::is_same<T, float>::value)
+        alpha = job.alpha_f32;
+    else
+        alpha = __builtin_bit_cast(_Float16, job.alpha_f16);
 
     T st_n1[NR], st_n2[NR];
     uint32_t st_meta[NR];
@@ -202,7 +212,7 @@ __global__ __launch_bounds__(384) void ldpc_layered_tb(cuphyLDPCDecodeDesc_t des
 
     const bool active = tid < Z;
     shift_ptr sh = (shift_ptr)(const void*)&g_shift_tables.v[BG - 1][lifting_size_index((int)Z)][0];
-    for (int it = 0; it < cfg.max_iterations; ++it) {
+    for (int it = 0; it < job.max_iter; ++it) {
         // Opaque copies of the lane index and of the shift pointer, renewed
         // per row: the 316 (BG1) per-edge shifts and addresses are reloaded
         // and recomputed where used instead of being hoisted out of the
@@ -219,7 +229,7 @@ __global__ __launch_bounds__(384) void ldpc_layered_tb(cuphyLDPCDecodeDesc_t des
                 __syncthreads();
             }
         });
-        if constexpr (ET) {
+        if (job.early_term) {
             uint32_t fail = 0;
             static_for<NR>([&](auto Rc) {
                 constexpr int R = decltype(Rc)::value;
@@ -235,9 +245,8 @@ __global__ __launch_bounds__(384) void ldpc_layered_tb(cuphyLDPCDecodeDesc_t des
     }
 
     // ---- hard decisions: Kb*Z bits, 64 per wave per step (ballot) --------
-    const int nbits = (int)cfg.Kb * (int)Z;
+    const int nbits = job.Kb * (int)Z;
     const int nwords = (nbits + 31) >> 5;
-    uint32_t* dst = desc.tb_output[loc.tb].addr + (size_t)loc.idx * (size_t)desc.tb_output[loc.tb].stride_words;
     {
         const int lane = (int)(tid & 63u);
         const int wave = (int)(tid >> 6);
@@ -247,16 +256,85 @@ __global__ __launch_bounds__(384) void ldpc_layered_tb(cuphyLDPCDecodeDesc_t des
             const bool bit = (n < nbits) && (app[n] <= (T)0);
             const uint64_t m = __ballot(bit);
             if (lane == 0) {
-                dst[2 * c] = (uint32_t)m;
-                if (2 * c + 1 < nwords) dst[2 * c + 1] = (uint32_t)(m >> 32);
+                job.out[2 * c] = (uint32_t)m;
+                if (2 * c + 1 < nwords) job.out[2 * c + 1] = (uint32_t)(m >> 32);
             }
         }
     }
     // ---- soft outputs: Kb*Z binary16 APP values ---------------------------
-    if ((cfg.flags & CUPHY_LDPC_DECODE_WRITE_SOFT_OUTPUTS) && desc.llr_output[loc.tb].addr) {
-        _Float16* so = static_cast<_Float16*>(desc.llr_output[loc.tb].addr) +
-                       (size_t)loc.idx * (size_t)desc.llr_output[loc.tb].stride_elements;
-        for (int n = tid; n < nbits; n += nthr) so[n] = (_Float16)app[n];
+    if (job.soft)
+        for (int n = tid; n < nbits; n += nthr) job.soft[n] = (_Float16)app[n];
+}
+
+template <int BG, typename T, typename TIn>
+__global__ __launch_bounds__(384) void ldpc_layered_tb(cuphyLDPCDecodeDesc_t desc)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const cuphyLDPCDecodeConfigDesc_t& cfg = desc.config;
+    const cw_location loc = locate_cw(desc, blockIdx.x);
+    cb_job job;
+    job.llr = static_cast<const TIn*>(desc.llr_input[loc.tb].addr) +
+              (size_t)loc.idx * (size_t)desc.llr_input[loc.tb].stride_elements;
+    job.out = desc.tb_output[loc.tb].addr + (size_t)loc.idx * (size_t)desc.tb_output[loc.tb].stride_words;
+    job.soft = ((cfg.flags & CUPHY_LDPC_DECODE_WRITE_SOFT_OUTPUTS) && desc.llr_output[loc.tb].addr)
+                   ? static_cast<_Float16*>(desc.llr_output[loc.tb].addr) +
+                         (size_t)loc.idx * (size_t)desc.llr_output[loc.tb].stride_elements
+                   : nullptr;
+    job.Z = cfg.Z;
+    job.mb = cfg.num_parity_nodes;
+    job.Kb = cfg.Kb;
+    job.max_iter = cfg.max_iterations;
+    job.early_term = (cfg.flags & CUPHY_LDPC_DECODE_EARLY_TERM) != 0;
+    job.clamp = cfg.clamp_value;
+    job.alpha_f32 = cfg.norm.f32;
+    job.alpha_f16 = (uint16_t)(cfg.norm.f16x2 & 0xFFFFu);
+    decode_one<BG, T, TIn>(reinterpret_cast<T*>(lds_raw), job);
+}
+
+// Persistent heterogeneous decoder: each workgroup repeatedly takes the
+// next work item from a device queue (atomic counter) and decodes it, so
+// code blocks of any (BG, Z, mb) share one launch.  The queue state
+// (2 words: next item, finished workgroups) is returned to zero by the
+// last workgroup to finish, so the launch can be replayed from a graph.
+template <typename T, typename TIn>
+__global__ __launch_bounds__(384) void ldpc_layered_persistent(const cuphyHipLDPCWorkItem_t* __restrict__ items,
+                                                               int n_items, uint32_t* __restrict__ queue)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __shared__ int s_item;
+    T* app = reinterpret_cast<T*>(lds_raw);
+    for (;;) {
+        if (threadIdx.x == 0) s_item = (int)atomicAdd(&queue[0], 1u);
+        __syncthreads();
+        const int i = s_item;
+        __syncthreads();
+        if (i >= n_items) break;
+        const cuphyHipLDPCWorkItem_t& w = items[i];
+        cb_job job;
+        job.llr = w.llr;
+        job.out = w.out;
+        job.soft = static_cast<_Float16*>(w.soft);
+        job.Z = w.Z;
+        job.mb = w.num_parity_nodes;
+        job.Kb = w.Kb;
+        job.max_iter = w.max_iterations;
+        job.early_term = (w.flags & CUPHY_LDPC_DECODE_EARLY_TERM) != 0;
+        job.clamp = w.clamp_value;
+        job.alpha_f32 = w.norm.f32;
+        job.alpha_f16 = (uint16_t)(w.norm.f16x2 & 0xFFFFu);
+        if (w.BG == 1)
+            decode_one<1, T, TIn>(app, job);
+        else
+            decode_one<2, T, TIn>(app, job);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const uint32_t done = atomicAdd(&queue[1], 1u);
+        if (done == gridDim.x - 1) {
+            atomicExch(&queue[0], 0u);
+            atomicExch(&queue[1], 0u);
+        }
     }
 }
 
@@ -265,24 +343,15 @@ __global__ __launch_bounds__(384) void ldpc_layered_tb(cuphyLDPCDecodeDesc_t des
 // ---------------------------------------------------------------------------
 namespace {
 
-template <int BG, typename T, typename TIn, bool ET>
-void* kernel_ptr()
+void* select_kernel(int bg, cuphyDataType_t llr_type, bool fp16_math)
 {
-    return reinterpret_cast<void*>(&ldpc_layered_tb<BG, T, TIn, ET>);
-}
-
-void* select_kernel(int bg, cuphyDataType_t llr_type, bool fp16_math, bool et)
-{
-#define NRLDPC_SEL(BG)                                                                              \
-    if (llr_type == CUPHY_R_32F && !fp16_math)                                                       \
-        return et ? kernel_ptr<BG, float, float, true>() : kernel_ptr<BG, float, float, false>();     \
-    if (llr_type == CUPHY_R_32F && fp16_math)                                                        \
-        return et ? kernel_ptr<BG, _Float16, float, true>() : kernel_ptr<BG, _Float16, float, false>(); \
-    if (llr_type == CUPHY_R_16F)                                                                     \
-        return et ? kernel_ptr<BG, _Float16, _Float16, true>() : kernel_ptr<BG, _Float16, _Float16, false>();
-    if (bg == 1) { NRLDPC_SEL(1) }
-    if (bg == 2) { NRLDPC_SEL(2) }
-#undef NRLDPC_SEL
+    if (bg == 1) {
+        if (llr_type == CUPHY_R_32F) return fp16_math ? (void*)&ldpc_layered_tb<1, _Float16, float> : (void*)&ldpc_layered_tb<1, float, float>;
+        if (llr_type == CUPHY_R_16F) return (void*)&ldpc_layered_tb<1, _Float16, _Float16>;
+    } else if (bg == 2) {
+        if (llr_type == CUPHY_R_32F) return fp16_math ? (void*)&ldpc_layered_tb<2, _Float16, float> : (void*)&ldpc_layered_tb<2, float, float>;
+        if (llr_type == CUPHY_R_16F) return (void*)&ldpc_layered_tb<2, _Float16, _Float16>;
+    }
     return nullptr;
 }
 
@@ -321,7 +390,7 @@ cuphyStatus_t fill_launch(const cuphyLDPCDecodeDesc_t& desc, launch_geometry& g)
     if (!config_supported(c)) return CUPHY_STATUS_UNSUPPORTED_CONFIG;
     if (desc.num_tbs < 1 || desc.num_tbs > CUPHY_LDPC_DECODE_DESC_MAX_TB) return CUPHY_STATUS_INVALID_ARGUMENT;
     const bool fp16 = use_fp16_math(c);
-    g.func = select_kernel(c.BG, c.llr_type, fp16, (c.flags & CUPHY_LDPC_DECODE_EARLY_TERM) != 0);
+    g.func = select_kernel(c.BG, c.llr_type, fp16);
     if (!g.func) return CUPHY_STATUS_UNSUPPORTED_CONFIG;
     const int nsys = (c.BG == 1) ? 22 : 10;
     g.lds_bytes = (size_t)(nsys + c.num_parity_nodes) * (size_t)c.Z * (fp16 ? 2u : 4u);
@@ -330,13 +399,24 @@ cuphyStatus_t fill_launch(const cuphyLDPCDecodeDesc_t& desc, launch_geometry& g)
     return CUPHY_STATUS_SUCCESS;
 }
 
+// Raise a kernel's dynamic-LDS limit above the 64 KB default, once per
+// (kernel, size) high-water mark.
 static cuphyStatus_t ensure_lds_attr(void* func, size_t bytes)
 {
     static std::mutex mu;
-    std::lock_guard<std::mutex> g(mu);
+    static std::vector<std::pair<void*, size_t>> done;
     if (bytes <= 65536) return CUPHY_STATUS_SUCCESS;
+    std::lock_guard<std::mutex> g(mu);
+    for (auto& d : done)
+        if (d.first == func && d.second >= bytes) return CUPHY_STATUS_SUCCESS;
     if (hipFuncSetAttribute(func, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess)
         return CUPHY_STATUS_INTERNAL_ERROR;
+    for (auto& d : done)
+        if (d.first == func) {
+            d.second = bytes;
+            return CUPHY_STATUS_SUCCESS;
+        }
+    done.emplace_back(func, bytes);
     return CUPHY_STATUS_SUCCESS;
 }
 
@@ -349,6 +429,38 @@ cuphyStatus_t decode_tb(const cuphyLDPCDecodeDesc_t& desc, hipStream_t strm)
     if ((s = ensure_lds_attr(g.func, g.lds_bytes)) != CUPHY_STATUS_SUCCESS) return s;
     void* args[1] = {const_cast<cuphyLDPCDecodeDesc_t*>(&desc)};
     if (hipLaunchKernel(g.func, g.grid, g.block, args, g.lds_bytes, strm) != hipSuccess)
+        return CUPHY_STATUS_INTERNAL_ERROR;
+    return CUPHY_STATUS_SUCCESS;
+}
+
+cuphyStatus_t decode_batch(const cuphyHipLDPCWorkItem_t* d_items, int n_items, cuphyDataType_t llr_type,
+                           bool fp16_math, int max_values_per_cb, uint32_t* d_queue, int cu_count, hipStream_t strm)
+{
+    if (n_items < 0 || !d_queue || (n_items > 0 && !d_items)) return CUPHY_STATUS_INVALID_ARGUMENT;
+    if (max_values_per_cb <= 0 || max_values_per_cb > 68 * 384) return CUPHY_STATUS_INVALID_ARGUMENT;
+    if (n_items == 0) return CUPHY_STATUS_SUCCESS;
+    void* func = nullptr;
+    size_t elem = 0;
+    if (llr_type == CUPHY_R_32F && !fp16_math) {
+        func = (void*)&ldpc_layered_persistent<float, float>;
+        elem = 4;
+    } else if (llr_type == CUPHY_R_32F) {
+        func = (void*)&ldpc_layered_persistent<_Float16, float>;
+        elem = 2;
+    } else if (llr_type == CUPHY_R_16F) {
+        func = (void*)&ldpc_layered_persistent<_Float16, _Float16>;
+        elem = 2;
+    } else {
+        return CUPHY_STATUS_UNSUPPORTED_TYPE;
+    }
+    const size_t lds = (size_t)max_values_per_cb * elem;
+    cuphyStatus_t s = ensure_lds_attr(func, lds);
+    if (s != CUPHY_STATUS_SUCCESS) return s;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, func, 384, lds) != hipSuccess || per_cu < 1) per_cu = 1;
+    const int grid = std::min(n_items, std::max(1, cu_count) * per_cu);
+    void* args[3] = {const_cast<cuphyHipLDPCWorkItem_t**>(&d_items), &n_items, &d_queue};
+    if (hipLaunchKernel(func, dim3((unsigned)grid), dim3(384), args, lds, strm) != hipSuccess)
         return CUPHY_STATUS_INTERNAL_ERROR;
     return CUPHY_STATUS_SUCCESS;
 }

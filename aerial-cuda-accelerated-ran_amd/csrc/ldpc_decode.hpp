@@ -28,4 +28,8 @@ cuphyStatus_t decode_tb(const cuphyLDPCDecodeDesc_t& desc, hipStream_t strm);
 // Populate a kernel-node launch configuration (graph capture / driver launch).
 cuphyStatus_t get_launch_config(cuphyLDPCDecodeLaunchConfig_t& lc);
 
+// Enqueue one persistent launch over a heterogeneous batch of work items.
+cuphyStatus_t decode_batch(const cuphyHipLDPCWorkItem_t* d_items, int n_items, cuphyDataType_t llr_type,
+                           bool fp16_math, int max_values_per_cb, uint32_t* d_queue, int cu_count, hipStream_t strm);
+
 }  // namespace nrldpc

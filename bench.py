@@ -134,13 +134,41 @@ def main():
         bit_errs += int(diff.sum().item())
         blk_errs += int(diff.any(dim=1).sum().item())
 
+    # ---- batched form: one persistent launch per slot, graph-replayed -------
+    from aerial_amd.batch import BatchDecoder
+    for tb in tbs:
+        tb.out.zero_()
+    bdec = BatchDecoder(dec).build(tbs, llr_type=L.CUPHY_R_16F, fp16_math=True, max_iterations=ITERS,
+                                   alpha=0.8125, clamp_value=32.0)
+    bdec.capture()
+    for _ in range(a.warmup):
+        bdec.replay()
+    torch.cuda.synchronize()
+    bev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(a.steps)]
+    gstream = torch.cuda.current_stream()
+    for s_ in range(a.steps):
+        bev[s_][0].record(gstream)
+        bdec.replay()
+        bev[s_][1].record(gstream)
+    torch.cuda.synchronize()
+    batched_ms = sum(e0.elapsed_time(e1) for e0, e1 in bev) / a.steps
+    batched_gbps = info_bits / (batched_ms * 1e-3) / 1e9
+    bbit_errs = 0
+    for tb in tbs:
+        bits = tb.out.view(torch.int32)
+        got = ((bits.unsqueeze(-1) >> torch.arange(32, device="cuda", dtype=torch.int32)) & 1).reshape(tb.ncb, -1)
+        bbit_errs += int((got[:, :tb.info_bits_per_cb].to(torch.uint8) != tb.info).sum().item())
+
     # ---- roofline of the decoder kernel (VALU-bound) ----------------------
-    achieved_tops = edge_updates * OPS_PER_EDGE / (kernel_ms * 1e-3) / 1e12
+    # dominant kernel = the batched persistent decoder: one launch per step
+    achieved_tops = edge_updates * OPS_PER_EDGE / (batched_ms * 1e-3) / 1e12
+    plain_tops = edge_updates * OPS_PER_EDGE / (kernel_ms * 1e-3) / 1e12
     roofline = {"bound": "valu", "achieved": round(achieved_tops, 3), "peak": round(VALU_PEAK_TOPS, 2),
                 "unit": "TOP/s", "frac": round(achieved_tops / VALU_PEAK_TOPS, 4), "traffic": None,
-                "kernel": "nrldpc::ldpc_layered_tb<BG, _Float16, _Float16, false>",
-                "work": "%d edge updates x %d VALU ops per step" % (edge_updates, OPS_PER_EDGE),
-                "avg_launch_us": round(kernel_ms * 1e3 / len(descs), 2)}
+                "kernel": "nrldpc::ldpc_layered_persistent<_Float16, _Float16>",
+                "work": "%d edge updates x %d VALU ops per launch" % (edge_updates, OPS_PER_EDGE),
+                "avg_launch_us": round(batched_ms * 1e3, 2),
+                "plain_kernel": {"achieved": round(plain_tops, 3), "avg_launch_us": round(kernel_ms * 1e3 / len(descs), 2)}}
 
     # ---- configuration 1 (plain, binary32): GPU time -----------------------
     c1 = W.config1_tb(seed=1)
@@ -164,12 +192,12 @@ def main():
 
     line = {
         "metric": METRIC,
-        "value": round(plain_gbps, 4),
+        "value": round(batched_gbps, 4),
         "unit": "Gbps",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(plain_ms, 4),
+        "ms_per_step": round(batched_ms, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -177,12 +205,16 @@ def main():
         "data": "synthetic (seeded device RNG, HIP-encoded code blocks, BPSK-AWGN)",
         "config": {"workload": "heterogeneous slot: 16 UEs, BG1/BG2, Zc 64..384, R 0.2..0.93, 1-24 CBs/TB, "
                                "Es/N0 -1..8 dB, fp16 LLRs, 10 iterations",
-                   "form": "plain (one launch per TB)", "code_blocks": ncb, "info_bits": info_bits,
+                   "form": "batched (one persistent launch per slot, graph replay)", "code_blocks": ncb,
+                   "info_bits": info_bits,
                    "tbs": len(tbs), "parallelism": "1 GPU"},
-        "forms": {"plain_gbps": round(plain_gbps, 4), "batched_gbps": None},
+        "forms": {"plain_gbps": round(plain_gbps, 4), "plain_ms": round(plain_ms, 4),
+                  "batched_gbps": round(batched_gbps, 4), "batched_ms": round(batched_ms, 4),
+                  "batched_over_plain": round(batched_gbps / plain_gbps, 3)},
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "check": {"bit_errors": bit_errs, "block_errors": blk_errs, "code_blocks": ncb},
+        "check": {"plain_bit_errors": bit_errs, "plain_block_errors": blk_errs, "batched_bit_errors": bbit_errs,
+                  "code_blocks": ncb},
         "config1_plain_fp32": {"us_per_tb": round(c1_us, 2), "gbps": round(c1_gbps, 4)},
     }
     print(json.dumps(line))

@@ -344,6 +344,41 @@ cuphyStatus_t CUPHYWINAPI cuphyHipLDPCEncodeCodeBlocks(int BG, int Z, const uint
                                                        int num_code_blocks, uint8_t* cw, int cw_stride,
                                                        hipStream_t strm);
 
+/* One code block of a heterogeneous batch (any BG, Z, mb per item).  The
+ * batch form has no cuphy.h counterpart: the reference groups TBs of equal
+ * (BG, Z, mb) into descriptors of <= 32 TBs (pusch_rx.cpp:6275-6388) and
+ * launches one kernel per group. */
+typedef struct {
+    const void* llr;                 /* (nsys + mb) * Z LLRs of the batch's llr_type, 2Z punctured first */
+    uint32_t* out;                   /* Kb * Z hard bits, LSB-first 32-bit words */
+    void* soft;                      /* optional Kb * Z binary16 soft outputs, NULL for none */
+    int16_t BG;
+    int16_t Z;
+    int16_t num_parity_nodes;        /* mb */
+    int16_t Kb;
+    int16_t max_iterations;
+    uint16_t flags;                  /* CUPHY_LDPC_DECODE_EARLY_TERM */
+    cuphyLDPCNormalization_t norm;   /* f32, or f16x2 when the batch computes in binary16 */
+    float clamp_value;
+    uint32_t reserved;
+} cuphyHipLDPCWorkItem_t;
+
+/* Workspace of cuphyHipLDPCBatchDecode: the device work-queue state.  The
+ * caller zero-fills it once; every completed launch leaves it zeroed, so
+ * launches can be replayed from a captured graph.  One workspace per
+ * concurrently running batch. */
+cuphyStatus_t CUPHYWINAPI cuphyHipLDPCBatchDecodeGetWorkspaceSize(size_t* sizeInBytes);
+
+/* Decode num_items code blocks (device array d_items) with one persistent
+ * launch: each workgroup takes the next item from the device queue until
+ * the batch is drained.  llr_type: CUPHY_R_32F or CUPHY_R_16F for every
+ * item; fp16_math: binary16 arithmetic (always for CUPHY_R_16F);
+ * max_values_per_cb: max over items of (nsys + mb) * Z (sizes LDS); order
+ * the items longest first for the shortest makespan. */
+cuphyStatus_t CUPHYWINAPI cuphyHipLDPCBatchDecode(cuphyLDPCDecoder_t decoder, const cuphyHipLDPCWorkItem_t* d_items,
+                                                  int num_items, cuphyDataType_t llr_type, int fp16_math,
+                                                  int max_values_per_cb, void* d_workspace, hipStream_t strm);
+
 #ifdef __cplusplus
 }
 #endif
