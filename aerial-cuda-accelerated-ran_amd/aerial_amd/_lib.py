@@ -118,6 +118,16 @@ class TensorPrm(C.Structure):
     _fields_ = [("desc", C.c_void_p), ("pAddr", C.c_void_p)]
 
 
+class RateMatchLaunchCfg(C.Structure):
+    _fields_ = [("kernelNodeParamsDriver", HipKernelNodeParams), ("resetKernelNodeParamsDriver", HipKernelNodeParams),
+                ("clampKernelNodeParamsDriver", HipKernelNodeParams), ("desc", C.c_void_p),
+                ("kernelArgs", C.c_void_p * 1)]
+
+
+class CrcDecodeLaunchCfg(C.Structure):
+    _fields_ = [("kernelNodeParamsDriver", HipKernelNodeParams), ("desc", C.c_void_p), ("kernelArgs", C.c_void_p * 1)]
+
+
 # Every entry point declared in include/cuphy_hip.h, with its ctypes signature.
 _P = C.POINTER
 _VP = C.c_void_p

@@ -21,6 +21,8 @@ ARCH = os.environ.get("NRLDPC_OFFLOAD_ARCH", "gfx950")
 SOURCES = [
     "ldpc_decode.hip",
     "ldpc_encode.hip",
+    "derate_match.hip",
+    "crc_decode.hip",
     "cuphy_api.cpp",
 ]
 

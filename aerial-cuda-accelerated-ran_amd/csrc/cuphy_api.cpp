@@ -12,6 +12,8 @@
 #include "cuphy_hip.h"
 #include "ldpc_decode.hpp"
 #include "ldpc_encode.hpp"
+#include "derate_match.hpp"
+#include "crc_decode.hpp"
 
 // ---------------------------------------------------------------------------
 // context (cuphy.h:1156-1208): records the device it was created on
@@ -41,7 +43,27 @@ struct cuphyLDPCDecoder {
     int cu_count = 0;
 };
 
+// ---------------------------------------------------------------------------
+// PUSCH de-rate-match and CRC-decode objects (cuphy.h:3349-3490, 3154-3293)
+// ---------------------------------------------------------------------------
+struct rm_object : cuphyPuschRxRateMatch {
+    int fpconfig = 0;
+};
+struct crc_object : cuphyPuschRxCrcDecode {
+    int reverse_bytes = 1;
+};
+
 namespace {
+
+void set_node(hipKernelNodeParams& n, void* func, dim3 grid, dim3 block, void** args)
+{
+    n.func = func;
+    n.gridDim = grid;
+    n.blockDim = block;
+    n.sharedMemBytes = 0;
+    n.kernelParams = args;
+    n.extra = nullptr;
+}
 
 size_t type_bytes(cuphyDataType_t t)
 {
@@ -390,6 +412,151 @@ cuphyStatus_t CUPHYWINAPI cuphyErrorCorrectionLDPCDecodeGetLaunchDescriptor(cuph
 {
     if (!decoder || !launchConfig) return CUPHY_STATUS_INVALID_ARGUMENT;
     return nrldpc::get_launch_config(*launchConfig);
+}
+
+// ---------------------------------------------------------------------------
+// PUSCH de-rate-matching (cuphy_ldpc.cpp:228-307, rate_matching.cu:982-1097)
+// ---------------------------------------------------------------------------
+cuphyStatus_t CUPHYWINAPI cuphyPuschRxRateMatchGetDescrInfo(size_t* pDescrSizeBytes, size_t* pDescrAlignBytes)
+{
+    if (!pDescrSizeBytes || !pDescrAlignBytes) return CUPHY_STATUS_INVALID_ARGUMENT;
+    *pDescrSizeBytes = sizeof(nrldpc::rm_descriptor);
+    *pDescrAlignBytes = alignof(nrldpc::rm_descriptor);
+    return CUPHY_STATUS_SUCCESS;
+}
+
+cuphyStatus_t CUPHYWINAPI cuphyCreatePuschRxRateMatch(cuphyPuschRxRateMatchHndl_t* pPuschRxRateMatchHndl, int FPconfig,
+                                                      int descramblingOn)
+{
+    if (!pPuschRxRateMatchHndl) return CUPHY_STATUS_INVALID_ARGUMENT;
+    *pPuschRxRateMatchHndl = nullptr;
+    if (descramblingOn) return CUPHY_STATUS_NOT_SUPPORTED;
+    if (!nrldpc::rm_kernel(FPconfig)) return CUPHY_STATUS_INVALID_ARGUMENT;
+    rm_object* o = new (std::nothrow) rm_object;
+    if (!o) return CUPHY_STATUS_ALLOC_FAILED;
+    o->fpconfig = FPconfig;
+    *pPuschRxRateMatchHndl = o;
+    return CUPHY_STATUS_SUCCESS;
+}
+
+cuphyStatus_t CUPHYWINAPI cuphySetupPuschRxRateMatch(cuphyPuschRxRateMatchHndl_t puschRxRateMatchHndl, uint16_t nSchUes,
+                                                     uint16_t* pSchUserIdxsCpu, const PerTbParams* pTbPrmsCpu,
+                                                     const PerTbParams* pTbPrmsGpu, cuphyTensorPrm_t* pTPrmRmIn,
+                                                     cuphyTensorPrm_t* pTPrmCdm1RmIn, void** ppRmOut, void* pCpuDesc,
+                                                     void* pGpuDesc, uint8_t enableCpuToGpuDescrAsyncCpy,
+                                                     cuphyPuschRxRateMatchLaunchCfg_t* pLaunchCfg, hipStream_t strm)
+{
+    if (!puschRxRateMatchHndl || !pTbPrmsCpu || !pTbPrmsGpu || !pTPrmRmIn || !pTPrmCdm1RmIn || !ppRmOut || !pCpuDesc ||
+        !pGpuDesc || !pLaunchCfg || !pSchUserIdxsCpu || nSchUes > MAX_N_TBS_SUPPORTED)
+        return CUPHY_STATUS_INVALID_ARGUMENT;
+    const rm_object& o = static_cast<const rm_object&>(*puschRxRateMatchHndl);
+    nrldpc::rm_descriptor& d = *static_cast<nrldpc::rm_descriptor*>(pCpuDesc);
+    uint32_t cmax = 0;
+    for (uint32_t i = 0; i < nSchUes; ++i) {
+        const uint16_t ue = pSchUserIdxsCpu[i];
+        const PerTbParams& p = pTbPrmsCpu[ue];
+        if (p.Qm != 1 && p.Qm != 2 && p.Qm != 4 && p.Qm != 6 && p.Qm != 8) return CUPHY_STATUS_INVALID_ARGUMENT;
+        d.sch_user_idxs[i] = ue;
+        if (p.uciOnPuschFlag)
+            d.llr_in[i] = p.d_schAndCsi2LLRs;
+        else
+            d.llr_in[i] = (p.nDmrsCdmGrpsNoData == 1 ? pTPrmCdm1RmIn : pTPrmRmIn)[p.userGroupIndex].pAddr;
+        cmax = std::max(cmax, p.num_CBs);
+    }
+    d.out = ppRmOut;
+    d.tb_prms = pTbPrmsGpu;
+    d.descrambling_on = 0;
+    if (enableCpuToGpuDescrAsyncCpy &&
+        hipMemcpyAsync(pGpuDesc, pCpuDesc, sizeof(nrldpc::rm_descriptor), hipMemcpyHostToDevice, strm) != hipSuccess)
+        return CUPHY_STATUS_MEMCPY_ERROR;
+    pLaunchCfg->desc = pGpuDesc;
+    pLaunchCfg->kernelArgs[0] = &pLaunchCfg->desc;
+    set_node(pLaunchCfg->kernelNodeParamsDriver, nrldpc::rm_kernel(o.fpconfig), dim3(std::max(1u, cmax), std::max<unsigned>(1, nSchUes), 1),
+             dim3(nrldpc::RM_BLOCK), pLaunchCfg->kernelArgs);
+    set_node(pLaunchCfg->resetKernelNodeParamsDriver, nrldpc::rm_noop_kernel(), dim3(1), dim3(64), pLaunchCfg->kernelArgs);
+    set_node(pLaunchCfg->clampKernelNodeParamsDriver, nrldpc::rm_noop_kernel(), dim3(1), dim3(64), pLaunchCfg->kernelArgs);
+    if (nSchUes == 0) pLaunchCfg->kernelNodeParamsDriver.func = nrldpc::rm_noop_kernel();
+    return CUPHY_STATUS_SUCCESS;
+}
+
+cuphyStatus_t CUPHYWINAPI cuphyDestroyPuschRxRateMatch(cuphyPuschRxRateMatchHndl_t puschRxRateMatchHndl)
+{
+    if (!puschRxRateMatchHndl) return CUPHY_STATUS_INVALID_ARGUMENT;
+    delete static_cast<rm_object*>(puschRxRateMatchHndl);
+    return CUPHY_STATUS_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
+// PUSCH CB/TB CRC decode (cuphy_ldpc.cpp:312-392, crc.cu:1174-1254)
+// ---------------------------------------------------------------------------
+cuphyStatus_t CUPHYWINAPI cuphyPuschRxCrcDecodeGetDescrInfo(size_t* pDescrSizeBytes, size_t* pDescrAlignBytes)
+{
+    if (!pDescrSizeBytes || !pDescrAlignBytes) return CUPHY_STATUS_INVALID_ARGUMENT;
+    *pDescrSizeBytes = sizeof(nrldpc::crc_descriptor);
+    *pDescrAlignBytes = alignof(nrldpc::crc_descriptor);
+    return CUPHY_STATUS_SUCCESS;
+}
+
+cuphyStatus_t CUPHYWINAPI cuphyCreatePuschRxCrcDecode(cuphyPuschRxCrcDecodeHndl_t* puschRxCrcDecodeHndl, int reverseBytes)
+{
+    if (!puschRxCrcDecodeHndl) return CUPHY_STATUS_INVALID_ARGUMENT;
+    *puschRxCrcDecodeHndl = nullptr;
+    crc_object* o = new (std::nothrow) crc_object;
+    if (!o) return CUPHY_STATUS_ALLOC_FAILED;
+    o->reverse_bytes = reverseBytes ? 1 : 0;
+    *puschRxCrcDecodeHndl = o;
+    return CUPHY_STATUS_SUCCESS;
+}
+
+cuphyStatus_t CUPHYWINAPI cuphySetupPuschRxCrcDecode(cuphyPuschRxCrcDecodeHndl_t puschRxCrcDecodeHndl, uint16_t nSchUes,
+                                                     uint16_t* pSchUserIdxsCpu, uint32_t* pOutputCBCRCs,
+                                                     uint8_t* pOutputTBs, const uint32_t* pInputCodeBlocks,
+                                                     uint32_t* pOutputTBCRCs, const PerTbParams* pTbPrmsCpu,
+                                                     const PerTbParams* pTbPrmsGpu, void* pCpuDesc, void* pGpuDesc,
+                                                     uint8_t enableCpuToGpuDescrAsyncCpy,
+                                                     cuphyPuschRxCrcDecodeLaunchCfg_t* pCbCrcLaunchCfg,
+                                                     cuphyPuschRxCrcDecodeLaunchCfg_t* pTbCrcLaunchCfg, hipStream_t strm)
+{
+    if (!puschRxCrcDecodeHndl || !pOutputCBCRCs || !pOutputTBs || !pInputCodeBlocks || !pOutputTBCRCs || !pTbPrmsCpu ||
+        !pTbPrmsGpu || !pCpuDesc || !pGpuDesc || !pCbCrcLaunchCfg || !pTbCrcLaunchCfg || !pSchUserIdxsCpu ||
+        nSchUes > MAX_N_TBS_SUPPORTED)
+        return CUPHY_STATUS_INVALID_ARGUMENT;
+    const crc_object& o = static_cast<const crc_object&>(*puschRxCrcDecodeHndl);
+    nrldpc::crc_descriptor& d = *static_cast<nrldpc::crc_descriptor*>(pCpuDesc);
+    uint32_t cmax = 0;
+    for (uint32_t i = 0; i < nSchUes; ++i) {
+        d.sch_user_idxs[i] = pSchUserIdxsCpu[i];
+        const PerTbParams& p = pTbPrmsCpu[pSchUserIdxsCpu[i]];
+        if (p.num_CBs > MAX_N_CBS_PER_TB_SUPPORTED || p.K < p.F || p.K - p.F > MAX_DECODED_CODE_BLOCK_BIT_SIZE)
+            return CUPHY_STATUS_INVALID_ARGUMENT;
+        cmax = std::max(cmax, p.num_CBs);
+    }
+    d.cb_crc_out = pOutputCBCRCs;
+    d.tb_out = pOutputTBs;
+    d.cb_in = pInputCodeBlocks;
+    d.tb_crc_out = pOutputTBCRCs;
+    d.tb_prms = pTbPrmsGpu;
+    d.reverse_bytes = o.reverse_bytes;
+    if (enableCpuToGpuDescrAsyncCpy &&
+        hipMemcpyAsync(pGpuDesc, pCpuDesc, sizeof(nrldpc::crc_descriptor), hipMemcpyHostToDevice, strm) != hipSuccess)
+        return CUPHY_STATUS_MEMCPY_ERROR;
+    pCbCrcLaunchCfg->desc = pGpuDesc;
+    pCbCrcLaunchCfg->kernelArgs[0] = &pCbCrcLaunchCfg->desc;
+    set_node(pCbCrcLaunchCfg->kernelNodeParamsDriver, nrldpc::crc_cb_kernel_ptr(),
+             dim3(std::max(1u, cmax), std::max<unsigned>(1, nSchUes), 1), dim3(nrldpc::CRC_CB_BLOCK),
+             pCbCrcLaunchCfg->kernelArgs);
+    pTbCrcLaunchCfg->desc = pGpuDesc;
+    pTbCrcLaunchCfg->kernelArgs[0] = &pTbCrcLaunchCfg->desc;
+    set_node(pTbCrcLaunchCfg->kernelNodeParamsDriver, nrldpc::crc_tb_kernel_ptr(), dim3(std::max<unsigned>(1, nSchUes)),
+             dim3(nrldpc::CRC_TB_BLOCK), pTbCrcLaunchCfg->kernelArgs);
+    return CUPHY_STATUS_SUCCESS;
+}
+
+cuphyStatus_t CUPHYWINAPI cuphyDestroyPuschRxCrcDecode(cuphyPuschRxCrcDecodeHndl_t puschRxCrcDecodeHndl)
+{
+    if (!puschRxCrcDecodeHndl) return CUPHY_STATUS_INVALID_ARGUMENT;
+    delete static_cast<crc_object*>(puschRxCrcDecodeHndl);
+    return CUPHY_STATUS_SUCCESS;
 }
 
 // ---------------------------------------------------------------------------

@@ -1,0 +1,27 @@
+// Host/device interface of the PUSCH de-rate-match kernel (derate_match.hip).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include "cuphy_hip.h"
+
+namespace nrldpc {
+
+// Device descriptor built by cuphySetupPuschRxRateMatch (the reference's
+// puschRxRateMatchDescr, rate_matching.hpp:23-31).
+struct rm_descriptor {
+    const void* llr_in[MAX_N_TBS_SUPPORTED];  // input LLRs of scheduled TB slot i
+    uint16_t sch_user_idxs[MAX_N_TBS_SUPPORTED];
+    void** out;                               // HARQ buffer of each UE index
+    const PerTbParams* tb_prms;               // device copy, indexed by UE index
+    int descrambling_on;
+};
+
+// Kernel of a FP configuration (0: fp32->fp32, 1: fp16->fp32, 2: fp32->fp16,
+// 3: fp16->fp16), nullptr otherwise.
+void* rm_kernel(int fpconfig);
+void* rm_noop_kernel();
+
+constexpr unsigned RM_BLOCK = 256;
+
+}  // namespace nrldpc
