@@ -28,3 +28,18 @@ def test_gpu_encoder_matches_oracle(bg, Z):
         ref = O.encode(bg, Z, info[i])
         assert np.array_equal(got[i], ref)
         assert O.syndrome(bg, Z, 46 if bg == 1 else 42, got[i]) == 0
+
+
+def test_gpu_encoder_matches_reference_goldens():
+    """MATLAB encoder output shipped with the reference (BG1, Zc = 52)."""
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import os
+    from aerial_amd import workloads as W
+    print("[case] encoder reference goldens BG1 Z=52", flush=True)
+    f = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "encoder_goldens.npz"))
+    src = np.concatenate([f["src_1_0"], f["src_752_0"]])
+    enc = np.concatenate([f["enc_1_0"], f["enc_752_0"]])
+    cw = W.encode(1, 52, torch.from_numpy(src).cuda())
+    torch.cuda.synchronize()
+    assert np.array_equal(cw.cpu().numpy(), enc)
