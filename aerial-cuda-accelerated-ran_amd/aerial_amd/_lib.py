@@ -38,6 +38,7 @@ MAX_N_LAYERS_PUSCH = 8
 CUPHY_MAX_N_CSI1_PRMS = 4
 CUPHY_MAX_N_CSI2_REPORTS_PER_UE = 16
 MAX_WORDS_PER_CODE_BLOCK = 264
+SLOT_CB_ERROR = 0xFFFFFFFF  # CRC value of a CB/TB the fused slot kernel rejected
 
 
 class Norm(C.Union):
@@ -114,6 +115,14 @@ class LDPCWorkItem(C.Structure):
                 ("clamp_value", C.c_float), ("reserved", C.c_uint32)]
 
 
+class PuschSlotTb(C.Structure):
+    _fields_ = [("prm", C.c_void_p), ("llr", C.c_void_p), ("harq", C.c_void_p), ("cb_bits", C.c_void_p),
+                ("cb_crc", C.c_void_p), ("tb_bytes", C.c_void_p), ("tb_crc", C.c_void_p),
+                ("cb_iterations", C.c_void_p), ("num_parity_nodes", C.c_int16), ("max_iterations", C.c_int16),
+                ("flags", C.c_uint16), ("reserved", C.c_uint16), ("llr_scale", C.c_float), ("norm", Norm),
+                ("clamp_value", C.c_float)]
+
+
 class TensorPrm(C.Structure):
     _fields_ = [("desc", C.c_void_p), ("pAddr", C.c_void_p)]
 
@@ -161,7 +170,9 @@ SIGNATURES = {
     "cuphyDestroyPuschRxCrcDecode": ([_VP], _I),
     "cuphyHipLDPCEncodeCodeBlocks": ([_I, _I, _VP, _I, _I, _VP, _I, _VP], _I),
     "cuphyHipLDPCBatchDecodeGetWorkspaceSize": ([_P(C.c_size_t)], _I),
-    "cuphyHipLDPCBatchDecode": ([_VP, _VP, _I, _I, _I, _I, _VP, _VP], _I),
+    "cuphyHipLDPCBatchDecode": ([_VP, _VP, _I, _I, _I, _I, _I, _VP, _VP], _I),
+    "cuphyHipPuschSlotDecodeGetWorkspaceSize": ([_I, _P(C.c_size_t)], _I),
+    "cuphyHipPuschSlotDecode": ([_VP, _VP, _I, _VP, _I, _I, _I, _I, _VP, _VP], _I),
 }
 
 _lib = None

@@ -14,6 +14,7 @@
 #include "ldpc_encode.hpp"
 #include "derate_match.hpp"
 #include "crc_decode.hpp"
+#include "pusch_fused.hpp"
 
 // ---------------------------------------------------------------------------
 // context (cuphy.h:1156-1208): records the device it was created on
@@ -578,11 +579,29 @@ cuphyStatus_t CUPHYWINAPI cuphyHipLDPCBatchDecodeGetWorkspaceSize(size_t* sizeIn
 
 cuphyStatus_t CUPHYWINAPI cuphyHipLDPCBatchDecode(cuphyLDPCDecoder_t decoder, const cuphyHipLDPCWorkItem_t* d_items,
                                                   int num_items, cuphyDataType_t llr_type, int fp16_math,
-                                                  int max_values_per_cb, void* d_workspace, hipStream_t strm)
+                                                  int max_values_per_cb, int max_parity_nodes, void* d_workspace,
+                                                  hipStream_t strm)
 {
-    if (!decoder || !d_workspace) return CUPHY_STATUS_INVALID_ARGUMENT;
-    return nrldpc::decode_batch(d_items, num_items, llr_type, fp16_math != 0, max_values_per_cb,
+    if (!decoder || !d_workspace || max_parity_nodes < 4 || max_parity_nodes > 46) return CUPHY_STATUS_INVALID_ARGUMENT;
+    return nrldpc::decode_batch(d_items, num_items, llr_type, fp16_math != 0, max_values_per_cb, max_parity_nodes,
                                 static_cast<uint32_t*>(d_workspace), decoder->cu_count, strm);
+}
+
+cuphyStatus_t CUPHYWINAPI cuphyHipPuschSlotDecodeGetWorkspaceSize(int num_tbs, size_t* sizeInBytes)
+{
+    if (!sizeInBytes || num_tbs < 0 || num_tbs > 65536) return CUPHY_STATUS_INVALID_ARGUMENT;
+    *sizeInBytes = (size_t)(nrldpc::FUSED_WS_QUEUE_WORDS + num_tbs) * sizeof(uint32_t);
+    return CUPHY_STATUS_SUCCESS;
+}
+
+cuphyStatus_t CUPHYWINAPI cuphyHipPuschSlotDecode(cuphyLDPCDecoder_t decoder, const cuphyHipPuschSlotTb_t* d_tbs,
+                                                  int num_tbs, const uint32_t* d_cb_items, int num_cb_items,
+                                                  cuphyDataType_t llr_type, int max_lds_values, int max_parity_nodes,
+                                                  void* d_workspace, hipStream_t strm)
+{
+    if (!decoder) return CUPHY_STATUS_INVALID_ARGUMENT;
+    return nrldpc::slot_decode(d_tbs, num_tbs, d_cb_items, num_cb_items, llr_type, max_lds_values, max_parity_nodes,
+                               d_workspace, decoder->cu_count, strm);
 }
 
 }  // extern "C"

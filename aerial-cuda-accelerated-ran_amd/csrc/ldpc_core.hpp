@@ -1,0 +1,365 @@
+// Device core of the layered normalized min-sum decoder (gfx950), shared by
+// the transport-block, persistent-batch and fused-slot kernels.
+#pragma once
+
+#include "nr_ldpc_common.hpp"
+
+namespace nrldpc {
+
+// Row capacity of the high-code-rate kernel variants (mb <= 8).
+constexpr int SMALL_MAXR = 8;
+
+// Reduced shifts of every (BG, Z) in device memory, constant-initialised in
+// the code object.  Read through an address-space-4 pointer so that the
+// uniform loads become scalar (s_load) loads.
+static __device__ const reduced_shift_tables g_shift_tables = make_reduced_shift_tables();
+typedef const __attribute__((address_space(4))) uint32_t* shift_ptr;
+
+// Position of every lifting size among the 51 (index Z, -1 for other
+// values), as a constant table: a runtime search of a host constexpr array
+// may be placed in per-lane memory, whose loads count as divergent.
+struct lifting_index_table {
+    int32_t v[385];
+};
+constexpr lifting_index_table make_lifting_index_table()
+{
+    lifting_index_table t{};
+    for (int z = 0; z <= 384; ++z) t.v[z] = lifting_size_index(z);
+    return t;
+}
+static __device__ const lifting_index_table g_lifting_index = make_lifting_index_table();
+
+// Index of Z among the lifting sizes, -1 if Z is not one (uniform load).
+__device__ __forceinline__ int z_index(int Z)
+{
+    typedef const __attribute__((address_space(4))) int32_t* cptr;
+    return ((unsigned)Z <= 384u) ? ((cptr)(const void*)g_lifting_index.v)[Z] : -1;
+}
+
+template <typename T> struct num;
+template <> struct num<float> {
+    static __device__ __forceinline__ float vmin(float a, float b) { return __builtin_fminf(a, b); }
+    static __device__ __forceinline__ float vmax(float a, float b) { return __builtin_fmaxf(a, b); }
+    static __device__ __forceinline__ float vabs(float a) { return __builtin_fabsf(a); }
+    static __device__ __forceinline__ float inf() { return __builtin_huge_valf(); }
+};
+template <> struct num<_Float16> {
+    static __device__ __forceinline__ _Float16 vmin(_Float16 a, _Float16 b) { return __builtin_fminf16(a, b); }
+    static __device__ __forceinline__ _Float16 vmax(_Float16 a, _Float16 b) { return __builtin_fmaxf16(a, b); }
+    static __device__ __forceinline__ _Float16 vabs(_Float16 a) { return __builtin_fabsf16(a); }
+    static __device__ __forceinline__ _Float16 inf() { return (_Float16)__builtin_huge_valf(); }
+};
+
+template <typename TIn, typename T> struct llr_in;
+template <> struct llr_in<float, float> {
+    static __device__ __forceinline__ float cvt(float v) { return v; }
+};
+template <> struct llr_in<float, _Float16> {
+    static __device__ __forceinline__ _Float16 cvt(float v) { return (_Float16)v; }
+};
+template <> struct llr_in<_Float16, _Float16> {
+    static __device__ __forceinline__ _Float16 cvt(_Float16 v) { return v; }
+};
+
+// Locate code word `cw` of the launch in the TB list (the reference's
+// decode_desc_output_addr, ldpc2_dec_output.cuh:58-81).
+struct cw_location {
+    int tb, idx;
+};
+__device__ __forceinline__ cw_location locate_cw(const cuphyLDPCDecodeDesc_t& d, int cw)
+{
+    int tb = 0;
+    for (; tb < d.num_tbs - 1; ++tb) {
+        if (cw < d.llr_input[tb].num_codewords) break;
+        cw -= d.llr_input[tb].num_codewords;
+    }
+    return {tb, cw};
+}
+
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+
+template <int BG> constexpr uint64_t kBarrierMask = barrier_mask<bg_traits<BG>>();
+
+__device__ __forceinline__ uint32_t f2u(float x) { return __builtin_bit_cast(uint32_t, x); }
+__device__ __forceinline__ float u2f(uint32_t x) { return __builtin_bit_cast(float, x); }
+__device__ __forceinline__ uint16_t h2u(_Float16 x) { return __builtin_bit_cast(uint16_t, x); }
+__device__ __forceinline__ _Float16 u2h(uint16_t x) { return __builtin_bit_cast(_Float16, x); }
+
+// (z + s) mod Z for z, s < Z: the smaller of p and p - Z as unsigned.
+__device__ __forceinline__ uint32_t rot_index(uint32_t z, uint32_t s, uint32_t Z)
+{
+    const uint32_t p = z + s;
+    const uint32_t q = p - Z;
+    return q < p ? q : p;
+}
+
+// ---------------------------------------------------------------------------
+// binary32 rows: compressed C2V per (row, lane) -- normalised min1/min2 and a
+// word holding the argmin (bits 27..31) and the C2V sign of every edge
+// (edge j at bit D-1-j).  Per edge: address 3, previous-message rebuild 4,
+// subtract 1, min/min2/parity 3, new message + write-back 6 VALU ops.
+// ---------------------------------------------------------------------------
+template <int BG, int R, int NR>
+__device__ __forceinline__ void process_row(float* __restrict__ app, shift_ptr sh, uint32_t z, uint32_t Z, float alpha,
+                                            float (&st_n1)[NR], float (&st_n2)[NR], uint32_t (&st_meta)[NR])
+{
+    using BGT = bg_traits<BG>;
+    constexpr int E0 = BGT::row_start(R);
+    constexpr int D = BGT::row_start(R + 1) - E0;
+    float t[D];
+    uint32_t off[D];
+    const float on1 = st_n1[R], on2 = st_n2[R];
+    const uint32_t om = st_meta[R];
+    const uint32_t oidx = om >> 27;
+    static_for<D>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        off[j] = (uint32_t)BGT::col(E0 + j) * Z + rot_index(z, sh[E0 + j], Z);
+        const float mag = (oidx == (uint32_t)j) ? on2 : on1;
+        const float old = u2f(f2u(mag) | ((om << (32 - D + j)) & 0x80000000u));
+        t[j] = app[off[j]] - old;
+    });
+    float m1 = __builtin_huge_valf(), m2 = __builtin_huge_valf();
+    uint32_t par = 0;
+    static_for<D>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        const float a = __builtin_fabsf(t[j]);
+        m2 = __builtin_amdgcn_fmed3f(m1, a, m2);  // min(m2, max(m1, a)); m1 <= m2
+        m1 = __builtin_fminf(m1, a);
+        par ^= f2u(t[j]);
+    });
+    par &= 0x80000000u;
+    const float a1 = alpha * m1, a2 = alpha * m2;
+    uint32_t idx = 0, fs = 0;
+    static_for<D>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        // |t| == min1 marks the argmin; on a tie min2 == min1, so every tied
+        // edge gets the same value as the first-index rule gives it
+        const bool is_min = __builtin_fabsf(t[j]) == m1;
+        const float mag = is_min ? a2 : a1;
+        idx = is_min ? (uint32_t)j : idx;
+        const uint32_t cb = f2u(mag) | ((f2u(t[j]) ^ par) & 0x80000000u);
+        app[off[j]] = t[j] + u2f(cb);
+        fs = __builtin_amdgcn_alignbit(fs, cb, 31);  // (fs << 1) | sign(c_j)
+    });
+    st_n1[R] = a1;
+    st_n2[R] = a2;
+    st_meta[R] = fs | (idx << 27);
+}
+
+// ---------------------------------------------------------------------------
+// binary16 rows: the normalised C2V message of every edge kept explicitly,
+// two edges per VGPR (BG1: 158 registers).  Per edge: address 3, subtract
+// 1, min/min2/parity 3, new message + write-back 5 VALU ops; no argmin.
+// ---------------------------------------------------------------------------
+template <int BG, int R, int NW>
+__device__ __forceinline__ void process_row(_Float16* __restrict__ app, shift_ptr sh, uint32_t z, uint32_t Z,
+                                            _Float16 alpha, half2_t (&st)[NW])
+{
+    using BGT = bg_traits<BG>;
+    constexpr int E0 = BGT::row_start(R);
+    constexpr int D = BGT::row_start(R + 1) - E0;
+    _Float16 t[D];
+    uint32_t off[D];
+    static_for<D>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        constexpr int e = E0 + j;
+        off[j] = (uint32_t)BGT::col(e) * Z + rot_index(z, sh[e], Z);
+        const _Float16 old = (e & 1) ? st[e >> 1].y : st[e >> 1].x;
+        t[j] = app[off[j]] - old;
+    });
+    _Float16 m1 = (_Float16)__builtin_huge_valf(), m2 = (_Float16)__builtin_huge_valf();
+    uint32_t par = 0;
+    static_for<D>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        const _Float16 a = __builtin_fabsf16(t[j]);
+        m2 = __builtin_amdgcn_fmed3h(m1, a, m2);
+        m1 = __builtin_fminf16(m1, a);
+        par ^= h2u(t[j]);
+    });
+    par &= 0x8000u;
+    const _Float16 a1 = alpha * m1, a2 = alpha * m2;
+    static_for<D>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        constexpr int e = E0 + j;
+        const _Float16 mag = (__builtin_fabsf16(t[j]) == m1) ? a2 : a1;
+        const _Float16 c = u2h((uint16_t)(h2u(mag) | ((h2u(t[j]) ^ par) & 0x8000u)));
+        app[off[j]] = t[j] + c;
+        if (e & 1)
+            st[e >> 1].y = c;
+        else
+            st[e >> 1].x = c;
+    });
+}
+
+// Parity of check row R, lane z: hard bit = APP < 0 (syndrome_check.m:38-40).
+template <int BG, int R, typename T>
+__device__ __forceinline__ uint32_t row_syndrome(const T* __restrict__ app, shift_ptr sh, uint32_t z, uint32_t Z)
+{
+    using BGT = bg_traits<BG>;
+    constexpr int E0 = BGT::row_start(R);
+    constexpr int D = BGT::row_start(R + 1) - E0;
+    uint32_t par = 0;
+    static_for<D>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        par ^= (uint32_t)(app[(uint32_t)BGT::col(E0 + j) * Z + rot_index(z, sh[E0 + j], Z)] < (T)0);
+    });
+    return par;
+}
+
+// Everything one code block's decode needs, resolved from a TB descriptor
+// (transport-block kernel) or from a batch work item (persistent kernel).
+struct cb_job {
+    const void* llr;
+    uint32_t* out;
+    _Float16* soft;  // nullptr: no soft output
+    int Z, mb, Kb, max_iter;
+    bool early_term;
+    float clamp;
+    float alpha_f32;
+    uint16_t alpha_f16;
+};
+
+// Phase 1 of a code block's decode: LLRs (global, type TIn) -> LDS `app`
+// (type T), converted, clamped to +-clamp (clamp > 0) and with a signed zero
+// made +0.  Every thread of the workgroup calls it; ends with a barrier.
+template <typename T, typename TIn>
+__device__ __forceinline__ void load_llrs(T* __restrict__ app, const TIn* __restrict__ src, int nval, float clamp)
+{
+    const T clampv = (T)clamp;
+    const bool do_clamp = clamp > 0.0f;
+    for (int n = threadIdx.x; n < nval; n += blockDim.x) {
+        T v = llr_in<TIn, T>::cvt(src[n]);
+        if (do_clamp) v = num<T>::vmax(num<T>::vmin(v, clampv), -clampv);
+        app[n] = (v == (T)0) ? (T)0 : v;
+    }
+    __syncthreads();
+}
+
+template <typename T> __device__ __forceinline__ T job_alpha(const cb_job& job);
+template <> __device__ __forceinline__ float job_alpha<float>(const cb_job& job) { return job.alpha_f32; }
+template <> __device__ __forceinline__ _Float16 job_alpha<_Float16>(const cb_job& job)
+{
+    return __builtin_bit_cast(_Float16, job.alpha_f16);
+}
+
+// Phase 2: the layered iterations over the APP values in LDS (already
+// loaded).  Returns the number of iterations run (< max_iter only with
+// early termination).  Every thread of the workgroup calls it.
+// MAXR: rows the register state is sized for (mb <= MAXR); a small MAXR
+// (high code rates) needs far fewer VGPRs and so allows more resident waves.
+template <int BG, typename T, int MAXR>
+__device__ __forceinline__ int run_iterations(T* __restrict__ app, const cb_job& job)
+{
+    using BGT = bg_traits<BG>;
+    constexpr int NR = (MAXR < BGT::NROWS) ? MAXR : BGT::NROWS;
+    const uint32_t Z = (uint32_t)job.Z;
+    const int mb = job.mb;
+    const uint32_t tid = threadIdx.x;
+    const T alpha = job_alpha<T>(job);
+
+    // C2V state: compressed per row (binary32) or explicit per edge (binary16)
+    constexpr int NW = (BGT::row_start(NR) + 1) / 2;
+    float st_n1[std::is_same<T, float>::value ? NR : 1], st_n2[std::is_same<T, float>::value ? NR : 1];
+    uint32_t st_meta[std::is_same<T, float>::value ? NR : 1];
+    half2_t st_h[std::is_same<T, float>::value ? 1 : NW];
+    if constexpr (std::is_same<T, float>::value) {
+        static_for<NR>([&](auto R) {
+            st_n1[R] = 0.0f;
+            st_n2[R] = 0.0f;
+            st_meta[R] = 0;
+        });
+    } else {
+        static_for<NW>([&](auto W) { st_h[W] = half2_t{(_Float16)0, (_Float16)0}; });
+    }
+
+    const bool active = tid < Z;
+    shift_ptr sh = (shift_ptr)(const void*)&g_shift_tables.v[BG - 1][z_index((int)Z)][0];
+    int it = 0;
+    while (it < job.max_iter) {
+        ++it;
+        // Opaque copies of the lane index and of the shift pointer, renewed
+        // per row: the 316 (BG1) per-edge shifts and addresses are reloaded
+        // and recomputed where used instead of being hoisted out of the
+        // loops into (spilled) registers.
+        uint32_t zl = tid;
+        asm volatile("" : "+v"(zl));
+        static_for<NR>([&](auto Rc) {
+            constexpr int R = decltype(Rc)::value;
+            if (R < mb) {
+                shift_ptr shr = sh;
+                uint32_t Zr = Z;
+                asm volatile("" : "+s"(shr), "+s"(Zr));
+                if (active) {
+                    if constexpr (std::is_same<T, float>::value)
+                        process_row<BG, R, NR>(app, shr, zl, Zr, alpha, st_n1, st_n2, st_meta);
+                    else
+                        process_row<BG, R, NW>(app, shr, zl, Zr, alpha, st_h);
+                }
+                // barrier before the next row only where it shares a column
+                // with the rows since the last barrier, and at the iteration end
+                constexpr bool next_needs = (R + 1 >= NR) || ((kBarrierMask<BG> >> (R + 1)) & 1ull);
+                if (next_needs || R + 1 == mb) __syncthreads();
+            }
+        });
+        if (job.early_term) {
+            uint32_t fail = 0;
+            static_for<NR>([&](auto Rc) {
+                constexpr int R = decltype(Rc)::value;
+                if (R < mb) {
+                    shift_ptr shr = sh;
+                    uint32_t Zr = Z;
+                    asm volatile("" : "+s"(shr), "+s"(Zr));
+                    if (active) fail |= row_syndrome<BG, R, T>(app, shr, zl, Zr);
+                }
+            });
+            if (!__syncthreads_or((int)fail)) break;
+        }
+    }
+    return it;
+}
+
+// Phase 3: hard decisions of the first nbits APP values (APP <= 0 -> 1),
+// 64 per wave per step by ballot, into `out` (global, LSB-first words) and,
+// when `lds_words` is given, into LDS as well (ends with a barrier then).
+template <typename T>
+__device__ __forceinline__ void store_hard(const T* __restrict__ app, int nbits, uint32_t* __restrict__ out,
+                                          uint32_t* __restrict__ lds_words)
+{
+    const int nwords = (nbits + 31) >> 5;
+    const int tid = (int)threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int nwaves = (int)((blockDim.x + 63u) >> 6);
+    for (int c = wave; c * 64 < nbits; c += nwaves) {
+        const int n = c * 64 + lane;
+        const bool bit = (n < nbits) && (app[n] <= (T)0);
+        const uint64_t m = __ballot(bit);
+        if (lane == 0) {
+            out[2 * c] = (uint32_t)m;
+            if (lds_words) lds_words[2 * c] = (uint32_t)m;
+            if (2 * c + 1 < nwords) {
+                out[2 * c + 1] = (uint32_t)(m >> 32);
+                if (lds_words) lds_words[2 * c + 1] = (uint32_t)(m >> 32);
+            }
+        }
+    }
+    if (lds_words) __syncthreads();
+}
+
+// Decode one code block with the whole workgroup; LDS `app` holds
+// (NSYS + mb) * Z values of type T.  Every thread of the workgroup calls it.
+template <int BG, typename T, typename TIn, int MAXR>
+__device__ __forceinline__ int decode_one(T* __restrict__ app, const cb_job& job)
+{
+    const int nval = (bg_traits<BG>::NSYS + job.mb) * job.Z;
+    load_llrs<T, TIn>(app, static_cast<const TIn*>(job.llr), nval, job.clamp);
+    const int iters = run_iterations<BG, T, MAXR>(app, job);
+    const int nbits = job.Kb * job.Z;
+    store_hard<T>(app, nbits, job.out, nullptr);
+    // soft outputs: Kb*Z binary16 APP values
+    if (job.soft)
+        for (int n = threadIdx.x; n < nbits; n += blockDim.x) job.soft[n] = (_Float16)app[n];
+    return iters;
+}
+
+}  // namespace nrldpc

@@ -22,6 +22,10 @@ bool config_supported(const cuphyLDPCDecodeConfigDesc_t& c);
 // Kernel and launch shape for a transport-block descriptor.
 cuphyStatus_t fill_launch(const cuphyLDPCDecodeDesc_t& desc, launch_geometry& g);
 
+// Raise a kernel's dynamic-LDS limit above the 64 KB default, once per
+// (kernel, size) high-water mark.
+cuphyStatus_t ensure_lds_attr(void* func, size_t bytes);
+
 // Enqueue the decode of every code word of `desc` on `strm`.
 cuphyStatus_t decode_tb(const cuphyLDPCDecodeDesc_t& desc, hipStream_t strm);
 
@@ -29,7 +33,10 @@ cuphyStatus_t decode_tb(const cuphyLDPCDecodeDesc_t& desc, hipStream_t strm);
 cuphyStatus_t get_launch_config(cuphyLDPCDecodeLaunchConfig_t& lc);
 
 // Enqueue one persistent launch over a heterogeneous batch of work items.
+// max_mb: the largest num_parity_nodes of the batch (<= 8 selects the
+// high-code-rate variant with the smaller register footprint).
 cuphyStatus_t decode_batch(const cuphyHipLDPCWorkItem_t* d_items, int n_items, cuphyDataType_t llr_type,
-                           bool fp16_math, int max_values_per_cb, uint32_t* d_queue, int cu_count, hipStream_t strm);
+                           bool fp16_math, int max_values_per_cb, int max_mb, uint32_t* d_queue, int cu_count,
+                           hipStream_t strm);
 
 }  // namespace nrldpc

@@ -14,6 +14,32 @@ namespace nrldpc {
 
 // Compile-time view of a TS 38.212 base graph (tables in nr_bg_tables.h).
 template <int BG> struct bg_traits;
+// Rows are processed in order; a workgroup barrier is needed before row r
+// only if r shares a column with a row processed since the last barrier
+// (the layered schedule's read-after-write and write-after-read hazards
+// across waves).  Bit r of the mask: barrier before row r.
+template <typename Tr> constexpr uint64_t barrier_mask()
+{
+    uint64_t need = 0;
+    uint64_t acc_lo = 0, acc_hi = 0;
+    for (int r = 0; r < Tr::NROWS; ++r) {
+        uint64_t lo = 0, hi = 0;
+        for (int e = Tr::row_start(r); e < Tr::row_start(r + 1); ++e) {
+            const int c = Tr::col(e);
+            if (c < 64) lo |= 1ull << c; else hi |= 1ull << (c - 64);
+        }
+        if ((lo & acc_lo) || (hi & acc_hi)) {
+            need |= 1ull << r;
+            acc_lo = lo;
+            acc_hi = hi;
+        } else {
+            acc_lo |= lo;
+            acc_hi |= hi;
+        }
+    }
+    return need;
+}
+
 template <> struct bg_traits<1> {
     static constexpr int NROWS = NR_BG1_NUM_ROWS;  // 46
     static constexpr int NSYS = 22;                // systematic columns incl. 2 punctured

@@ -188,6 +188,37 @@ def main():
     c1_us = e0.elapsed_time(e1) * 1e3 / n_c1
     c1_gbps = 8 * 8448 / (c1_us * 1e-6) / 1e9
 
+    # ---- 20-cell full-load slot (2800 CBs, BG1 Z=384, R=0.93): both forms ---
+    c3 = W.full_load_slot(seed=3)
+    c3_bits = sum(tb.ncb * tb.info_bits_per_cb for tb in c3)
+    c3_descs = [dec.build_desc(LdpcDecodeConfig(tb.bg, tb.Z, tb.mb, llr_type=L.CUPHY_R_16F, max_iterations=ITERS,
+                                                alpha=0.8125, clamp_value=32.0), [tb.llr], [tb.out]) for tb in c3]
+    c3b = BatchDecoder(dec).build(c3, llr_type=L.CUPHY_R_16F, fp16_math=True, max_iterations=ITERS, alpha=0.8125)
+    c3b.capture()
+
+    def timed(fn, n):
+        for _ in range(2):
+            fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(n):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / n
+
+    c3_plain_ms = timed(lambda: [dec.launch(d, stream) for d in c3_descs], max(2, a.steps // 4))
+    c3_batch_ms = timed(c3b.replay, max(2, a.steps // 2))
+    c3_edges = sum(tb.edge_updates(ITERS) for tb in c3)
+    config3 = {"code_blocks": sum(tb.ncb for tb in c3), "info_bits": c3_bits,
+               "plain_ms": round(c3_plain_ms, 4), "plain_gbps": round(c3_bits / (c3_plain_ms * 1e-3) / 1e9, 3),
+               "batched_ms": round(c3_batch_ms, 4), "batched_gbps": round(c3_bits / (c3_batch_ms * 1e-3) / 1e9, 3),
+               "batched_valu_frac": round(c3_edges * OPS_PER_EDGE / (c3_batch_ms * 1e-3) / 1e12 / VALU_PEAK_TOPS, 4),
+               "slot_budget_ms": 0.5}
+    del c3, c3b
+
     cpu = None if a.no_cpu else cpu_baseline(tbs, a.cpu_seconds)
 
     line = {
@@ -216,6 +247,7 @@ def main():
         "check": {"plain_bit_errors": bit_errs, "plain_block_errors": blk_errs, "batched_bit_errors": bbit_errs,
                   "code_blocks": ncb},
         "config1_plain_fp32": {"us_per_tb": round(c1_us, 2), "gbps": round(c1_gbps, 4)},
+        "config3_full_load_20_cells": config3,
     }
     print(json.dumps(line))
 

@@ -373,11 +373,57 @@ cuphyStatus_t CUPHYWINAPI cuphyHipLDPCBatchDecodeGetWorkspaceSize(size_t* sizeIn
  * launch: each workgroup takes the next item from the device queue until
  * the batch is drained.  llr_type: CUPHY_R_32F or CUPHY_R_16F for every
  * item; fp16_math: binary16 arithmetic (always for CUPHY_R_16F);
- * max_values_per_cb: max over items of (nsys + mb) * Z (sizes LDS); order
- * the items longest first for the shortest makespan. */
+ * max_values_per_cb: max over items of (nsys + mb) * Z (sizes LDS);
+ * max_parity_nodes: max over items of mb (<= 8 selects the register-lean
+ * high-code-rate variant).  Order the items longest first for the shortest
+ * makespan. */
 cuphyStatus_t CUPHYWINAPI cuphyHipLDPCBatchDecode(cuphyLDPCDecoder_t decoder, const cuphyHipLDPCWorkItem_t* d_items,
                                                   int num_items, cuphyDataType_t llr_type, int fp16_math,
-                                                  int max_values_per_cb, void* d_workspace, hipStream_t strm);
+                                                  int max_values_per_cb, int max_parity_nodes, void* d_workspace,
+                                                  hipStream_t strm);
+
+/* One transport block (one transmission) of a fused slot decode.  The
+ * reference runs de-rate-matching, LDPC decoding and CRC checking as three
+ * kernels per TB group (pusch_rx.cpp:3007-3021); cuphyHipPuschSlotDecode
+ * runs them fused, code block by code block, in one persistent launch. */
+typedef struct {
+    const PerTbParams* prm;          /* device pointer: the TB's parameters, as for cuphySetupPuschRxRateMatch */
+    const void* llr;                 /* this transmission's LLRs in the rate-match input layout (QAM_STRIDE
+                                        slots per symbol), element type = llr_type of the launch */
+    void* harq;                      /* binary16 HARQ buffers: CB r at r * Ncb_padded values, [2Zc | Ncb];
+                                        ndi = 1 overwrites, ndi = 0 combines */
+    uint32_t* cb_bits;               /* decoded bits: CB r at r * 264 words, bit i of word w = bit 32w + i */
+    uint32_t* cb_crc;                /* CRC remainder of CB r (0 = pass; 0xFFFFFFFF = parameters rejected) */
+    uint8_t* tb_bytes;               /* TB payload bytes, MSB-first per byte, num_CBs * data bytes, padded to 4 */
+    uint32_t* tb_crc;                /* TB CRC remainder (0 = pass; 0xFFFFFFFF = parameters rejected) */
+    uint8_t* cb_iterations;          /* optional: iterations run per CB r, NULL for none */
+    int16_t num_parity_nodes;        /* mb of the decoder, 4..46 (BG2: ..42) */
+    int16_t max_iterations;
+    uint16_t flags;                  /* CUPHY_LDPC_DECODE_EARLY_TERM */
+    uint16_t reserved;
+    float llr_scale;                 /* combined LLR = stored value * llr_scale, rounded to binary16
+                                        (int8 LLRs with 4 steps per unit LLR: 0.25) */
+    cuphyLDPCNormalization_t norm;   /* f16x2: binary16 normalisation */
+    float clamp_value;               /* decoder input clamp */
+} cuphyHipPuschSlotTb_t;
+
+/* Workspace of cuphyHipPuschSlotDecode for up to num_tbs TBs: queue state
+ * and per-TB completion counters.  Zero-fill once; every completed launch
+ * leaves it zeroed (graph-replayable).  One workspace per running launch. */
+cuphyStatus_t CUPHYWINAPI cuphyHipPuschSlotDecodeGetWorkspaceSize(int num_tbs, size_t* sizeInBytes);
+
+/* Fused de-rate-match + HARQ combine + layered NMS decode (binary16) + CB
+ * CRC + TB assembly + TB CRC for num_cb_items code blocks in one persistent
+ * launch.  d_cb_items[i] = (TB index << 16) | CB index within the TB;
+ * every CB of a TB must be in the same launch.  llr_type: CUPHY_R_8I,
+ * CUPHY_R_16F or CUPHY_R_32F.  max_lds_values: max over TBs of
+ * max(2Zc + Ncb, (nsys + mb) * Zc) (sizes LDS); max_parity_nodes: max mb
+ * (<= 8 selects the register-lean high-code-rate variant; a CB with a
+ * larger mb is rejected). */
+cuphyStatus_t CUPHYWINAPI cuphyHipPuschSlotDecode(cuphyLDPCDecoder_t decoder, const cuphyHipPuschSlotTb_t* d_tbs,
+                                                  int num_tbs, const uint32_t* d_cb_items, int num_cb_items,
+                                                  cuphyDataType_t llr_type, int max_lds_values, int max_parity_nodes,
+                                                  void* d_workspace, hipStream_t strm);
 
 #ifdef __cplusplus
 }

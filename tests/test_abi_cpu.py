@@ -117,3 +117,45 @@ def test_descriptor_info_queries(L):
     if hasattr(lib, "cuphyPuschRxCrcDecodeGetDescrInfo"):
         assert lib.cuphyPuschRxCrcDecodeGetDescrInfo(C.byref(a), C.byref(b)) == 0
         assert a.value > 0 and b.value > 0
+
+
+# ctypes mirror -> C type name in include/cuphy_hip.h
+_MIRRORS = [("LDPCDecodeConfigDesc", "cuphyLDPCDecodeConfigDesc_t"),
+            ("TransportBlockLLRDesc", "cuphyTransportBlockLLRDesc_t"),
+            ("TransportBlockDataDesc", "cuphyTransportBlockDataDesc_t"),
+            ("LDPCDecodeDesc", "cuphyLDPCDecodeDesc_t"),
+            ("LDPCDecodeLaunchConfig", "cuphyLDPCDecodeLaunchConfig_t"),
+            ("PerTbParams", "PerTbParams"),
+            ("LDPCWorkItem", "cuphyHipLDPCWorkItem_t"),
+            ("PuschSlotTb", "cuphyHipPuschSlotTb_t"),
+            ("TensorPrm", "cuphyTensorPrm_t"),
+            ("RateMatchLaunchCfg", "cuphyPuschRxRateMatchLaunchCfg_t"),
+            ("CrcDecodeLaunchCfg", "cuphyPuschRxCrcDecodeLaunchCfg_t")]
+
+
+def test_ctypes_layouts_match_the_c_header(tmp_path):
+    """Size and every field offset of each ctypes mirror equal the C
+    compiler's for the header struct (gcc, x86-64)."""
+    import shutil
+    import subprocess
+    from aerial_amd import _lib
+    if shutil.which("gcc") is None or not os.path.isdir("/opt/rocm/include"):
+        pytest.skip("gcc or ROCm headers not available")
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "cuphy_hip.h"', "int main(void) {"]
+    expect = []
+    for py, cname in _MIRRORS:
+        cls = getattr(_lib, py)
+        lines.append('printf("%%zu\\n", sizeof(%s));' % cname)
+        expect.append(("%s size" % py, C.sizeof(cls)))
+        for f in cls._fields_:
+            lines.append('printf("%%zu\\n", offsetof(%s, %s));' % (cname, f[0]))
+            expect.append(("%s.%s" % (py, f[0]), getattr(cls, f[0]).offset))
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I" + os.path.join(ROOT, "include"), "-I/opt/rocm/include",
+                    "-D__HIP_PLATFORM_AMD__", str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    bad = [(name, want, g) for (name, want), g in zip(expect, got) if want != g]
+    assert len(got) == len(expect) and not bad, bad
