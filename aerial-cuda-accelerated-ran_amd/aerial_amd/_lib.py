@@ -39,6 +39,7 @@ CUPHY_MAX_N_CSI1_PRMS = 4
 CUPHY_MAX_N_CSI2_REPORTS_PER_UE = 16
 MAX_WORDS_PER_CODE_BLOCK = 264
 SLOT_CB_ERROR = 0xFFFFFFFF  # CRC value of a CB/TB the fused slot kernel rejected
+CUPHY_HIP_HARQ_LAST_TX = 0x100
 
 
 class Norm(C.Union):
@@ -123,6 +124,34 @@ class PuschSlotTb(C.Structure):
                 ("clamp_value", C.c_float)]
 
 
+class HarqServerConfig(C.Structure):
+    _fields_ = [("harq_pool_bytes", C.c_size_t), ("max_inflight", C.c_int), ("max_tbs_per_launch", C.c_int),
+                ("max_cbs_per_launch", C.c_int), ("llr_type", C.c_int)]
+
+
+class HarqRequest(C.Structure):
+    _fields_ = [("harq_id", C.c_uint64), ("tag", C.c_uint64), ("prm", PerTbParams), ("llr", C.c_void_p),
+                ("cb_bits", C.c_void_p), ("cb_crc", C.c_void_p), ("tb_bytes", C.c_void_p),
+                ("cb_iterations", C.c_void_p), ("num_parity_nodes", C.c_int16), ("max_iterations", C.c_int16),
+                ("flags", C.c_uint16), ("reserved", C.c_uint16), ("llr_scale", C.c_float), ("norm", Norm),
+                ("clamp_value", C.c_float)]
+
+
+class HarqCompletion(C.Structure):
+    _fields_ = [("tag", C.c_uint64), ("harq_id", C.c_uint64), ("tb_crc", C.c_uint32), ("status", C.c_int32),
+                ("t_submit_us", C.c_double), ("t_launch_us", C.c_double), ("t_done_us", C.c_double)]
+
+
+class HarqServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("launches", C.c_uint64),
+                ("pool_bytes_in_use", C.c_uint64), ("pool_bytes_peak", C.c_uint64), ("pending", C.c_uint64),
+                ("admission_waits", C.c_uint64)]
+
+
+class HarqTraceEntry(C.Structure):
+    _fields_ = [("req", HarqRequest), ("arrival_us", C.c_double), ("next", C.c_int32), ("reserved", C.c_int32)]
+
+
 class TensorPrm(C.Structure):
     _fields_ = [("desc", C.c_void_p), ("pAddr", C.c_void_p)]
 
@@ -173,6 +202,13 @@ SIGNATURES = {
     "cuphyHipLDPCBatchDecode": ([_VP, _VP, _I, _I, _I, _I, _I, _VP, _VP], _I),
     "cuphyHipPuschSlotDecodeGetWorkspaceSize": ([_I, _P(C.c_size_t)], _I),
     "cuphyHipPuschSlotDecode": ([_VP, _VP, _I, _VP, _I, _I, _I, _I, _VP, _VP], _I),
+    "cuphyHipHarqServerCreate": ([_VP, _P(HarqServerConfig), _P(_VP)], _I),
+    "cuphyHipHarqServerDestroy": ([_VP], _I),
+    "cuphyHipHarqServerSubmit": ([_VP, _P(HarqRequest)], _I),
+    "cuphyHipHarqServerPoll": ([_VP, _VP, _I, _P(_I)], _I),
+    "cuphyHipHarqServerRelease": ([_VP, C.c_uint64], _I),
+    "cuphyHipHarqServerGetStats": ([_VP, _P(HarqServerStats)], _I),
+    "cuphyHipHarqServeTrace": ([_VP, _VP, _I, C.c_double, _VP, _P(_I)], _I),
 }
 
 _lib = None

@@ -37,13 +37,17 @@ class BatchDecoder:
     """Heterogeneous batch over one or two persistent launches.
 
     Code blocks with mb <= 8 (high code rates) go to the register-lean
-    kernel variant, the rest to the full one; the two launches run on two
-    streams (fork/join), so they share the GPU concurrently."""
+    kernel variant when there are at least `split_min` of them (enough to
+    fill the GPU on their own), the rest to the full one; the two launches
+    run on two streams (fork/join), so they share the GPU concurrently.  A
+    smaller high-rate class joins the full variant's launch: measured on the
+    16-UE heterogeneous slot, one launch is faster than two there."""
 
-    def __init__(self, decoder, split_by_mb=True):
+    def __init__(self, decoder, split_by_mb=True, split_min=256):
         self.dec = decoder
         self._lib = L.lib()
         self.split_by_mb = split_by_mb
+        self.split_min = split_min
         sz = C.c_size_t()
         L.check("cuphyHipLDPCBatchDecodeGetWorkspaceSize",
                 self._lib.cuphyHipLDPCBatchDecodeGetWorkspaceSize(C.byref(sz)))
@@ -65,11 +69,14 @@ class BatchDecoder:
         """tbs: objects with bg, Z, mb, ncb, llr (ncb, stride) and out (ncb, words)
         device tensors (aerial_amd.workloads.TransportBlock).  soft: optional
         list of (ncb, >= Kb*Z) float16 tensors per TB."""
+        n_small = sum(tb.ncb for tb in tbs if tb.mb <= SMALL_MAXR)
+        n_all = sum(tb.ncb for tb in tbs)
+        split = self.split_by_mb and (n_small >= self.split_min or n_small == n_all)
         classes = {}
         for t, tb in enumerate(tbs):
             Kb = getattr(tb, "Kb", None) or nsys(tb.bg)
             cost = tb.Z * num_edges(tb.bg, tb.mb)
-            key = (tb.mb <= SMALL_MAXR) if self.split_by_mb else True
+            key = (tb.mb <= SMALL_MAXR) if split else False
             for c in range(tb.ncb):
                 classes.setdefault(key, []).append((cost, t, c, Kb))
         h = _half_bits(alpha)

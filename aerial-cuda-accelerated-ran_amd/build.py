@@ -25,6 +25,7 @@ SOURCES = [
     "crc_decode.hip",
     "pusch_fused.hip",
     "cuphy_api.cpp",
+    "harq_server.cpp",
 ]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function",

@@ -118,6 +118,75 @@ __device__ uint32_t block_crc(int nw, int kind, WordFn word, uint32_t* lds_red)
     return r;
 }
 
+// Byte-granular shift factors: x^(8m) mod g for m < CRC_SHIFT_LO (shifts
+// inside one code block, up to 1056 bytes), and for CRC24A x^(8 * 1024 * m)
+// mod g for m < 256 (shifts inside a TB of up to 256 KiB: x^(8n) =
+// x^(8 (n mod 1024)) * x^(8 * 1024 * (n / 1024))).  Built at compile time:
+// each entry is the previous one advanced by one byte-wise CRC step.
+constexpr int CRC_SHIFT_LO = 1088;
+struct crc_shift_tables {
+    uint32_t lo[3][CRC_SHIFT_LO];
+    uint32_t tb_hi[256];
+};
+
+constexpr crc_shift_tables make_crc_shift_tables()
+{
+    crc_shift_tables t{};
+    const crc_poly ps[3] = {CRC24A_P, CRC24B_P, CRC16_P};
+    for (int k = 0; k < 3; ++k) {
+        const uint32_t mask = (1u << ps[k].L) - 1u;
+        uint32_t v = 1;  // x^0
+        for (int m = 0; m < CRC_SHIFT_LO; ++m) {
+            t.lo[k][m] = v;
+            // v * x^8 mod g: one byte-wise step with a zero input byte
+            v = ((v << 8) & mask) ^ byte_entry((v >> (ps[k].L - 8)) & 0xFFu, ps[k]);
+        }
+    }
+    const uint32_t step = gf2_mulmod(t.lo[0][1023], t.lo[0][1], CRC24A_P);  // x^(8*1024)
+    uint32_t v = 1;
+    for (int m = 0; m < 256; ++m) {
+        t.tb_hi[m] = v;
+        v = gf2_mulmod(v, step, CRC24A_P);
+    }
+    return t;
+}
+
+static __device__ const crc_shift_tables g_crc_shift = make_crc_shift_tables();
+
+__device__ __forceinline__ uint32_t crc_poly_of(int kind) { return kind == K_CRC24A ? CRC24A_P.g : (kind == K_CRC24B ? CRC24B_P.g : CRC16_P.g); }
+__device__ __forceinline__ int crc_len_of(int kind) { return kind == K_CRC16 ? 16 : 24; }
+
+// Remainder of the nbytes message bytes byte_at(0 .. nbytes-1) (MSB-first
+// bytes), nbytes <= CRC_SHIFT_LO, computed by the workgroup with the byte
+// table T of the kind (in LDS): every thread a contiguous chunk, shifted by
+// x^(8 * bytes after it) with one multiplication, XOR-reduced.
+template <typename ByteFn>
+__device__ uint32_t block_crc_bytes(int nbytes, int kind, ByteFn byte_at, const uint32_t* T, uint32_t* lds_red)
+{
+    const int L = crc_len_of(kind);
+    const uint32_t g = crc_poly_of(kind);
+    const uint32_t mask = (1u << L) - 1u;
+    const int nt = blockDim.x, tid = threadIdx.x;
+    const int per = (nbytes + nt - 1) / nt;
+    const int b0 = min(nbytes, tid * per), b1 = min(nbytes, b0 + per);
+    uint32_t reg = 0;
+    for (int b = b0; b < b1; ++b) reg = ((reg << 8) & mask) ^ T[((reg >> (L - 8)) ^ byte_at(b)) & 0xFFu];
+    if (reg && nbytes > b1) reg = dmulmod(reg, g_crc_shift.lo[kind][nbytes - b1], L, g);
+    for (int off = 32; off > 0; off >>= 1) reg ^= __shfl_xor(reg, off, 64);
+    if ((tid & 63) == 0) lds_red[tid >> 6] = reg;
+    __syncthreads();
+    uint32_t r = 0;
+    for (int i = 0; i < (nt + 63) / 64; ++i) r ^= lds_red[i];
+    __syncthreads();
+    return r;
+}
+
+// x^(8n) mod g24A for n < 256 KiB.
+__device__ __forceinline__ uint32_t crc24a_shift_bytes(uint32_t n)
+{
+    return dmulmod(g_crc_shift.lo[K_CRC24A][n & 1023u], g_crc_shift.tb_hi[(n >> 10) & 255u], 24, CRC24A_P.g);
+}
+
 // CRC bits attached to each CB: 24 (CRC24B) when the TB is segmented.
 __device__ __forceinline__ int cb_crc_bits(const PerTbParams& p)
 {

@@ -10,6 +10,7 @@
 #include <new>
 
 #include "cuphy_hip.h"
+#include "api_objects.hpp"
 #include "ldpc_decode.hpp"
 #include "ldpc_encode.hpp"
 #include "derate_match.hpp"
@@ -33,15 +34,6 @@ struct cuphyTensorDescriptor {
     int rank = 0;
     int dims[CUPHY_DIM_MAX] = {0};
     int strides[CUPHY_DIM_MAX] = {0};
-};
-
-// ---------------------------------------------------------------------------
-// LDPC decoder object: immutable after creation (decodes from several host
-// threads on different streams are safe), owns no device memory.
-// ---------------------------------------------------------------------------
-struct cuphyLDPCDecoder {
-    int device = 0;
-    int cu_count = 0;
 };
 
 // ---------------------------------------------------------------------------
@@ -590,7 +582,7 @@ cuphyStatus_t CUPHYWINAPI cuphyHipLDPCBatchDecode(cuphyLDPCDecoder_t decoder, co
 cuphyStatus_t CUPHYWINAPI cuphyHipPuschSlotDecodeGetWorkspaceSize(int num_tbs, size_t* sizeInBytes)
 {
     if (!sizeInBytes || num_tbs < 0 || num_tbs > 65536) return CUPHY_STATUS_INVALID_ARGUMENT;
-    *sizeInBytes = (size_t)(nrldpc::FUSED_WS_QUEUE_WORDS + num_tbs) * sizeof(uint32_t);
+    *sizeInBytes = (size_t)(nrldpc::FUSED_WS_QUEUE_WORDS + 2 * num_tbs) * sizeof(uint32_t);
     return CUPHY_STATUS_SUCCESS;
 }
 

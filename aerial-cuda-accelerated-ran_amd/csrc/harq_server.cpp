@@ -1,0 +1,546 @@
+// HARQ serving engine: continuous batching of PUSCH transmissions into fused
+// slot launches (cuphyHipPuschSlotDecode) with device-resident HARQ soft
+// buffers under a byte budget.
+//
+// The reference has no such layer (cuPHY is driven slot by slot by
+// cuPHY-CP); its per-slot pipeline (pusch_rx.cpp) is what one launch here
+// corresponds to.  Design:
+//   * one device pool of harq_pool_bytes; a first-fit allocator with
+//     coalescing hands each HARQ process a buffer of (first CB + C) *
+//     Ncb_padded binary16 values when its first transmission (ndi = 1) is
+//     admitted; the buffer is freed when a transmission passes the TB CRC,
+//     carries CUPHY_HIP_HARQ_LAST_TX, or on Release.  A new TB whose buffer
+//     does not fit waits (admission by HBM budget); retransmissions always
+//     have theirs.
+//   * max_inflight launch slots, each with its own stream, event, pinned
+//     staging area and device descriptor area.  Poll() retires finished
+//     slots, then fills every free slot with the admitted requests in
+//     arrival order (a HARQ process's next transmission waits for its
+//     previous one): requests that arrive while launches run are batched
+//     into the next free slot, so launches overlap on the device.
+//   * per launch: one host->device copy of the PerTbParams, job and item
+//     arrays, one or two persistent fused launches (the register-lean
+//     variant for mb <= 8 when at least one CU's worth of such CBs is
+//     present), one device->host copy of the TB CRCs, an event.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <new>
+#include <unordered_map>
+#include <vector>
+
+#include "api_objects.hpp"
+#include "cuphy_hip.h"
+#include "nr_bg_tables.h"
+#include "pusch_fused.hpp"
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+
+constexpr size_t POOL_ALIGN = 256;
+constexpr uint32_t CB_REJECTED = 0xFFFFFFFFu;
+constexpr int SMALL_MB = 8;
+
+size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// First-fit allocator over [0, size) with a coalescing free list.
+class pool_allocator {
+public:
+    void init(size_t size)
+    {
+        free_.clear();
+        if (size) free_[0] = size;
+        in_use_ = peak_ = 0;
+    }
+    bool alloc(size_t bytes, size_t& off)
+    {
+        bytes = round_up(bytes, POOL_ALIGN);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < bytes) continue;
+            off = it->first;
+            const size_t rest = it->second - bytes;
+            free_.erase(it);
+            if (rest) free_[off + bytes] = rest;
+            in_use_ += bytes;
+            peak_ = std::max(peak_, in_use_);
+            return true;
+        }
+        return false;
+    }
+    void release(size_t off, size_t bytes)
+    {
+        bytes = round_up(bytes, POOL_ALIGN);
+        in_use_ -= bytes;
+        auto it = free_.emplace(off, bytes).first;
+        auto next = std::next(it);
+        if (next != free_.end() && it->first + it->second == next->first) {
+            it->second += next->second;
+            free_.erase(next);
+        }
+        if (it != free_.begin()) {
+            auto prev = std::prev(it);
+            if (prev->first + prev->second == it->first) {
+                prev->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    size_t in_use() const { return in_use_; }
+    size_t peak() const { return peak_; }
+
+private:
+    std::map<size_t, size_t> free_;
+    size_t in_use_ = 0, peak_ = 0;
+};
+
+struct harq_proc {
+    size_t off = 0, bytes = 0;
+    bool busy = false;
+};
+
+struct queued {
+    cuphyHipHarqRequest_t req;
+    double t_submit;
+};
+
+struct launch_slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    bool busy = false;
+    unsigned char* h_stage = nullptr;  // pinned: params | jobs | items
+    unsigned char* d_stage = nullptr;
+    uint32_t* h_crc = nullptr;         // pinned TB CRCs of the launch
+    uint32_t* d_crc = nullptr;
+    uint32_t* d_ws[2] = {nullptr, nullptr};
+    std::vector<queued> reqs;
+    double t_launch = 0.0;
+};
+
+int num_edges(int bg, int mb)
+{
+    return bg == 1 ? nr_bg1_row_start[mb] : nr_bg2_row_start[mb];
+}
+
+int nsys_of(int bg) { return bg == 1 ? 22 : 10; }
+
+bool is_lifting(int Z)
+{
+    for (int a : {2, 3, 5, 7, 9, 11, 13, 15})
+        for (int v = a; v <= 384; v <<= 1)
+            if (v == Z) return true;
+    return false;
+}
+
+}  // namespace
+
+struct cuphyHipHarqServer {
+    int device = 0;
+    int cu_count = 1;
+    cuphyHipHarqServerConfig_t cfg{};
+    unsigned char* d_pool = nullptr;
+    pool_allocator pool;
+    std::unordered_map<uint64_t, harq_proc> procs;
+    std::deque<queued> pending;
+    std::vector<launch_slot> slots;
+    std::deque<cuphyHipHarqCompletion_t> ready;
+    clk::time_point t0;
+    size_t ws_bytes = 0, stage_bytes = 0;
+    uint64_t submitted = 0, completed = 0, launches = 0, admission_waits = 0;
+
+    double now_us() const { return std::chrono::duration<double, std::micro>(clk::now() - t0).count(); }
+    size_t params_off() const { return 0; }
+    size_t jobs_off() const { return round_up((size_t)cfg.max_tbs_per_launch * sizeof(PerTbParams), 256); }
+    size_t items_off() const
+    {
+        return jobs_off() + round_up((size_t)cfg.max_tbs_per_launch * sizeof(cuphyHipPuschSlotTb_t), 256);
+    }
+
+    static size_t harq_bytes(const PerTbParams& p)
+    {
+        return (size_t)(p.firstCodeBlockIndex + p.num_CBs) * p.Ncb_padded * 2u;
+    }
+    void complete(const queued& q, uint32_t crc, int32_t status, double t_launch, double t_done)
+    {
+        cuphyHipHarqCompletion_t c;
+        c.tag = q.req.tag;
+        c.harq_id = q.req.harq_id;
+        c.tb_crc = crc;
+        c.status = status;
+        c.t_submit_us = q.t_submit;
+        c.t_launch_us = t_launch;
+        c.t_done_us = t_done;
+        ready.push_back(c);
+        ++completed;
+    }
+    void free_proc(uint64_t id)
+    {
+        auto it = procs.find(id);
+        if (it == procs.end()) return;
+        pool.release(it->second.off, it->second.bytes);
+        procs.erase(it);
+    }
+    cuphyStatus_t retire();
+    cuphyStatus_t admit_and_launch();
+    cuphyStatus_t launch(launch_slot& s);
+    void destroy();
+};
+
+void cuphyHipHarqServer::destroy()
+{
+    // teardown: errors are not actionable here, the handles are dropped either way
+    for (auto& s : slots) {
+        if (s.stream) {
+            (void)hipStreamSynchronize(s.stream);
+            (void)hipStreamDestroy(s.stream);
+        }
+        if (s.done) (void)hipEventDestroy(s.done);
+        if (s.h_stage) (void)hipHostFree(s.h_stage);
+        if (s.h_crc) (void)hipHostFree(s.h_crc);
+        if (s.d_stage) (void)hipFree(s.d_stage);
+        if (s.d_crc) (void)hipFree(s.d_crc);
+        for (auto w : s.d_ws)
+            if (w) (void)hipFree(w);
+    }
+    slots.clear();
+    if (d_pool) (void)hipFree(d_pool);
+    d_pool = nullptr;
+}
+
+// Retire every finished launch: TB CRCs -> completions, HARQ buffers of
+// decoded (or abandoned) TBs back to the pool.
+cuphyStatus_t cuphyHipHarqServer::retire()
+{
+    for (auto& s : slots) {
+        if (!s.busy) continue;
+        const hipError_t q = hipEventQuery(s.done);
+        if (q == hipErrorNotReady) continue;
+        if (q != hipSuccess) return CUPHY_STATUS_INTERNAL_ERROR;
+        const double t = now_us();
+        for (size_t i = 0; i < s.reqs.size(); ++i) {
+            const queued& r = s.reqs[i];
+            const uint32_t crc = s.h_crc[i];
+            auto it = procs.find(r.req.harq_id);
+            if (it != procs.end()) it->second.busy = false;
+            if (crc == 0 || crc == CB_REJECTED || (r.req.flags & CUPHY_HIP_HARQ_LAST_TX)) free_proc(r.req.harq_id);
+            complete(r, crc, crc == CB_REJECTED ? CUPHY_STATUS_UNSUPPORTED_CONFIG : CUPHY_STATUS_SUCCESS,
+                     s.t_launch, t);
+        }
+        s.reqs.clear();
+        s.busy = false;
+    }
+    return CUPHY_STATUS_SUCCESS;
+}
+
+cuphyStatus_t cuphyHipHarqServer::admit_and_launch()
+{
+    for (auto& s : slots) {
+        if (s.busy) continue;
+        if (pending.empty()) break;
+        std::deque<queued> rest;
+        bool new_blocked = false;
+        int cbs = 0;
+        while (!pending.empty()) {
+            queued q = pending.front();
+            pending.pop_front();
+            const PerTbParams& p = q.req.prm;
+            const int C = (int)p.num_CBs;
+            if ((int)s.reqs.size() >= cfg.max_tbs_per_launch || cbs + C > cfg.max_cbs_per_launch) {
+                rest.push_back(q);
+                continue;
+            }
+            auto it = procs.find(q.req.harq_id);
+            if (it != procs.end() && it->second.busy) {  // previous transmission still running
+                rest.push_back(q);
+                continue;
+            }
+            if (p.ndi) {
+                const size_t need = harq_bytes(p);
+                if (it != procs.end() && it->second.bytes < need) {
+                    free_proc(q.req.harq_id);
+                    it = procs.end();
+                }
+                if (it == procs.end()) {
+                    size_t off = 0;
+                    if (new_blocked || !pool.alloc(need, off)) {
+                        if (!new_blocked) ++admission_waits;
+                        new_blocked = true;  // later new TBs wait behind this one
+                        rest.push_back(q);
+                        continue;
+                    }
+                    harq_proc hp;
+                    hp.off = off;
+                    hp.bytes = need;
+                    it = procs.emplace(q.req.harq_id, hp).first;
+                }
+            } else if (it == procs.end()) {  // retransmission without a HARQ buffer
+                complete(q, CB_REJECTED, CUPHY_STATUS_INVALID_ARGUMENT, now_us(), now_us());
+                continue;
+            }
+            it->second.busy = true;
+            s.reqs.push_back(q);
+            cbs += C;
+        }
+        pending.swap(rest);
+        if (s.reqs.empty()) break;
+        const cuphyStatus_t st = launch(s);
+        if (st != CUPHY_STATUS_SUCCESS) return st;
+    }
+    return CUPHY_STATUS_SUCCESS;
+}
+
+cuphyStatus_t cuphyHipHarqServer::launch(launch_slot& s)
+{
+    const int n = (int)s.reqs.size();
+    PerTbParams* h_params = reinterpret_cast<PerTbParams*>(s.h_stage + params_off());
+    cuphyHipPuschSlotTb_t* h_jobs = reinterpret_cast<cuphyHipPuschSlotTb_t*>(s.h_stage + jobs_off());
+    uint32_t* h_items = reinterpret_cast<uint32_t*>(s.h_stage + items_off());
+    const PerTbParams* d_params = reinterpret_cast<const PerTbParams*>(s.d_stage + params_off());
+    struct cost_item {
+        uint64_t cost;
+        uint32_t id;
+    };
+    std::vector<cost_item> small, big;
+    int lds_small = 1, lds_big = 1, mb_small = 4, mb_big = 4;
+    for (int t = 0; t < n; ++t) {
+        const cuphyHipHarqRequest_t& r = s.reqs[t].req;
+        h_params[t] = r.prm;
+        cuphyHipPuschSlotTb_t& j = h_jobs[t];
+        j.prm = d_params + t;
+        j.llr = r.llr;
+        j.harq = d_pool + procs[r.harq_id].off;
+        j.cb_bits = r.cb_bits;
+        j.cb_crc = r.cb_crc;
+        j.tb_bytes = r.tb_bytes;
+        j.tb_crc = s.d_crc + t;
+        j.cb_iterations = r.cb_iterations;
+        j.num_parity_nodes = r.num_parity_nodes;
+        j.max_iterations = r.max_iterations;
+        j.flags = (uint16_t)(r.flags & CUPHY_LDPC_DECODE_EARLY_TERM);
+        j.reserved = 0;
+        j.llr_scale = r.llr_scale;
+        j.norm = r.norm;
+        j.clamp_value = r.clamp_value;
+        const PerTbParams& p = r.prm;
+        const int mb = r.num_parity_nodes;
+        const uint64_t cost = (uint64_t)p.Zc * (uint64_t)num_edges((int)p.bg, mb) * (uint64_t)std::max(1, (int)r.max_iterations);
+        const int lds = (int)std::max<uint32_t>(2 * p.Zc + p.Ncb, (uint32_t)(nsys_of((int)p.bg) + mb) * p.Zc);
+        auto& dst = (mb <= SMALL_MB) ? small : big;
+        for (uint32_t c = 0; c < p.num_CBs; ++c) dst.push_back({cost, ((uint32_t)t << 16) | c});
+        if (mb <= SMALL_MB) {
+            lds_small = std::max(lds_small, lds);
+            mb_small = std::max(mb_small, mb);
+        } else {
+            lds_big = std::max(lds_big, lds);
+            mb_big = std::max(mb_big, mb);
+        }
+    }
+    // high-rate CBs run on the register-lean variant only when they fill the GPU
+    if (!small.empty() && !big.empty() && (int)small.size() < cu_count) {
+        big.insert(big.end(), small.begin(), small.end());
+        small.clear();
+        lds_big = std::max(lds_big, lds_small);
+        mb_big = std::max(mb_big, mb_small);
+    }
+    auto by_cost = [](const cost_item& a, const cost_item& b) { return a.cost > b.cost; };
+    std::stable_sort(small.begin(), small.end(), by_cost);
+    std::stable_sort(big.begin(), big.end(), by_cost);
+    size_t k = 0;
+    for (auto& c : small) h_items[k++] = c.id;
+    for (auto& c : big) h_items[k++] = c.id;
+    const size_t bytes = items_off() + k * sizeof(uint32_t);
+    if (hipMemcpyAsync(s.d_stage, s.h_stage, bytes, hipMemcpyHostToDevice, s.stream) != hipSuccess)
+        return CUPHY_STATUS_INTERNAL_ERROR;
+    const cuphyHipPuschSlotTb_t* d_jobs = reinterpret_cast<const cuphyHipPuschSlotTb_t*>(s.d_stage + jobs_off());
+    const uint32_t* d_items = reinterpret_cast<const uint32_t*>(s.d_stage + items_off());
+    cuphyStatus_t st = CUPHY_STATUS_SUCCESS;
+    if (!small.empty())
+        st = nrldpc::slot_decode(d_jobs, n, d_items, (int)small.size(), cfg.llr_type, lds_small, mb_small, s.d_ws[0],
+                                 cu_count, s.stream);
+    if (st == CUPHY_STATUS_SUCCESS && !big.empty())
+        st = nrldpc::slot_decode(d_jobs, n, d_items + small.size(), (int)big.size(), cfg.llr_type, lds_big, mb_big,
+                                 s.d_ws[1], cu_count, s.stream);
+    if (st != CUPHY_STATUS_SUCCESS) return st;
+    if (hipMemcpyAsync(s.h_crc, s.d_crc, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost, s.stream) != hipSuccess)
+        return CUPHY_STATUS_INTERNAL_ERROR;
+    if (hipEventRecord(s.done, s.stream) != hipSuccess) return CUPHY_STATUS_INTERNAL_ERROR;
+    s.t_launch = now_us();
+    s.busy = true;
+    ++launches;
+    return CUPHY_STATUS_SUCCESS;
+}
+
+extern "C" {
+
+cuphyStatus_t CUPHYWINAPI cuphyHipHarqServerCreate(cuphyLDPCDecoder_t decoder, const cuphyHipHarqServerConfig_t* cfg,
+                                                   cuphyHipHarqServer_t* server)
+{
+    if (!decoder || !cfg || !server) return CUPHY_STATUS_INVALID_ARGUMENT;
+    if (cfg->max_inflight < 1 || cfg->max_inflight > 8 || cfg->max_tbs_per_launch < 1 ||
+        cfg->max_tbs_per_launch > 65536 || cfg->max_cbs_per_launch < 1 || cfg->harq_pool_bytes == 0)
+        return CUPHY_STATUS_INVALID_ARGUMENT;
+    if (cfg->llr_type != CUPHY_R_8I && cfg->llr_type != CUPHY_R_16F && cfg->llr_type != CUPHY_R_32F)
+        return CUPHY_STATUS_UNSUPPORTED_TYPE;
+    cuphyHipHarqServer* s = new (std::nothrow) cuphyHipHarqServer();
+    if (!s) return CUPHY_STATUS_ALLOC_FAILED;
+    s->device = decoder->device;
+    s->cu_count = std::max(1, decoder->cu_count);
+    s->cfg = *cfg;
+    s->t0 = clk::now();
+    size_t ws = 0;
+    cuphyHipPuschSlotDecodeGetWorkspaceSize(cfg->max_tbs_per_launch, &ws);
+    s->ws_bytes = ws;
+    s->stage_bytes = s->items_off() + (size_t)cfg->max_cbs_per_launch * sizeof(uint32_t);
+    bool ok = hipMalloc(&s->d_pool, cfg->harq_pool_bytes) == hipSuccess;
+    s->pool.init(cfg->harq_pool_bytes);
+    s->slots.resize(cfg->max_inflight);
+    for (auto& sl : s->slots) {
+        if (!ok) break;
+        ok = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) == hipSuccess &&
+             hipHostMalloc(reinterpret_cast<void**>(&sl.h_stage), s->stage_bytes, hipHostMallocDefault) == hipSuccess &&
+             hipHostMalloc(reinterpret_cast<void**>(&sl.h_crc), (size_t)cfg->max_tbs_per_launch * 4, hipHostMallocDefault) ==
+                 hipSuccess &&
+             hipMalloc(reinterpret_cast<void**>(&sl.d_stage), s->stage_bytes) == hipSuccess &&
+             hipMalloc(reinterpret_cast<void**>(&sl.d_crc), (size_t)cfg->max_tbs_per_launch * 4) == hipSuccess;
+        for (auto& w : sl.d_ws)
+            ok = ok && hipMalloc(reinterpret_cast<void**>(&w), ws) == hipSuccess && hipMemset(w, 0, ws) == hipSuccess;
+    }
+    if (!ok) {
+        s->destroy();
+        delete s;
+        return CUPHY_STATUS_ALLOC_FAILED;
+    }
+    *server = s;
+    return CUPHY_STATUS_SUCCESS;
+}
+
+cuphyStatus_t CUPHYWINAPI cuphyHipHarqServerDestroy(cuphyHipHarqServer_t server)
+{
+    if (!server) return CUPHY_STATUS_INVALID_ARGUMENT;
+    server->destroy();
+    delete server;
+    return CUPHY_STATUS_SUCCESS;
+}
+
+cuphyStatus_t CUPHYWINAPI cuphyHipHarqServerSubmit(cuphyHipHarqServer_t server, const cuphyHipHarqRequest_t* req)
+{
+    if (!server || !req) return CUPHY_STATUS_INVALID_ARGUMENT;
+    const PerTbParams& p = req->prm;
+    const int bg = (int)p.bg;
+    if ((bg != 1 && bg != 2) || !is_lifting((int)p.Zc) || p.num_CBs < 1 || p.num_CBs > 65535 || p.Qm < 1 ||
+        p.Qm > 8 || p.Nl < 1 || p.Ncb < 1 || p.Ncb > (uint32_t)(bg == 1 ? 66 : 50) * p.Zc ||
+        p.Ncb_padded < 2 * p.Zc + p.Ncb || p.K != (uint32_t)nsys_of(bg) * p.Zc || p.F >= p.K)
+        return CUPHY_STATUS_INVALID_ARGUMENT;
+    if (!req->llr || !req->cb_bits || !req->cb_crc || !req->tb_bytes) return CUPHY_STATUS_INVALID_ARGUMENT;
+    const int mb = req->num_parity_nodes;
+    if (mb < 4 || mb > (bg == 1 ? 46 : 42) || req->max_iterations < 0) return CUPHY_STATUS_INVALID_ARGUMENT;
+    if ((int)p.num_CBs > server->cfg.max_cbs_per_launch) return CUPHY_STATUS_INVALID_ARGUMENT;
+    if (server->harq_bytes(p) > server->cfg.harq_pool_bytes) return CUPHY_STATUS_INVALID_ARGUMENT;
+    queued q;
+    q.req = *req;
+    q.t_submit = server->now_us();
+    server->pending.push_back(q);
+    ++server->submitted;
+    return CUPHY_STATUS_SUCCESS;
+}
+
+cuphyStatus_t CUPHYWINAPI cuphyHipHarqServerPoll(cuphyHipHarqServer_t server, cuphyHipHarqCompletion_t* out,
+                                                 int max_out, int* n_out)
+{
+    if (!server || !n_out || max_out < 0 || (max_out > 0 && !out)) return CUPHY_STATUS_INVALID_ARGUMENT;
+    cuphyStatus_t st = server->retire();
+    if (st == CUPHY_STATUS_SUCCESS) st = server->admit_and_launch();
+    int n = 0;
+    while (n < max_out && !server->ready.empty()) {
+        out[n++] = server->ready.front();
+        server->ready.pop_front();
+    }
+    *n_out = n;
+    return st;
+}
+
+cuphyStatus_t CUPHYWINAPI cuphyHipHarqServerRelease(cuphyHipHarqServer_t server, uint64_t harq_id)
+{
+    if (!server) return CUPHY_STATUS_INVALID_ARGUMENT;
+    auto it = server->procs.find(harq_id);
+    if (it == server->procs.end()) return CUPHY_STATUS_INVALID_ARGUMENT;
+    if (it->second.busy) return CUPHY_STATUS_NOT_SUPPORTED;  // a launch is using it
+    server->free_proc(harq_id);
+    return CUPHY_STATUS_SUCCESS;
+}
+
+cuphyStatus_t CUPHYWINAPI cuphyHipHarqServerGetStats(cuphyHipHarqServer_t server, cuphyHipHarqServerStats_t* stats)
+{
+    if (!server || !stats) return CUPHY_STATUS_INVALID_ARGUMENT;
+    stats->submitted = server->submitted;
+    stats->completed = server->completed;
+    stats->launches = server->launches;
+    stats->pool_bytes_in_use = server->pool.in_use();
+    stats->pool_bytes_peak = server->pool.peak();
+    stats->pending = server->pending.size();
+    stats->admission_waits = server->admission_waits;
+    return CUPHY_STATUS_SUCCESS;
+}
+
+cuphyStatus_t CUPHYWINAPI cuphyHipHarqServeTrace(cuphyHipHarqServer_t server, const cuphyHipHarqTraceEntry_t* trace,
+                                                 int n, double harq_rtt_us, cuphyHipHarqCompletion_t* done,
+                                                 int* n_done)
+{
+    if (!server || n < 0 || (n > 0 && (!trace || !done)) || !n_done || harq_rtt_us < 0.0)
+        return CUPHY_STATUS_INVALID_ARGUMENT;
+    for (int i = 0; i < n; ++i)
+        if (trace[i].next >= n || trace[i].next == i) return CUPHY_STATUS_INVALID_ARGUMENT;
+    for (int i = 0; i < n; ++i) {
+        std::memset(&done[i], 0, sizeof(done[i]));
+        done[i].tag = (uint64_t)i;
+        done[i].harq_id = trace[i].req.harq_id;
+        done[i].status = CUPHY_STATUS_NOT_SUPPORTED;
+    }
+    // release schedule: (time, entry)
+    std::multimap<double, int> due;
+    for (int i = 0; i < n; ++i)
+        if (trace[i].arrival_us >= 0.0) due.emplace(trace[i].arrival_us, i);
+    const double base = server->now_us();
+    int outstanding = 0, got = 0;
+    std::vector<cuphyHipHarqCompletion_t> buf(256);
+    while (!due.empty() || outstanding > 0) {
+        const double now = server->now_us() - base;
+        while (!due.empty() && due.begin()->first <= now) {
+            const int i = due.begin()->second;
+            due.erase(due.begin());
+            cuphyHipHarqRequest_t r = trace[i].req;
+            r.tag = (uint64_t)i;
+            const cuphyStatus_t st = cuphyHipHarqServerSubmit(server, &r);
+            if (st != CUPHY_STATUS_SUCCESS) {
+                done[i].status = st;
+                continue;
+            }
+            ++outstanding;
+        }
+        int k = 0;
+        const cuphyStatus_t st = cuphyHipHarqServerPoll(server, buf.data(), (int)buf.size(), &k);
+        if (st != CUPHY_STATUS_SUCCESS) return st;
+        for (int c = 0; c < k; ++c) {
+            cuphyHipHarqCompletion_t e = buf[c];
+            const int i = (int)e.tag;
+            if (i < 0 || i >= n) continue;
+            e.t_submit_us -= base;
+            e.t_launch_us -= base;
+            e.t_done_us -= base;
+            done[i] = e;
+            --outstanding;
+            ++got;
+            const int nx = trace[i].next;
+            if (e.status == CUPHY_STATUS_SUCCESS && e.tb_crc != 0 && nx >= 0) due.emplace(e.t_done_us + harq_rtt_us, nx);
+        }
+    }
+    *n_done = got;
+    return CUPHY_STATUS_SUCCESS;
+}
+
+}  // extern "C"

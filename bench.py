@@ -18,6 +18,8 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 import argparse
 import json
+
+import numpy as np
 import os
 import sys
 import time
@@ -38,7 +40,118 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU work budget of the baseline leg")
+    ap.add_argument("--sections", default="main,config1,config3,fused,serving",
+                    help="comma-separated subset of: main, config1, config3, fused, serving")
+    ap.add_argument("--serve-ms", type=float, default=40.0, help="arrival window of each serving load point")
     return ap.parse_args()
+
+
+def fused_slot_section(dec, steps):
+    """64-cell fused receive-side slot (int8 LLRs, one persistent graph per
+    slot: de-rate-matching + HARQ write + decode with early termination + CB
+    and TB CRC), first transmission of every TB."""
+    import numpy as np
+    import torch
+    from aerial_amd import pusch_workloads as PW
+    from aerial_amd.slot import SlotDecoder, SlotTb
+    from aerial_amd.workloads import num_edges
+
+    specs = PW.fused_slot_specs(seed=5)
+    tbs = PW.build_tbs(specs, seed=5)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(5)
+    slot = []
+    for tb in tbs:
+        llr = PW.tx_llrs(tb, 0, gen, mode="cg", dtype=torch.int8)
+        p, mb = tb.params(0, 1)
+        harq = torch.zeros(tb.C * p.Ncb_padded, dtype=torch.float16, device="cuda")
+        slot.append(SlotTb(p, llr, harq, mb, max_iterations=8, early_term=True, llr_scale=0.25))
+    sd = SlotDecoder(dec).build(slot)
+    sd.launch()
+    torch.cuda.synchronize()
+    sd.capture()
+    stream = torch.cuda.current_stream()
+    for _ in range(2):
+        sd.replay()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    n = max(3, steps // 2)
+    e0.record(stream)
+    for _ in range(n):
+        sd.replay()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / n
+    info_bits = sum(tb.info_bits for tb in tbs)
+    ncb = sum(tb.C for tb in tbs)
+    cb_fail = sum(int((st.cb_crc != 0).sum().item()) for st in slot)
+    tb_fail = sum(int((st.tb_crc != 0).sum().item()) for st in slot)
+    payload_err = 0
+    for tb, st in zip(tbs, slot):
+        got = np.unpackbits(st.tb_bytes.cpu().numpy())[:tb.A]
+        payload_err += int(not np.array_equal(got, tb.payload))
+    iters = torch.cat([st.cb_iterations.to(torch.int64) for st in slot])
+    edges = 0
+    for tb, st in zip(tbs, slot):
+        edges += tb.Zc * num_edges(tb.bg, st.mb) * int(st.cb_iterations.to(torch.int64).sum().item())
+    hbm = 0
+    for tb, st in zip(tbs, slot):
+        p = st.params
+        hbm += tb.G // tb.Qm * 8 + tb.C * (2 * (2 * p.Zc + p.Ncb) + 4 * ((tb.Kp + 31) // 32) + (tb.Kp + 7) // 8)
+    return {"workload": "64 cells x (140 CBs BG1 Zc=384 R=0.93 + 28 BG2 CBs Zc 104/208 R=0.5), int8 LLRs "
+                        "(4 steps/unit, +-127), consistent-Gaussian mu = SNR/5 at SNR U[18,24] dB, 8 iterations "
+                        "with early termination, first transmission",
+            "code_blocks": ncb, "tbs": len(tbs), "info_bits": info_bits, "ms_per_slot": round(ms, 4),
+            "gbps": round(info_bits / (ms * 1e-3) / 1e9, 3), "slot_budget_ms": 0.5,
+            "mean_iterations": round(float(iters.float().mean().item()), 3),
+            "cb_crc_failures": cb_fail, "tb_crc_failures": tb_fail, "tb_payload_mismatches": payload_err,
+            "valu_frac": round(edges * OPS_PER_EDGE / (ms * 1e-3) / 1e12 / VALU_PEAK_TOPS, 4),
+            "hbm_gbs": round(hbm / (ms * 1e-3) / 1e9, 1), "hbm_frac": round(hbm / (ms * 1e-3) / 8e12, 4)}
+
+
+def serving_section(dec, serve_ms, config3_gbps=None):
+    """Serving loop: Poisson TB arrivals from 32 cells (TB shapes as in the
+    heterogeneous slot, binary16 BPSK-AWGN LLRs), HARQ with rv 0, 2, 3, 1
+    and up to 4 transmissions, retransmissions released one HARQ round trip
+    (2 ms) after a failing TB CRC and soft-combined into device-resident
+    HARQ buffers (8 GiB pool, admission by that budget), syndrome early
+    termination within 10 iterations, continuous batching over 3 in-flight
+    fused launches.  Saturation capacity first (all TBs at t = 0), then
+    offered loads of 60/90/120% of it; latency = completion seen by the
+    engine - release of the transmission."""
+    from aerial_amd.server import HarqServer, TraceTemplates, build_trace, poisson_arrivals, serving_specs, trace_report
+    n_templates = 256
+    templates = TraceTemplates(serving_specs(n_templates, seed=41), seed=41)
+    srv = HarqServer(dec, pool_bytes=8 << 30, max_inflight=3, max_tbs=2048, max_cbs=16384)
+    rtt = 2000.0
+    mean_bits = float(np.mean([tb.info_bits for tb in templates.tbs]))
+    n_sat = 1500
+    sat = [(0.0, i % 32, i % n_templates) for i in range(n_sat)]
+    entries, outs, meta = build_trace(srv, templates, sat)
+    done = srv.serve_trace(entries, rtt)
+    rep = trace_report(done, meta, templates, sat)
+    first_tx_span_us = max(done[4 * a].t_done_us for a in range(n_sat))
+    cap_tbs = n_sat / (first_tx_span_us * 1e-6)
+    out = {"saturation": rep, "capacity_first_tx_tbs_per_s": round(cap_tbs, 1),
+           "capacity_first_tx_gbps": round(cap_tbs * mean_bits / 1e9, 3), "loads": []}
+    for x in (0.6, 0.9, 1.2):
+        arr = poisson_arrivals(32, x * cap_tbs, serve_ms * 1e3, n_templates, seed=int(100 * x))
+        entries, outs, meta = build_trace(srv, templates, arr)
+        done = srv.serve_trace(entries, rtt)
+        r = trace_report(done, meta, templates, arr)
+        r["offered_load"] = x
+        r["offered_gbps"] = round(x * cap_tbs * mean_bits / 1e9, 3)
+        if config3_gbps:
+            r["offered_vs_config3"] = round(x * cap_tbs * mean_bits / 1e9 / config3_gbps, 4)
+        r["engine"] = srv.stats()
+        out["loads"].append(r)
+        print("serving load %.1f: %s" % (x, json.dumps(r)), file=sys.stderr, flush=True)
+    srv.close()
+    out["workload"] = ("32 cells, Poisson TB arrivals, TB shapes as the heterogeneous slot (256 templates), "
+                       "rv 0,2,3,1 up to 4 transmissions, HARQ RTT 2 ms, fp16 HARQ buffers in an 8 GiB pool, "
+                       "early termination <= 10 iterations, 3 in-flight fused launches; %.0f ms per load" % serve_ms)
+    return out
 
 
 def cpu_baseline(tbs, budget_s):
@@ -73,6 +186,7 @@ def cpu_baseline(tbs, budget_s):
 
 def main():
     a = parse()
+    sections = set(a.sections.split(","))
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
               "(the decode path shards by cell with no collective -- see the design notes)" % a.gpus,
@@ -152,6 +266,21 @@ def main():
         bev[s_][1].record(gstream)
     torch.cuda.synchronize()
     batched_ms = sum(e0.elapsed_time(e1) for e0, e1 in bev) / a.steps
+    # the same slot as one launch of the full-capacity variant only
+    one = BatchDecoder(dec, split_by_mb=False).build(tbs, llr_type=L.CUPHY_R_16F, fp16_math=True,
+                                                     max_iterations=ITERS, alpha=0.8125, clamp_value=32.0)
+    one.capture()
+    for _ in range(a.warmup):
+        one.replay()
+    torch.cuda.synchronize()
+    oev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(a.steps)]
+    for s_ in range(a.steps):
+        oev[s_][0].record(gstream)
+        one.replay()
+        oev[s_][1].record(gstream)
+    torch.cuda.synchronize()
+    single_ms = sum(e0.elapsed_time(e1) for e0, e1 in oev) / a.steps
+    del one
     batched_gbps = info_bits / (batched_ms * 1e-3) / 1e9
     bbit_errs = 0
     for tb in tbs:
@@ -170,7 +299,29 @@ def main():
                 "avg_launch_us": round(batched_ms * 1e3, 2),
                 "plain_kernel": {"achieved": round(plain_tops, 3), "avg_launch_us": round(kernel_ms * 1e3 / len(descs), 2)}}
 
-    # ---- configuration 1 (plain, binary32): GPU time -----------------------
+    config1 = None
+    if "config1" in sections:
+        config1 = config1_section(dec, stream)
+    config3 = None
+    if "config3" in sections:
+        config3 = config3_section(dec, stream, a.steps)
+    fused = None
+    if "fused" in sections:
+        fused = fused_slot_section(dec, a.steps)
+    serving = None
+    if "serving" in sections:
+        serving = serving_section(dec, a.serve_ms, config3["batched_gbps"] if config3 else None)
+
+    cpu = None if a.no_cpu else cpu_baseline(tbs, a.cpu_seconds)
+    return line_of(a, locals())
+
+
+def config1_section(dec, stream):
+    """Configuration 1 on the GPU: BG1 Z=384 mb=46, 8 CBs, binary32, plain."""
+    import torch
+    from aerial_amd import _lib as L
+    from aerial_amd import workloads as W
+    from aerial_amd.decoder import LdpcDecodeConfig
     c1 = W.config1_tb(seed=1)
     c1cfg = LdpcDecodeConfig(1, 384, 46, llr_type=L.CUPHY_R_32F, max_iterations=ITERS, alpha=0.8125)
     c1desc = dec.build_desc(c1cfg, [c1.llr], [c1.out])
@@ -187,8 +338,16 @@ def main():
     torch.cuda.synchronize()
     c1_us = e0.elapsed_time(e1) * 1e3 / n_c1
     c1_gbps = 8 * 8448 / (c1_us * 1e-6) / 1e9
+    return {"us_per_tb": round(c1_us, 2), "gbps": round(c1_gbps, 4)}
 
-    # ---- 20-cell full-load slot (2800 CBs, BG1 Z=384, R=0.93): both forms ---
+
+def config3_section(dec, stream, steps):
+    """20-cell full-load slot (2800 CBs, BG1 Z=384, R=0.93): both forms."""
+    import torch
+    from aerial_amd import _lib as L
+    from aerial_amd import workloads as W
+    from aerial_amd.batch import BatchDecoder
+    from aerial_amd.decoder import LdpcDecodeConfig
     c3 = W.full_load_slot(seed=3)
     c3_bits = sum(tb.ncb * tb.info_bits_per_cb for tb in c3)
     c3_descs = [dec.build_desc(LdpcDecodeConfig(tb.bg, tb.Z, tb.mb, llr_type=L.CUPHY_R_16F, max_iterations=ITERS,
@@ -209,18 +368,21 @@ def main():
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / n
 
-    c3_plain_ms = timed(lambda: [dec.launch(d, stream) for d in c3_descs], max(2, a.steps // 4))
-    c3_batch_ms = timed(c3b.replay, max(2, a.steps // 2))
+    c3_plain_ms = timed(lambda: [dec.launch(d, stream) for d in c3_descs], max(2, steps // 4))
+    c3_batch_ms = timed(c3b.replay, max(2, steps // 2))
     c3_edges = sum(tb.edge_updates(ITERS) for tb in c3)
     config3 = {"code_blocks": sum(tb.ncb for tb in c3), "info_bits": c3_bits,
                "plain_ms": round(c3_plain_ms, 4), "plain_gbps": round(c3_bits / (c3_plain_ms * 1e-3) / 1e9, 3),
                "batched_ms": round(c3_batch_ms, 4), "batched_gbps": round(c3_bits / (c3_batch_ms * 1e-3) / 1e9, 3),
                "batched_valu_frac": round(c3_edges * OPS_PER_EDGE / (c3_batch_ms * 1e-3) / 1e12 / VALU_PEAK_TOPS, 4),
                "slot_budget_ms": 0.5}
-    del c3, c3b
+    return config3
 
-    cpu = None if a.no_cpu else cpu_baseline(tbs, a.cpu_seconds)
 
+def line_of(a, v):
+    """The one JSON line of the run (v: main()'s locals)."""
+    info_bits, ncb, tbs = v["info_bits"], v["ncb"], v["tbs"]
+    batched_gbps, batched_ms, plain_gbps, plain_ms = v["batched_gbps"], v["batched_ms"], v["plain_gbps"], v["plain_ms"]
     line = {
         "metric": METRIC,
         "value": round(batched_gbps, 4),
@@ -241,16 +403,19 @@ def main():
                    "tbs": len(tbs), "parallelism": "1 GPU"},
         "forms": {"plain_gbps": round(plain_gbps, 4), "plain_ms": round(plain_ms, 4),
                   "batched_gbps": round(batched_gbps, 4), "batched_ms": round(batched_ms, 4),
-                  "batched_over_plain": round(batched_gbps / plain_gbps, 3)},
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-        "check": {"plain_bit_errors": bit_errs, "plain_block_errors": blk_errs, "batched_bit_errors": bbit_errs,
-                  "code_blocks": ncb},
-        "config1_plain_fp32": {"us_per_tb": round(c1_us, 2), "gbps": round(c1_gbps, 4)},
-        "config3_full_load_20_cells": config3,
+                  "batched_over_plain": round(batched_gbps / plain_gbps, 3),
+                  "batched_single_launch_ms": round(v["single_ms"], 4)},
+        "roofline": v["roofline"],
+        "cpu_baseline": v["cpu"],
+        "check": {"plain_bit_errors": v["bit_errs"], "plain_block_errors": v["blk_errs"],
+                  "batched_bit_errors": v["bbit_errs"], "code_blocks": ncb},
+        "config1_plain_fp32": v["config1"],
+        "config3_full_load_20_cells": v["config3"],
+        "fused_slot_64_cells_int8": v["fused"],
+        "serving": v["serving"],
     }
-    print(json.dumps(line))
+    return line
 
 
 if __name__ == "__main__":
-    main()
+    print(json.dumps(main()))

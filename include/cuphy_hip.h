@@ -425,6 +425,96 @@ cuphyStatus_t CUPHYWINAPI cuphyHipPuschSlotDecode(cuphyLDPCDecoder_t decoder, co
                                                   cuphyDataType_t llr_type, int max_lds_values, int max_parity_nodes,
                                                   void* d_workspace, hipStream_t strm);
 
+/* ------------------------------------------------------------------------
+ * HARQ serving engine (no cuphy.h counterpart: cuPHY is driven slot by slot
+ * by its caller).  Transmissions of transport blocks are submitted as they
+ * arrive; the engine keeps each HARQ process's soft buffer (binary16,
+ * [2Zc | Ncb] per CB) in a device pool of a fixed byte budget, admits a new
+ * TB only when its buffer fits (admission by HBM budget), and continuously
+ * batches admitted transmissions into fused slot launches
+ * (cuphyHipPuschSlotDecode), up to max_inflight launches running at once on
+ * their own streams.  Single-threaded: every call from one host thread.
+ * ---------------------------------------------------------------------- */
+struct cuphyHipHarqServer;
+typedef struct cuphyHipHarqServer* cuphyHipHarqServer_t;
+
+#define CUPHY_HIP_HARQ_LAST_TX 0x100u   /* request flag: free the HARQ buffer after this transmission */
+
+typedef struct {
+    size_t harq_pool_bytes;          /* device bytes for HARQ soft buffers */
+    int max_inflight;                /* launches in flight at once, 1..8 */
+    int max_tbs_per_launch;          /* 1..65536 */
+    int max_cbs_per_launch;          /* >= 1 */
+    cuphyDataType_t llr_type;        /* element type of every request's LLRs */
+} cuphyHipHarqServerConfig_t;
+
+typedef struct {
+    uint64_t harq_id;                /* HARQ process; one TB at a time */
+    uint64_t tag;                    /* returned in the completion */
+    PerTbParams prm;                 /* this transmission's parameters (ndi, rv/k0, G, ...) */
+    const void* llr;                 /* device: this transmission's LLRs, rate-match input layout */
+    uint32_t* cb_bits;               /* device outputs as in cuphyHipPuschSlotTb_t */
+    uint32_t* cb_crc;
+    uint8_t* tb_bytes;
+    uint8_t* cb_iterations;          /* optional, NULL for none */
+    int16_t num_parity_nodes;        /* decoder mb: cover every position received so far */
+    int16_t max_iterations;
+    uint16_t flags;                  /* CUPHY_LDPC_DECODE_EARLY_TERM | CUPHY_HIP_HARQ_LAST_TX */
+    uint16_t reserved;
+    float llr_scale;
+    cuphyLDPCNormalization_t norm;   /* f16x2 */
+    float clamp_value;
+} cuphyHipHarqRequest_t;
+
+typedef struct {
+    uint64_t tag;
+    uint64_t harq_id;
+    uint32_t tb_crc;                 /* TB CRC remainder, 0 = pass */
+    int32_t status;                  /* CUPHY_STATUS_SUCCESS, or why the request was rejected */
+    double t_submit_us;              /* host steady clock, microseconds since server creation */
+    double t_launch_us;
+    double t_done_us;                /* when the engine observed the launch complete */
+} cuphyHipHarqCompletion_t;
+
+typedef struct {
+    uint64_t submitted, completed, launches;
+    uint64_t pool_bytes_in_use, pool_bytes_peak;
+    uint64_t pending;                /* submitted, not yet launched */
+    uint64_t admission_waits;        /* polls at which a new TB waited for pool space */
+} cuphyHipHarqServerStats_t;
+
+cuphyStatus_t CUPHYWINAPI cuphyHipHarqServerCreate(cuphyLDPCDecoder_t decoder, const cuphyHipHarqServerConfig_t* cfg,
+                                                   cuphyHipHarqServer_t* server);
+cuphyStatus_t CUPHYWINAPI cuphyHipHarqServerDestroy(cuphyHipHarqServer_t server);
+/* Queue one transmission (copied); returns at once. */
+cuphyStatus_t CUPHYWINAPI cuphyHipHarqServerSubmit(cuphyHipHarqServer_t server, const cuphyHipHarqRequest_t* req);
+/* Non-blocking progress: retire finished launches (their completions go to
+ * out[0 .. *n_out-1]; the rest wait for the next call), admit pending
+ * requests, start launches on free streams. */
+cuphyStatus_t CUPHYWINAPI cuphyHipHarqServerPoll(cuphyHipHarqServer_t server, cuphyHipHarqCompletion_t* out,
+                                                 int max_out, int* n_out);
+/* Free a HARQ process's buffer (e.g. the MAC gave up on the TB). */
+cuphyStatus_t CUPHYWINAPI cuphyHipHarqServerRelease(cuphyHipHarqServer_t server, uint64_t harq_id);
+cuphyStatus_t CUPHYWINAPI cuphyHipHarqServerGetStats(cuphyHipHarqServer_t server, cuphyHipHarqServerStats_t* stats);
+
+/* Load-test driver: run a trace of transmissions in real time through the
+ * engine.  trace[i].arrival_us >= 0: a first transmission, released at that
+ * time (microseconds from the call); arrival_us < 0: a retransmission,
+ * released harq_rtt_us after its predecessor (the entry whose next == i)
+ * completed with a failing TB CRC.  done[i] receives entry i's completion
+ * (tag = i, status CUPHY_STATUS_NOT_SUPPORTED if never released);
+ * *n_done = completions.  Returns when every released entry completed. */
+typedef struct {
+    cuphyHipHarqRequest_t req;
+    double arrival_us;
+    int32_t next;                    /* index of the TB's next transmission, -1: none */
+    int32_t reserved;
+} cuphyHipHarqTraceEntry_t;
+
+cuphyStatus_t CUPHYWINAPI cuphyHipHarqServeTrace(cuphyHipHarqServer_t server, const cuphyHipHarqTraceEntry_t* trace,
+                                                 int n, double harq_rtt_us, cuphyHipHarqCompletion_t* done,
+                                                 int* n_done);
+
 #ifdef __cplusplus
 }
 #endif

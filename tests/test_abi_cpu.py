@@ -130,7 +130,12 @@ _MIRRORS = [("LDPCDecodeConfigDesc", "cuphyLDPCDecodeConfigDesc_t"),
             ("PuschSlotTb", "cuphyHipPuschSlotTb_t"),
             ("TensorPrm", "cuphyTensorPrm_t"),
             ("RateMatchLaunchCfg", "cuphyPuschRxRateMatchLaunchCfg_t"),
-            ("CrcDecodeLaunchCfg", "cuphyPuschRxCrcDecodeLaunchCfg_t")]
+            ("CrcDecodeLaunchCfg", "cuphyPuschRxCrcDecodeLaunchCfg_t"),
+            ("HarqServerConfig", "cuphyHipHarqServerConfig_t"),
+            ("HarqRequest", "cuphyHipHarqRequest_t"),
+            ("HarqCompletion", "cuphyHipHarqCompletion_t"),
+            ("HarqServerStats", "cuphyHipHarqServerStats_t"),
+            ("HarqTraceEntry", "cuphyHipHarqTraceEntry_t")]
 
 
 def test_ctypes_layouts_match_the_c_header(tmp_path):
@@ -159,3 +164,19 @@ def test_ctypes_layouts_match_the_c_header(tmp_path):
     got = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
     bad = [(name, want, g) for (name, want), g in zip(expect, got) if want != g]
     assert len(got) == len(expect) and not bad, bad
+
+
+def test_harq_server_argument_validation(L):
+    """Creation arguments are checked before any device call."""
+    lib = L.lib()
+    INV = L.CUPHY_STATUS_INVALID_ARGUMENT
+    cfg = L.HarqServerConfig(harq_pool_bytes=1 << 20, max_inflight=2, max_tbs_per_launch=16, max_cbs_per_launch=64,
+                             llr_type=L.CUPHY_R_16F)
+    h = C.c_void_p()
+    assert lib.cuphyHipHarqServerCreate(None, C.byref(cfg), C.byref(h)) == INV
+    assert lib.cuphyHipHarqServerDestroy(None) == INV
+    assert lib.cuphyHipHarqServerSubmit(None, None) == INV
+    n = C.c_int()
+    assert lib.cuphyHipHarqServerPoll(None, None, 0, C.byref(n)) == INV
+    assert lib.cuphyHipHarqServerRelease(None, 1) == INV
+    assert lib.cuphyHipHarqServeTrace(None, None, 0, 0.0, None, C.byref(n)) == INV

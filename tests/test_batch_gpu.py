@@ -84,6 +84,13 @@ def test_batch_order_independent(dec):
     BatchDecoder(dec, split_by_mb=False).build(tbs, llr_type=L.CUPHY_R_16F, fp16_math=True, max_iterations=7).launch()
     torch.cuda.synchronize()
     check(tbs, refs)
+    for tb in tbs:
+        tb.out.fill_(-1)
+    b = BatchDecoder(dec, split_min=1).build(tbs, llr_type=L.CUPHY_R_16F, fp16_math=True, max_iterations=7)
+    assert len(b.subs) == 2  # high-rate CBs on the register-lean variant, concurrently
+    b.launch()
+    torch.cuda.synchronize()
+    check(tbs, refs)
 
 
 def test_batch_graph_replay(dec):

@@ -1,0 +1,159 @@
+"""HARQ serving engine: continuous batching with device-resident HARQ
+buffers.  Each TB's results in a mix of overlapping arrivals (concurrent
+launches, retransmissions combined across launches) are bit-identical to
+decoding that TB alone, transmission by transmission, with the fused slot
+decoder; admission by HBM budget holds the pool to its size."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def dec():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from aerial_amd.decoder import LdpcDecoder
+    d = LdpcDecoder()
+    yield d
+    d.close()
+
+
+@pytest.fixture(scope="module")
+def templates(dec):
+    from aerial_amd.server import TraceTemplates, serving_specs
+    return TraceTemplates(serving_specs(24, seed=31), seed=31)
+
+
+def alone(dec, templates, ti, n_tx, max_iterations):
+    """Decode template ti's first n_tx transmissions alone, in order."""
+    from aerial_amd.server import RV_SEQUENCE
+    from aerial_amd.slot import SlotDecoder, SlotTb
+    tb = templates.tbs[ti]
+    p0, _ = tb.params(0, 1)
+    harq = torch.zeros(tb.C * p0.Ncb_padded, dtype=torch.float16, device="cuda")
+    outs = SlotTb.alloc_outputs(p0)
+    sd = SlotDecoder(dec)
+    for k in range(n_tx):
+        rv = RV_SEQUENCE[k]
+        p, _ = tb.params(rv, 1 if k == 0 else 0)
+        st = SlotTb(p, tb.llrs[rv], harq, templates.mbs[ti][k], max_iterations=max_iterations, early_term=True,
+                    outputs=outs)
+        sd.build([st]).launch()
+    torch.cuda.synchronize()
+    return outs
+
+
+def check_against_alone(dec, templates, done, meta, outs, arrivals, max_iterations):
+    from aerial_amd import _lib as L
+    n_tx = {}
+    for i, (a, k, ti) in enumerate(meta):
+        if done[i].status == L.CUPHY_STATUS_SUCCESS:
+            n_tx[a] = max(n_tx.get(a, 0), k + 1)
+    assert len(n_tx) == len(arrivals), "every TB got at least its first transmission"
+    for a, (_, _, ti) in enumerate(arrivals):
+        ref = alone(dec, templates, ti, n_tx[a], max_iterations)
+        tb = templates.tbs[ti]
+        nw = (tb.Kp + 31) // 32
+        got = outs[a]
+        assert torch.equal(got[0][:, :nw], ref[0][:, :nw]), (a, ti)
+        for j in (1, 2, 3, 4):
+            assert torch.equal(got[j], ref[j]), (a, ti, j)
+        # the completion's TB CRC is the one the alone run produced last
+        last = 4 * a + n_tx[a] - 1
+        assert done[last].tb_crc == int(ref[3].cpu().numpy().view(np.uint32)[0]), (a, ti)
+
+
+def test_results_equal_alone_and_in_the_mix(dec, templates):
+    from aerial_amd.server import HarqServer, build_trace, poisson_arrivals
+    print("[case] server: 40 overlapping TBs, rtt 300us", flush=True)
+    srv = HarqServer(dec, pool_bytes=1 << 30, max_inflight=3, max_tbs=64, max_cbs=1024)
+    arrivals = poisson_arrivals(n_cells=8, rate_per_s=200000.0, duration_us=600.0, n_templates=len(templates.tbs),
+                                seed=5)
+    arrivals = arrivals[:40]
+    assert len(arrivals) == 40
+    entries, outs, meta = build_trace(srv, templates, arrivals, max_iterations=8)
+    done = srv.serve_trace(entries, harq_rtt_us=300.0)
+    st = srv.stats()
+    assert st["pending"] == 0 and st["pool_bytes_in_use"] == 0, st
+    assert st["launches"] >= 2
+    check_against_alone(dec, templates, done, meta, outs, arrivals, 8)
+    # some TBs needed retransmissions, and overlapping launches happened
+    ks = [k for i, (a, k, ti) in enumerate(meta) if done[i].status == 0]
+    assert max(ks) >= 1
+    srv.close()
+
+
+def test_admission_by_hbm_budget(dec, templates):
+    """A pool that holds about two TBs still serves a burst of ten; its peak
+    use never exceeds its size and new TBs waited for space."""
+    from aerial_amd.server import HarqServer, build_trace
+    print("[case] server: burst of 10 TBs into a small pool", flush=True)
+    sizes = []
+    for tb in templates.tbs[:10]:
+        p, _ = tb.params(0, 1)
+        sizes.append((p.firstCodeBlockIndex + p.num_CBs) * p.Ncb_padded * 2)
+    pool = (max(sizes) * 2 + 255) // 256 * 256 + 512
+    srv = HarqServer(dec, pool_bytes=pool, max_inflight=2, max_tbs=64, max_cbs=1024)
+    arrivals = [(0.0, 0, i) for i in range(10)]
+    entries, outs, meta = build_trace(srv, templates, arrivals, max_iterations=8)
+    done = srv.serve_trace(entries, harq_rtt_us=50.0)
+    st = srv.stats()
+    assert st["pool_bytes_peak"] <= pool, st
+    assert st["admission_waits"] > 0, st
+    assert st["pool_bytes_in_use"] == 0
+    check_against_alone(dec, templates, done, meta, outs, arrivals, 8)
+    srv.close()
+
+
+def test_retransmission_waits_for_its_predecessor(dec, templates):
+    """Two transmissions of one HARQ process submitted back to back run in
+    order (the second combines into the first's buffer)."""
+    from aerial_amd.server import HarqServer, RV_SEQUENCE
+    from aerial_amd.slot import SlotTb
+    print("[case] server: back-to-back submit of tx 0 and tx 1", flush=True)
+    ti = 3
+    tb = templates.tbs[ti]
+    srv = HarqServer(dec, pool_bytes=1 << 28, max_inflight=3, max_tbs=8, max_cbs=256)
+    p0, _ = tb.params(0, 1)
+    outs = SlotTb.alloc_outputs(p0)
+    for k in range(2):
+        p, _ = tb.params(RV_SEQUENCE[k], 1 if k == 0 else 0)
+        srv.submit(srv.request(77, p, tb.llrs[RV_SEQUENCE[k]], outs, templates.mbs[ti][k], max_iterations=8,
+                               tag=k, last=(k == 1)))
+    got = []
+    while len(got) < 2:
+        got += srv.poll()
+    assert [c.tag for c in got] == [0, 1]
+    torch.cuda.synchronize()
+    ref = alone(dec, templates, ti, 2, 8)
+    nw = (tb.Kp + 31) // 32
+    assert torch.equal(outs[0][:, :nw], ref[0][:, :nw])
+    assert torch.equal(outs[3], ref[3])
+    assert srv.stats()["pool_bytes_in_use"] == 0  # LAST_TX freed the buffer
+    srv.close()
+
+
+def test_invalid_requests_rejected(dec, templates):
+    from aerial_amd import _lib as L
+    from aerial_amd.server import HarqServer
+    from aerial_amd.slot import SlotTb
+    print("[case] server: request validation", flush=True)
+    srv = HarqServer(dec, pool_bytes=1 << 24, max_inflight=1, max_tbs=4, max_cbs=64)
+    tb = templates.tbs[0]
+    p, _ = tb.params(0, 1)
+    outs = SlotTb.alloc_outputs(p)
+    r = srv.request(1, p, tb.llrs[0], outs, 4)
+    r.num_parity_nodes = 3  # below the base graph's 4 core rows
+    assert L.lib().cuphyHipHarqServerSubmit(srv.h, r) == L.CUPHY_STATUS_INVALID_ARGUMENT
+    with pytest.raises(L.CuphyError):
+        srv.release(12345)
+    # a retransmission of an unknown HARQ process completes with an error
+    p1, _ = tb.params(2, 0)
+    srv.submit(srv.request(999, p1, tb.llrs[2], outs, templates.mbs[0][1], tag=5))
+    got = []
+    while not got:
+        got = srv.poll()
+    assert got[0].tag == 5 and got[0].status == L.CUPHY_STATUS_INVALID_ARGUMENT
+    srv.close()
