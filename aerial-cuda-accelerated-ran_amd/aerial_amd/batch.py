@@ -38,8 +38,8 @@ class BatchDecoder:
 
     Code blocks with mb <= 8 (high code rates) go to the register-lean
     kernel variant when there are at least `split_min` of them (enough to
-    fill the GPU on their own), the rest to the full one; the two launches
-    run on two streams (fork/join), so they share the GPU concurrently.  A
+    fill the GPU on their own), the rest to the full one; the launches run
+    on separate streams (fork/join) and share the GPU concurrently.  A
     smaller high-rate class joins the full variant's launch: measured on the
     16-UE heterogeneous slot, one launch is faster than two there."""
 
@@ -54,7 +54,7 @@ class BatchDecoder:
         self._ws_words = sz.value // 4
         self.subs = []
         self.graph = None
-        self.aux_stream = torch.cuda.Stream()
+        self.aux_streams = [torch.cuda.Stream()]
 
     @property
     def n_items(self):
@@ -126,11 +126,15 @@ class BatchDecoder:
             return
         if len(self.subs) == 1:
             return self._launch_one(self.subs[0], main)
-        # fork: the second variant runs on the auxiliary stream, then join
-        self.aux_stream.wait_stream(main)
+        # fork: the other variants run on auxiliary streams, then join
+        aux = self.aux_streams[:len(self.subs) - 1]
+        for a in aux:
+            a.wait_stream(main)
         self._launch_one(self.subs[0], main)
-        self._launch_one(self.subs[1], self.aux_stream)
-        main.wait_stream(self.aux_stream)
+        for sb, a in zip(self.subs[1:], aux):
+            self._launch_one(sb, a)
+        for a in aux:
+            main.wait_stream(a)
 
     def capture(self):
         """Capture one launch into a HIP graph (after a warm-up launch)."""

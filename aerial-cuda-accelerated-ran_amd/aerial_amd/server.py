@@ -60,13 +60,16 @@ class HarqServer:
     def request(self, harq_id, params, llr, outputs, mb, max_iterations=10, early_term=True, llr_scale=1.0,
                 clamp_value=32.0, last=False, tag=0):
         """A cuphyHipHarqRequest_t; outputs = (cb_bits, cb_crc, tb_bytes,
-        cb_iterations) device tensors (SlotTb.alloc_outputs layout)."""
+        tb_crc, cb_iterations) device tensors as SlotTb.alloc_outputs makes
+        them (the engine reports the TB CRC in the completion; tb_crc and
+        cb_iterations may be None)."""
         r = L.HarqRequest()
         r.harq_id = harq_id
         r.tag = tag
         r.prm = params
         r.llr = llr.data_ptr()
-        cb_bits, cb_crc, tb_bytes, cb_iters = outputs[:4]
+        cb_bits, cb_crc, tb_bytes = outputs[:3]
+        cb_iters = outputs[4] if len(outputs) > 4 else None
         r.cb_bits = cb_bits.data_ptr()
         r.cb_crc = cb_crc.data_ptr()
         r.tb_bytes = tb_bytes.data_ptr()

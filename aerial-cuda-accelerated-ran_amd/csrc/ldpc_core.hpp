@@ -362,4 +362,46 @@ __device__ __forceinline__ int decode_one(T* __restrict__ app, const cb_job& job
     return iters;
 }
 
+// The code block of a transport-block launch handled by this workgroup.
+template <typename TIn>
+__device__ __forceinline__ cb_job tb_job(const cuphyLDPCDecodeDesc_t& desc)
+{
+    const cuphyLDPCDecodeConfigDesc_t& cfg = desc.config;
+    const cw_location loc = locate_cw(desc, blockIdx.x);
+    cb_job job;
+    job.llr = static_cast<const TIn*>(desc.llr_input[loc.tb].addr) +
+              (size_t)loc.idx * (size_t)desc.llr_input[loc.tb].stride_elements;
+    job.out = desc.tb_output[loc.tb].addr + (size_t)loc.idx * (size_t)desc.tb_output[loc.tb].stride_words;
+    job.soft = ((cfg.flags & CUPHY_LDPC_DECODE_WRITE_SOFT_OUTPUTS) && desc.llr_output[loc.tb].addr)
+                   ? static_cast<_Float16*>(desc.llr_output[loc.tb].addr) +
+                         (size_t)loc.idx * (size_t)desc.llr_output[loc.tb].stride_elements
+                   : nullptr;
+    job.Z = cfg.Z;
+    job.mb = cfg.num_parity_nodes;
+    job.Kb = cfg.Kb;
+    job.max_iter = cfg.max_iterations;
+    job.early_term = (cfg.flags & CUPHY_LDPC_DECODE_EARLY_TERM) != 0;
+    job.clamp = cfg.clamp_value;
+    job.alpha_f32 = cfg.norm.f32;
+    job.alpha_f16 = (uint16_t)(cfg.norm.f16x2 & 0xFFFFu);
+    return job;
+}
+
+__device__ __forceinline__ cb_job item_job(const cuphyHipLDPCWorkItem_t& w)
+{
+    cb_job job;
+    job.llr = w.llr;
+    job.out = w.out;
+    job.soft = static_cast<_Float16*>(w.soft);
+    job.Z = w.Z;
+    job.mb = w.num_parity_nodes;
+    job.Kb = w.Kb;
+    job.max_iter = w.max_iterations;
+    job.early_term = (w.flags & CUPHY_LDPC_DECODE_EARLY_TERM) != 0;
+    job.clamp = w.clamp_value;
+    job.alpha_f32 = w.norm.f32;
+    job.alpha_f16 = (uint16_t)(w.norm.f16x2 & 0xFFFFu);
+    return job;
+}
+
 }  // namespace nrldpc

@@ -47,25 +47,7 @@ template <int BG, typename T, typename TIn, int MAXR>
 __global__ __launch_bounds__(384) void ldpc_layered_tb(cuphyLDPCDecodeDesc_t desc)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    const cuphyLDPCDecodeConfigDesc_t& cfg = desc.config;
-    const cw_location loc = locate_cw(desc, blockIdx.x);
-    cb_job job;
-    job.llr = static_cast<const TIn*>(desc.llr_input[loc.tb].addr) +
-              (size_t)loc.idx * (size_t)desc.llr_input[loc.tb].stride_elements;
-    job.out = desc.tb_output[loc.tb].addr + (size_t)loc.idx * (size_t)desc.tb_output[loc.tb].stride_words;
-    job.soft = ((cfg.flags & CUPHY_LDPC_DECODE_WRITE_SOFT_OUTPUTS) && desc.llr_output[loc.tb].addr)
-                   ? static_cast<_Float16*>(desc.llr_output[loc.tb].addr) +
-                         (size_t)loc.idx * (size_t)desc.llr_output[loc.tb].stride_elements
-                   : nullptr;
-    job.Z = cfg.Z;
-    job.mb = cfg.num_parity_nodes;
-    job.Kb = cfg.Kb;
-    job.max_iter = cfg.max_iterations;
-    job.early_term = (cfg.flags & CUPHY_LDPC_DECODE_EARLY_TERM) != 0;
-    job.clamp = cfg.clamp_value;
-    job.alpha_f32 = cfg.norm.f32;
-    job.alpha_f16 = (uint16_t)(cfg.norm.f16x2 & 0xFFFFu);
-    decode_one<BG, T, TIn, MAXR>(reinterpret_cast<T*>(lds_raw), job);
+    decode_one<BG, T, TIn, MAXR>(reinterpret_cast<T*>(lds_raw), tb_job<TIn>(desc));
 }
 
 // Persistent heterogeneous decoder: each workgroup repeatedly takes the
@@ -87,18 +69,7 @@ __global__ __launch_bounds__(384) void ldpc_layered_persistent(const cuphyHipLDP
         __syncthreads();
         if (i >= n_items) break;
         const cuphyHipLDPCWorkItem_t& w = items[i];
-        cb_job job;
-        job.llr = w.llr;
-        job.out = w.out;
-        job.soft = static_cast<_Float16*>(w.soft);
-        job.Z = w.Z;
-        job.mb = w.num_parity_nodes;
-        job.Kb = w.Kb;
-        job.max_iter = w.max_iterations;
-        job.early_term = (w.flags & CUPHY_LDPC_DECODE_EARLY_TERM) != 0;
-        job.clamp = w.clamp_value;
-        job.alpha_f32 = w.norm.f32;
-        job.alpha_f16 = (uint16_t)(w.norm.f16x2 & 0xFFFFu);
+        const cb_job job = item_job(w);
         if (w.BG == 1)
             decode_one<1, T, TIn, MAXR>(app, job);
         else
@@ -239,15 +210,17 @@ cuphyStatus_t decode_batch(const cuphyHipLDPCWorkItem_t* d_items, int n_items, c
     const bool f16 = fp16_math || llr_type == CUPHY_R_16F;
     void* func = max_mb <= SMALL_MAXR ? select_persistent_kernel<SMALL_MAXR>(llr_type, f16)
                                       : select_persistent_kernel<46>(llr_type, f16);
+    const unsigned block = 384;
     const size_t elem = f16 ? 2 : 4;
     const size_t lds = (size_t)max_values_per_cb * elem;
     cuphyStatus_t s = ensure_lds_attr(func, lds);
     if (s != CUPHY_STATUS_SUCCESS) return s;
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, func, 384, lds) != hipSuccess || per_cu < 1) per_cu = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, func, (int)block, lds) != hipSuccess || per_cu < 1)
+        per_cu = 1;
     const int grid = std::min(n_items, std::max(1, cu_count) * per_cu);
     void* args[3] = {const_cast<cuphyHipLDPCWorkItem_t**>(&d_items), &n_items, &d_queue};
-    if (hipLaunchKernel(func, dim3((unsigned)grid), dim3(384), args, lds, strm) != hipSuccess)
+    if (hipLaunchKernel(func, dim3((unsigned)grid), dim3(block), args, lds, strm) != hipSuccess)
         return CUPHY_STATUS_INTERNAL_ERROR;
     return CUPHY_STATUS_SUCCESS;
 }

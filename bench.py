@@ -41,7 +41,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU work budget of the baseline leg")
     ap.add_argument("--sections", default="main,config1,config3,fused,serving",
-                    help="comma-separated subset of: main, config1, config3, fused, serving")
+                    help="comma-separated subset of: main, config1, config3, fused, serving; or 'copyprobe' "
+                         "alone: three 1 GiB device copies (calibrates HBM byte counters)")
     ap.add_argument("--serve-ms", type=float, default=40.0, help="arrival window of each serving load point")
     return ap.parse_args()
 
@@ -184,9 +185,38 @@ def cpu_baseline(tbs, budget_s):
                       (passes, sum(tb.ncb for tb in tbs), bits_per_pass)}
 
 
+def measured_traffic():
+    """HBM bytes per launch of the batched decoder from the committed
+    counter summary (FETCH_SIZE and WRITE_SIZE passes, copy-calibrated), or
+    None when it is absent."""
+    import re
+    path = os.path.join(ROOT, "profiles", "ldpc_decoder_hetero_slot_hbm_traffic.md")
+    try:
+        m = re.search(r"^traffic_bytes_per_launch:\s*(\d+)", open(path).read(), re.M)
+    except OSError:
+        return None
+    return int(m.group(1)) if m else None
+
+
+def copy_probe():
+    """Three device-to-device copies of 1 GiB each: a kernel of known HBM
+    traffic (1 GiB read + 1 GiB written per dispatch) for calibrating the
+    FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = 1 << 30
+    src = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    return {"copy_probe_bytes_per_dispatch": n, "dispatches": 3}
+
+
 def main():
     a = parse()
     sections = set(a.sections.split(","))
+    if sections == {"copyprobe"}:
+        return copy_probe()
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
               "(the decode path shards by cell with no collective -- see the design notes)" % a.gpus,
@@ -293,7 +323,7 @@ def main():
     achieved_tops = edge_updates * OPS_PER_EDGE / (batched_ms * 1e-3) / 1e12
     plain_tops = edge_updates * OPS_PER_EDGE / (kernel_ms * 1e-3) / 1e12
     roofline = {"bound": "valu", "achieved": round(achieved_tops, 3), "peak": round(VALU_PEAK_TOPS, 2),
-                "unit": "TOP/s", "frac": round(achieved_tops / VALU_PEAK_TOPS, 4), "traffic": None,
+                "unit": "TOP/s", "frac": round(achieved_tops / VALU_PEAK_TOPS, 4), "traffic": measured_traffic(),
                 "kernel": "nrldpc::ldpc_layered_persistent<_Float16, _Float16>",
                 "work": "%d edge updates x %d VALU ops per launch" % (edge_updates, OPS_PER_EDGE),
                 "avg_launch_us": round(batched_ms * 1e3, 2),

@@ -115,6 +115,7 @@ def test_fp32_input_fp16_math_bit_exact(dec):
 SWEEP = [
     (1, 2, 4), (1, 3, 46), (1, 15, 10), (1, 36, 20), (1, 64, 4), (1, 104, 33), (1, 208, 46), (1, 352, 5),
     (2, 2, 42), (2, 7, 4), (2, 30, 12), (2, 64, 42), (2, 120, 8), (2, 240, 25), (2, 384, 4), (2, 384, 42),
+    (1, 288, 46), (2, 288, 9), (1, 320, 13), (2, 320, 30), (1, 384, 7), (1, 384, 30), (2, 352, 42),
 ]
 
 
@@ -161,6 +162,18 @@ def test_early_termination_matches_oracle(dec):
     h_gpu, s_gpu = run_gpu(dec, 1, 256, 12, llr, n_iter=20, flags=L.CUPHY_LDPC_DECODE_EARLY_TERM)
     h_ref, s_ref, iters = run_oracle(1, 256, 12, llr, n_iter=20, et=True)
     assert max(iters) < 20, iters  # ET actually stopped early
+    assert np.array_equal(h_gpu, h_ref)
+    assert_soft_equal(s_gpu, s_ref)
+
+
+def test_early_termination_fp16_large_z(dec):
+    from aerial_amd import _lib as L
+    _log("BG1 Z=352 mb=20 ET fp16, 4 CBs")
+    llr, info = V.make_cbs(1, 352, 20, 4, seed=19, snr_db=3.0)
+    llr = V.to_half_rn(llr).astype(np.float32)
+    h_gpu, s_gpu = run_gpu(dec, 1, 352, 20, llr, n_iter=20, flags=L.CUPHY_LDPC_DECODE_EARLY_TERM, dtype="fp16")
+    h_ref, s_ref, iters = run_oracle(1, 352, 20, llr, n_iter=20, et=True, mode=O.FP16)
+    assert max(iters) < 20, iters
     assert np.array_equal(h_gpu, h_ref)
     assert_soft_equal(s_gpu, s_ref)
 

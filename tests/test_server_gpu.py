@@ -58,7 +58,7 @@ def check_against_alone(dec, templates, done, meta, outs, arrivals, max_iteratio
         nw = (tb.Kp + 31) // 32
         got = outs[a]
         assert torch.equal(got[0][:, :nw], ref[0][:, :nw]), (a, ti)
-        for j in (1, 2, 3, 4):
+        for j in (1, 2, 4):  # CB CRCs, TB bytes, iterations per CB
             assert torch.equal(got[j], ref[j]), (a, ti, j)
         # the completion's TB CRC is the one the alone run produced last
         last = 4 * a + n_tx[a] - 1
@@ -113,7 +113,12 @@ def test_retransmission_waits_for_its_predecessor(dec, templates):
     from aerial_amd.server import HarqServer, RV_SEQUENCE
     from aerial_amd.slot import SlotTb
     print("[case] server: back-to-back submit of tx 0 and tx 1", flush=True)
-    ti = 3
+    # a TB whose first transmission fails (a passing TB's buffer is freed
+    # and its retransmission rejected, as the next test checks)
+    ti = next((i for i in range(len(templates.tbs))
+               if int(alone(dec, templates, i, 1, 8)[3].cpu().numpy().view(np.uint32)[0]) != 0), None)
+    if ti is None:
+        pytest.skip("no template fails its first transmission")
     tb = templates.tbs[ti]
     srv = HarqServer(dec, pool_bytes=1 << 28, max_inflight=3, max_tbs=8, max_cbs=256)
     p0, _ = tb.params(0, 1)
@@ -130,7 +135,7 @@ def test_retransmission_waits_for_its_predecessor(dec, templates):
     ref = alone(dec, templates, ti, 2, 8)
     nw = (tb.Kp + 31) // 32
     assert torch.equal(outs[0][:, :nw], ref[0][:, :nw])
-    assert torch.equal(outs[3], ref[3])
+    assert got[1].tb_crc == int(ref[3].cpu().numpy().view(np.uint32)[0])
     assert srv.stats()["pool_bytes_in_use"] == 0  # LAST_TX freed the buffer
     srv.close()
 
@@ -149,7 +154,8 @@ def test_invalid_requests_rejected(dec, templates):
     assert L.lib().cuphyHipHarqServerSubmit(srv.h, r) == L.CUPHY_STATUS_INVALID_ARGUMENT
     with pytest.raises(L.CuphyError):
         srv.release(12345)
-    # a retransmission of an unknown HARQ process completes with an error
+    # a retransmission of an unknown (or already decoded) HARQ process
+    # completes with an error
     p1, _ = tb.params(2, 0)
     srv.submit(srv.request(999, p1, tb.llrs[2], outs, templates.mbs[0][1], tag=5))
     got = []
