@@ -121,7 +121,7 @@ class PuschSlotTb(C.Structure):
                 ("cb_crc", C.c_void_p), ("tb_bytes", C.c_void_p), ("tb_crc", C.c_void_p),
                 ("cb_iterations", C.c_void_p), ("num_parity_nodes", C.c_int16), ("max_iterations", C.c_int16),
                 ("flags", C.c_uint16), ("reserved", C.c_uint16), ("llr_scale", C.c_float), ("norm", Norm),
-                ("clamp_value", C.c_float)]
+                ("clamp_value", C.c_float), ("share_offset", C.c_uint32)]
 
 
 class HarqServerConfig(C.Structure):

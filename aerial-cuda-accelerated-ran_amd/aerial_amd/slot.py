@@ -92,9 +92,10 @@ class SlotDecoder:
     two streams: the mb <= 8 register-lean variant when at least split_min
     CBs (enough to fill the GPU) have mb <= 8, and the full variant."""
 
-    def __init__(self, decoder, split_min=256):
+    def __init__(self, decoder, split_min=256, concurrent=True):
         self.dec = decoder
         self.split_min = split_min
+        self.concurrent = concurrent
         self._lib = L.lib()
         self.launches = []
         self.graph = None
@@ -131,6 +132,7 @@ class SlotDecoder:
         self._dev_params = torch.from_numpy(np.frombuffer(bytes(parr), dtype=np.uint8).copy()).cuda()
         psize = C.sizeof(L.PerTbParams)
         jobs = (L.PuschSlotTb * n)()
+        share_offset = 0
         n_small = sum(tb.params.num_CBs for tb in tbs if tb.mb <= SMALL_MAXR)
         n_all = sum(tb.params.num_CBs for tb in tbs)
         split = n_small >= self.split_min or n_small == n_all
@@ -153,6 +155,8 @@ class SlotDecoder:
             h = self._alpha_bits(tb)
             j.norm.f16x2 = h | (h << 16)
             j.clamp_value = tb.clamp_value
+            j.share_offset = share_offset
+            share_offset += p.num_CBs
             cost = p.Zc * num_edges(p.bg, tb.mb) * max(1, tb.max_iterations)
             key = (tb.mb <= SMALL_MAXR) if split else False
             for c in range(p.num_CBs):
@@ -161,7 +165,7 @@ class SlotDecoder:
         self.n_tbs = n
         sz = C.c_size_t()
         L.check("cuphyHipPuschSlotDecodeGetWorkspaceSize",
-                self._lib.cuphyHipPuschSlotDecodeGetWorkspaceSize(n, C.byref(sz)))
+                self._lib.cuphyHipPuschSlotDecodeGetWorkspaceSize(share_offset, C.byref(sz)))
         self.launches = []
         for key in sorted(classes, key=lambda k: not k):
             rows = classes[key]
@@ -194,8 +198,10 @@ class SlotDecoder:
         main = stream if stream is not None else torch.cuda.current_stream()
         if not self.launches:
             return
-        if len(self.launches) == 1:
-            return self._launch_one(self.launches[0], main)
+        if len(self.launches) == 1 or not self.concurrent:
+            for ln in self.launches:
+                self._launch_one(ln, main)
+            return
         self.aux_stream.wait_stream(main)
         self._launch_one(self.launches[0], main)
         self._launch_one(self.launches[1], self.aux_stream)

@@ -77,6 +77,20 @@ __device__ __forceinline__ uint32_t dmulmod(uint32_t a, uint32_t b, int L, uint3
     return r;
 }
 
+// a*b mod g for a compile-time polynomial (fully unrolled).
+template <int L, uint32_t G>
+__device__ __forceinline__ uint32_t cmulmod(uint32_t a, uint32_t b)
+{
+    constexpr uint32_t mask = (1u << L) - 1u;
+    uint32_t r = 0;
+#pragma unroll
+    for (int i = L - 1; i >= 0; --i) {
+        r = ((r << 1) & mask) ^ ((r >> (L - 1)) ? G : 0u);
+        r ^= ((b >> i) & 1u) ? a : 0u;
+    }
+    return r;
+}
+
 __device__ __forceinline__ uint32_t rbit8x4(uint32_t w)
 {
     // reverse the bits inside each byte (byte order kept)

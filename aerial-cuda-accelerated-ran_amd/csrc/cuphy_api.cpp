@@ -579,10 +579,10 @@ cuphyStatus_t CUPHYWINAPI cuphyHipLDPCBatchDecode(cuphyLDPCDecoder_t decoder, co
                                 static_cast<uint32_t*>(d_workspace), decoder->cu_count, strm);
 }
 
-cuphyStatus_t CUPHYWINAPI cuphyHipPuschSlotDecodeGetWorkspaceSize(int num_tbs, size_t* sizeInBytes)
+cuphyStatus_t CUPHYWINAPI cuphyHipPuschSlotDecodeGetWorkspaceSize(int num_cbs, size_t* sizeInBytes)
 {
-    if (!sizeInBytes || num_tbs < 0 || num_tbs > 65536) return CUPHY_STATUS_INVALID_ARGUMENT;
-    *sizeInBytes = (size_t)(nrldpc::FUSED_WS_QUEUE_WORDS + 2 * num_tbs) * sizeof(uint32_t);
+    if (!sizeInBytes || num_cbs < 0) return CUPHY_STATUS_INVALID_ARGUMENT;
+    *sizeInBytes = (size_t)(nrldpc::FUSED_WS_QUEUE_WORDS + (size_t)num_cbs) * sizeof(uint32_t);
     return CUPHY_STATUS_SUCCESS;
 }
 

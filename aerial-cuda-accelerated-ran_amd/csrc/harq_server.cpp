@@ -306,6 +306,7 @@ cuphyStatus_t cuphyHipHarqServer::launch(launch_slot& s)
     };
     std::vector<cost_item> small, big;
     int lds_small = 1, lds_big = 1, mb_small = 4, mb_big = 4;
+    uint32_t share_offset = 0;
     for (int t = 0; t < n; ++t) {
         const cuphyHipHarqRequest_t& r = s.reqs[t].req;
         h_params[t] = r.prm;
@@ -325,6 +326,8 @@ cuphyStatus_t cuphyHipHarqServer::launch(launch_slot& s)
         j.llr_scale = r.llr_scale;
         j.norm = r.norm;
         j.clamp_value = r.clamp_value;
+        j.share_offset = share_offset;
+        share_offset += r.prm.num_CBs;
         const PerTbParams& p = r.prm;
         const int mb = r.num_parity_nodes;
         const uint64_t cost = (uint64_t)p.Zc * (uint64_t)num_edges((int)p.bg, mb) * (uint64_t)std::max(1, (int)r.max_iterations);
@@ -392,7 +395,7 @@ cuphyStatus_t CUPHYWINAPI cuphyHipHarqServerCreate(cuphyLDPCDecoder_t decoder, c
     s->cfg = *cfg;
     s->t0 = clk::now();
     size_t ws = 0;
-    cuphyHipPuschSlotDecodeGetWorkspaceSize(cfg->max_tbs_per_launch, &ws);
+    cuphyHipPuschSlotDecodeGetWorkspaceSize(cfg->max_cbs_per_launch, &ws);
     s->ws_bytes = ws;
     s->stage_bytes = s->items_off() + (size_t)cfg->max_cbs_per_launch * sizeof(uint32_t);
     bool ok = hipMalloc(&s->d_pool, cfg->harq_pool_bytes) == hipSuccess;

@@ -319,11 +319,11 @@ __device__ __forceinline__ int run_iterations(T* __restrict__ app, const cb_job&
 }
 
 // Phase 3: hard decisions of the first nbits APP values (APP <= 0 -> 1),
-// 64 per wave per step by ballot, into `out` (global, LSB-first words) and,
-// when `lds_words` is given, into LDS as well (ends with a barrier then).
+// 64 per wave per step by ballot, into `out` (global, LSB-first words).
+// Used by the decoder kernels: it needs no per-bit constants, so a
+// persistent kernel keeps its register count.
 template <typename T>
-__device__ __forceinline__ void store_hard(const T* __restrict__ app, int nbits, uint32_t* __restrict__ out,
-                                          uint32_t* __restrict__ lds_words)
+__device__ __forceinline__ void store_hard(const T* __restrict__ app, int nbits, uint32_t* __restrict__ out)
 {
     const int nwords = (nbits + 31) >> 5;
     const int tid = (int)threadIdx.x;
@@ -336,14 +336,43 @@ __device__ __forceinline__ void store_hard(const T* __restrict__ app, int nbits,
         const uint64_t m = __ballot(bit);
         if (lane == 0) {
             out[2 * c] = (uint32_t)m;
-            if (lds_words) lds_words[2 * c] = (uint32_t)m;
-            if (2 * c + 1 < nwords) {
-                out[2 * c + 1] = (uint32_t)(m >> 32);
-                if (lds_words) lds_words[2 * c + 1] = (uint32_t)(m >> 32);
-            }
+            if (2 * c + 1 < nwords) out[2 * c + 1] = (uint32_t)(m >> 32);
         }
     }
-    if (lds_words) __syncthreads();
+}
+
+// Hard decisions as LSB-first 32-bit words into `out` (global) and into
+// `lds_words` (LDS), ending with a barrier.  Thread w builds word w from its
+// 32 APP values with 16-byte LDS reads (the buffer must hold whole groups of
+// 32 values); bits past nbits are 0.  Used by the fused slot kernel, whose
+// CRC pass reads the words back from LDS.
+template <typename T>
+__device__ __forceinline__ void store_hard_words(const T* __restrict__ app, int nbits, uint32_t* __restrict__ out,
+                                                 uint32_t* __restrict__ lds_words)
+{
+    typedef T vec_t __attribute__((ext_vector_type(16 / sizeof(T))));
+    constexpr int PER = 16 / (int)sizeof(T);  // values per 16-byte read
+    const int nwords = (nbits + 31) >> 5;
+    // opaque copy of the thread index: keeps the per-thread addresses from
+    // being hoisted out of a persistent kernel's item loop into registers
+    // held across the decode
+    int w0 = (int)threadIdx.x;
+    asm volatile("" : "+v"(w0));
+    for (int w = w0; w < nwords; w += blockDim.x) {
+        const vec_t* src = reinterpret_cast<const vec_t*>(app + 32 * w);
+        uint32_t word = 0;
+#pragma unroll
+        for (int q = 0; q < 32 / PER; ++q) {
+            const vec_t v = src[q];
+#pragma unroll
+            for (int e = 0; e < PER; ++e) word |= (uint32_t)(v[e] <= (T)0) << (q * PER + e);
+        }
+        const int tail = nbits - 32 * w;  // valid bits of this word
+        if (tail < 32) word &= (1u << tail) - 1u;
+        out[w] = word;
+        lds_words[w] = word;
+    }
+    __syncthreads();
 }
 
 // Decode one code block with the whole workgroup; LDS `app` holds
@@ -355,7 +384,7 @@ __device__ __forceinline__ int decode_one(T* __restrict__ app, const cb_job& job
     load_llrs<T, TIn>(app, static_cast<const TIn*>(job.llr), nval, job.clamp);
     const int iters = run_iterations<BG, T, MAXR>(app, job);
     const int nbits = job.Kb * job.Z;
-    store_hard<T>(app, nbits, job.out, nullptr);
+    store_hard<T>(app, nbits, job.out);
     // soft outputs: Kb*Z binary16 APP values
     if (job.soft)
         for (int n = threadIdx.x; n < nbits; n += blockDim.x) job.soft[n] = (_Float16)app[n];

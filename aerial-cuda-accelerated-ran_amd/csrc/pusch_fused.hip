@@ -23,10 +23,10 @@
 //
 // The TB CRC24A is assembled from per-CB shares (CRC linearity): each CB's
 // workgroup computes the remainder of its payload bytes shifted to their
-// place in the TB and XORs it into a per-TB accumulator with an atomic; the
-// workgroup that completes the TB reads the result.  Counters and
-// accumulators return to 0, and so does the queue (reset by the last
-// workgroup to leave), so the launch is replayable from a graph.
+// place in the TB and stores it in the workspace; a small kernel after the
+// persistent one XORs each TB's shares (no cross-workgroup synchronisation
+// inside the persistent kernel).  The queue is returned to 0 by the last
+// workgroup to leave, so the two launches are replayable from a graph.
 #include "crc_core.hpp"
 #include "derate_core.hpp"
 #include "ldpc_core.hpp"
@@ -46,11 +46,65 @@ typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 
 // LDS state of a workgroup besides the HARQ/APP array.
 struct fused_lds {
-    uint32_t words[SLOT_CB_WORDS];  // hard bits of the current CB
-    uint32_t crc_t[3][256];         // byte tables of the three CRCs
-    uint32_t red[8];
-    int item, last;
+    uint32_t words[SLOT_CB_WORDS];       // hard bits of the current CB
+    uint32_t crc_t[3][256];              // byte tables of the three CRCs
+    uint32_t shift[3][CRC_SHIFT_LO];     // x^(8m) mod g of the three CRCs
+    uint32_t red[2][8];
+    int item, next;
 };
+
+// Remainder of one word's 4 message bytes (MSB-first bytes) with byte table T.
+template <int L>
+__device__ __forceinline__ uint32_t crc_word(uint32_t w, const uint32_t* T)
+{
+    constexpr uint32_t mask = (1u << L) - 1u;
+    uint32_t reg = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) reg = ((reg << 8) & mask) ^ T[((reg >> (L - 8)) ^ (w >> (8 * b))) & 0xFFu];
+    return reg;
+}
+
+// The CB CRC (kind K of L bits over nb_cb = 4 * words bytes) and the CB's
+// TB-CRC24A share (over its db payload bytes, before the shift to its
+// place in the TB) in one pass: thread t takes word t, shifts its two
+// remainders by x^(8 * bytes after it) with factors from LDS, and both are
+// XOR-reduced together.
+template <int L, uint32_t G>
+__device__ __forceinline__ void cb_and_share_crc(const fused_lds& sh, const uint32_t* T, const uint32_t* S, int nb_cb,
+                                                 int db, uint32_t* red0, uint32_t* red1, uint32_t& crc,
+                                                 uint32_t& share)
+{
+    const int t = threadIdx.x;
+    uint32_t rc = 0, rs = 0;
+    if (4 * t < nb_cb) {
+        const uint32_t w = rbit8x4(sh.words[t]);
+        rc = crc_word<L>(w, T);
+        if (rc && nb_cb > 4 * t + 4) rc = cmulmod<L, G>(rc, S[nb_cb - 4 * t - 4]);
+        if (4 * t < db) {  // payload bytes [4t, min(4t + 4, db)) for the share
+            const int nb = min(4, db - 4 * t);
+            const uint32_t* TA = sh.crc_t[K_CRC24A];
+            uint32_t reg = 0;
+            for (int b = 0; b < nb; ++b) reg = ((reg << 8) & 0xFFFFFFu) ^ TA[((reg >> 16) ^ (w >> (8 * b))) & 0xFFu];
+            rs = reg;
+            if (rs && db > 4 * t + nb) rs = cmulmod<24, CRC24A_P.g>(rs, sh.shift[K_CRC24A][db - 4 * t - nb]);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        rc ^= __shfl_xor(rc, off, 64);
+        rs ^= __shfl_xor(rs, off, 64);
+    }
+    if ((t & 63) == 0) {
+        red0[t >> 6] = rc;
+        red1[t >> 6] = rs;
+    }
+    __syncthreads();
+    crc = share = 0;
+    for (int i = 0; i < (int)((blockDim.x + 63) / 64); ++i) {
+        crc ^= red0[i];
+        share ^= red1[i];
+    }
+    __syncthreads();
+}
 
 // Combine this transmission's E LLRs of the CB into the LDS circular buffer
 // `cbuf` (passes of distinct positions, barrier after each).  Indices are
@@ -95,6 +149,54 @@ __device__ __forceinline__ void combine_tx(_Float16* __restrict__ cbuf, const TI
         }
         __syncthreads();
     }
+}
+
+template <typename TI> struct slot_vec;  // the QAM_STRIDE LLR slots of one symbol
+template <> struct slot_vec<int8_t> {
+    typedef int8_t type __attribute__((ext_vector_type(8)));
+};
+template <> struct slot_vec<_Float16> {
+    typedef _Float16 type __attribute__((ext_vector_type(8)));
+};
+template <> struct slot_vec<float> {
+    typedef float type __attribute__((ext_vector_type(8)));
+};
+
+// Combining when no circular-buffer position repeats within this
+// transmission (E <= L), the layers map one to one onto the buffer's
+// layers and the LLRs are not UCI-multiplexed: thread t takes whole symbols
+// j = t, t + nthr, ... (one vector load of the symbol's QAM_STRIDE slots,
+// all loads of a thread issued before its LDS updates); bit k of symbol j
+// is de-interleaved bit i = k * E/Qm + j, at circle position c0 + i.
+template <typename TI>
+__device__ __forceinline__ void combine_symbols(_Float16* __restrict__ cbuf, const TI* __restrict__ in,
+                                                const rm_geometry& g, float scale)
+{
+    typedef typename slot_vec<TI>::type vec_t;
+    constexpr int NB = 4;  // symbols per thread in flight
+    const int nthr = blockDim.x;
+    const vec_t* sym = reinterpret_cast<const vec_t*>(in + g.adj);
+    for (int j0 = threadIdx.x; j0 < g.EQ; j0 += NB * nthr) {
+        vec_t v[NB];
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+            const int j = j0 + u * nthr;
+            if (j < g.EQ) v[u] = sym[j];
+        }
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+            const int j = j0 + u * nthr;
+            if (j >= g.EQ) break;
+            for (int k = 0; k < g.Qm; ++k) {
+                int c = g.c0 + k * g.EQ + j;
+                if (c >= g.L) c -= g.L;
+                const int o = (c < g.Kd) ? c : c + g.F;
+                const _Float16 x = rm_num<_Float16>::clamp((_Float16)(llr_as_float(v[u][k]) * scale));
+                cbuf[o] = rm_num<_Float16>::clamp(cbuf[o] + x);
+            }
+        }
+    }
+    __syncthreads();
 }
 
 // One CB.  Returns this CB's share of the TB CRC24A (the remainder of its
@@ -142,7 +244,15 @@ __device__ __forceinline__ uint32_t fused_cb(_Float16* __restrict__ hb, fused_ld
         for (int n = tid; n < nv; n += nthr) reinterpret_cast<half8_t*>(hb)[n] = z;
         for (int n = nv * 8 + tid; n < nh; n += nthr) hb[n] = (_Float16)0.0f;
     } else {
-        for (int n = tid; n < nv; n += nthr) reinterpret_cast<half8_t*>(hb)[n] = reinterpret_cast<const half8_t*>(harq)[n];
+        for (int n0 = tid; n0 < nv; n0 += 4 * nthr) {  // four 16-byte loads in flight per thread
+            half8_t v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (n0 + u * nthr < nv) v[u] = reinterpret_cast<const half8_t*>(harq)[n0 + u * nthr];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (n0 + u * nthr < nv) reinterpret_cast<half8_t*>(hb)[n0 + u * nthr] = v[u];
+        }
         for (int n = nv * 8 + tid; n < nh; n += nthr) hb[n] = harq[n];
     }
     __syncthreads();
@@ -150,7 +260,12 @@ __device__ __forceinline__ uint32_t fused_cb(_Float16* __restrict__ hb, fused_ld
         for (int n = tid; n < g.F; n += nthr) cbuf[g.Kd + n] = (_Float16)LLR_MAX_ABS;
 
     // ---- 2. combine this transmission -------------------------------------
-    combine_tx<TI>(cbuf, in, g, p, job.llr_scale);
+    const bool sym_path = !g.uci && g.Nl == g.nbbu && g.E <= g.L &&
+                          ((uintptr_t)(in + g.adj) % sizeof(typename slot_vec<TI>::type)) == 0;
+    if (sym_path)
+        combine_symbols<TI>(cbuf, in, g, job.llr_scale);
+    else
+        combine_tx<TI>(cbuf, in, g, p, job.llr_scale);
 
     // ---- 3. write back the HARQ buffer; APP = clamp(HARQ), +0 for -0 ----
     {
@@ -196,16 +311,25 @@ __device__ __forceinline__ uint32_t fused_cb(_Float16* __restrict__ hb, fused_ld
     dj.alpha_f32 = job.norm.f32;
     dj.alpha_f16 = (uint16_t)(job.norm.f16x2 & 0xFFFFu);
     const int iters = (bg == 1) ? run_iterations<1, _Float16, MAXR>(hb, dj) : run_iterations<2, _Float16, MAXR>(hb, dj);
-    store_hard<_Float16>(hb, Kb * Zc, dj.out, sh.words);
+    store_hard_words<_Float16>(hb, Kb * Zc, dj.out, sh.words);
     if (job.cb_iterations && tid == 0) job.cb_iterations[r] = (uint8_t)iters;
 
-    // ---- 5. CB CRC over ceil(K'/32) words, TB bytes (MSB-first) ---------
+    // ---- 5. CB CRC over ceil(K'/32) words and the TB-CRC share ------------
     const int Kp = (int)(p.K - p.F);
     const uint32_t* words = sh.words;
     auto byte_of = [&](int b) { return (rbit8x4(words[b >> 2]) >> (8 * (b & 3))) & 0xFFu; };
     const int kind = cb_crc_kind(p);
-    const uint32_t crc = block_crc_bytes(4 * ((Kp + 31) / 32), kind, byte_of, sh.crc_t[kind], sh.red);
     const int data_bytes = cb_data_bytes(p);
+    uint32_t crc, part;
+    if (kind == K_CRC16)
+        cb_and_share_crc<16, CRC16_P.g>(sh, sh.crc_t[kind], sh.shift[kind], 4 * ((Kp + 31) / 32), data_bytes,
+                                        sh.red[0], sh.red[1], crc, part);
+    else if (kind == K_CRC24B)
+        cb_and_share_crc<24, CRC24B_P.g>(sh, sh.crc_t[kind], sh.shift[kind], 4 * ((Kp + 31) / 32), data_bytes,
+                                         sh.red[0], sh.red[1], crc, part);
+    else
+        cb_and_share_crc<24, CRC24A_P.g>(sh, sh.crc_t[kind], sh.shift[kind], 4 * ((Kp + 31) / 32), data_bytes,
+                                         sh.red[0], sh.red[1], crc, part);
     for (int b = tid; b < data_bytes; b += nthr) job.tb_bytes[(size_t)r * data_bytes + b] = (uint8_t)byte_of(b);
     if (tid == 0) {
         job.cb_crc[r] = crc;
@@ -216,24 +340,47 @@ __device__ __forceinline__ uint32_t fused_cb(_Float16* __restrict__ hb, fused_ld
         }
     }
     if (C == 1) return 0;
-    // ---- 6. this CB's share of the TB CRC24A ------------------------------
+    // ---- 6. this CB's share of the TB CRC24A --------------------------------
     // TB CRC = XOR over CBs of crc(payload bytes of CB c) * x^(8 * TB bytes
     // after them), the TB zero-padded to whole words (CRC linearity)
-    uint32_t part = block_crc_bytes(data_bytes, K_CRC24A, byte_of, sh.crc_t[K_CRC24A], sh.red);
     const uint32_t total = (uint32_t)((C * data_bytes + 3) / 4 * 4);
     const uint32_t after = total - (uint32_t)(cb_in_tb + 1) * (uint32_t)data_bytes;
-    if (part && after) part = dmulmod(part, crc24a_shift_bytes(after), 24, CRC24A_P.g);
+    if (part && after)
+        part = cmulmod<24, CRC24A_P.g>(part, cmulmod<24, CRC24A_P.g>(sh.shift[K_CRC24A][after & 1023u],
+                                                                     g_crc_shift.tb_hi[(after >> 10) & 255u]));
     return part;
 }
 
+// TB CRCs of a launch's multi-CB TBs from the per-CB shares in the
+// workspace (one thread per TB, found as its item with CB index 0): XOR of
+// the shares, or the rejection marker if a CB was rejected.
+__global__ __launch_bounds__(256) void pusch_tb_crc_finish(const cuphyHipPuschSlotTb_t* __restrict__ tbs, int n_tbs,
+                                                           const uint32_t* __restrict__ items, int n_items,
+                                                           const uint32_t* __restrict__ shares)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n_items) return;
+    const uint32_t it = items[i];
+    const int t = (int)(it >> 16);
+    if ((it & 0xFFFFu) != 0 || t >= n_tbs) return;
+    const cuphyHipPuschSlotTb_t& job = tbs[t];
+    const uint32_t C = job.prm->num_CBs;
+    if (C <= 1) return;
+    uint32_t acc = 0;
+    bool rejected = false;
+    for (uint32_t c = 0; c < C; ++c) {
+        const uint32_t v = shares[job.share_offset + c];
+        rejected |= v == TB_ACC_ERROR;
+        acc ^= v;
+    }
+    *job.tb_crc = rejected ? FUSED_CB_ERROR : acc;
+}
+
 // Persistent fused slot kernel.  Workspace: queue state (2 words used of
-// FUSED_WS_QUEUE_WORDS), then per TB a completion counter and a CRC
-// accumulator.  A CB's share of its TB CRC is XORed into the accumulator by
-// a device-scope atomic that has completed (its old value returned) before
-// the same thread increments the TB's counter; the thread that brings the
-// counter to C reads and clears the accumulator with another atomic.  All
-// cross-workgroup data thus moves through atomics at the device coherence
-// point: no cache write-back fences per code block.
+// FUSED_WS_QUEUE_WORDS), then one TB-CRC share per CB at the TB's
+// share_offset + CB index, combined by pusch_tb_crc_finish after the
+// launch.  The next item index is fetched while the current CB is worked
+// on.  The queue is returned to 0 by the last workgroup to leave.
 template <typename TI, int MAXR>
 __global__ __launch_bounds__(384) void pusch_slot_fused(const cuphyHipPuschSlotTb_t* __restrict__ tbs, int n_tbs,
                                                         const uint32_t* __restrict__ items, int n_items,
@@ -243,34 +390,28 @@ __global__ __launch_bounds__(384) void pusch_slot_fused(const cuphyHipPuschSlotT
     __shared__ fused_lds sh;
     _Float16* hb = reinterpret_cast<_Float16*>(lds_raw);
     uint32_t* queue = ws;
-    uint32_t* tb_count = ws + FUSED_WS_QUEUE_WORDS;
-    uint32_t* tb_acc = tb_count + n_tbs;
+    uint32_t* shares = ws + FUSED_WS_QUEUE_WORDS;
     for (int i = threadIdx.x; i < 3 * 256; i += blockDim.x) (&sh.crc_t[0][0])[i] = (&g_crc_tables.byte[0][0])[i];
+    for (int i = threadIdx.x; i < 3 * CRC_SHIFT_LO; i += blockDim.x) (&sh.shift[0][0])[i] = (&g_crc_shift.lo[0][0])[i];
+    if (threadIdx.x == 0) sh.item = (int)atomicAdd(&queue[0], 1u);
+    __syncthreads();
     for (;;) {
-        if (threadIdx.x == 0) sh.item = (int)atomicAdd(&queue[0], 1u);
-        __syncthreads();
         const int i = sh.item;
-        __syncthreads();
         if (i >= n_items) break;
+        if (threadIdx.x == 0) sh.next = (int)atomicAdd(&queue[0], 1u);  // consumed after this CB
         const uint32_t it = items[i];
         const int t = (int)(it >> 16), cb = (int)(it & 0xFFFFu);
-        if (t >= n_tbs) continue;  // malformed item: skipped
-        const cuphyHipPuschSlotTb_t& job = tbs[t];
-        const PerTbParams& p = *job.prm;
-        const uint32_t C = (uint32_t)__builtin_amdgcn_readfirstlane((int)p.num_CBs);
-        if ((uint32_t)cb >= C) continue;
-        const uint32_t part = fused_cb<TI, MAXR>(hb, sh, job, p, cb, lds_values);
-        if (C > 1 && threadIdx.x == 0) {
-            // shares XOR; the rejection flag (bit 31, above every CRC value) ORs
-            const uint32_t prev = (part == TB_ACC_ERROR) ? atomicOr(&tb_acc[t], part) : atomicXor(&tb_acc[t], part);
-            // the XOR has been performed (its value returned) before the count
-            asm volatile("s_waitcnt vmcnt(0)" ::"v"(prev) : "memory");
-            if (atomicAdd(&tb_count[t], 1u) == C - 1) {
-                const uint32_t acc = atomicExch(&tb_acc[t], 0u);
-                atomicExch(&tb_count[t], 0u);
-                *job.tb_crc = (acc & TB_ACC_ERROR) ? FUSED_CB_ERROR : acc;
+        if (t < n_tbs) {
+            const cuphyHipPuschSlotTb_t& job = tbs[t];
+            const PerTbParams& p = *job.prm;
+            const uint32_t C = (uint32_t)__builtin_amdgcn_readfirstlane((int)p.num_CBs);
+            if ((uint32_t)cb < C) {
+                const uint32_t part = fused_cb<TI, MAXR>(hb, sh, job, p, cb, lds_values);
+                if (C > 1 && threadIdx.x == 0) shares[job.share_offset + (uint32_t)cb] = part;
             }
         }
+        __syncthreads();
+        if (threadIdx.x == 0) sh.item = sh.next;
         __syncthreads();
     }
     if (threadIdx.x == 0) {
@@ -313,7 +454,7 @@ cuphyStatus_t slot_decode(const cuphyHipPuschSlotTb_t* d_tbs, int n_tbs, const u
     if (max_parity_nodes < 4 || max_parity_nodes > 46) return CUPHY_STATUS_INVALID_ARGUMENT;
     void* func = fused_kernel(llr_type, max_parity_nodes);
     if (!func) return CUPHY_STATUS_UNSUPPORTED_TYPE;
-    const size_t lds = ((size_t)max_lds_values * 2 + 15) / 16 * 16;
+    const size_t lds = ((size_t)max_lds_values * 2 + 15) / 16 * 16 + 64u;  // + whole 32-value groups of the hard decisions
     cuphyStatus_t s = ensure_lds_attr(func, lds);  // also for an empty launch: prepares graph capture
     if (s != CUPHY_STATUS_SUCCESS) return s;
     if (n_items == 0) return CUPHY_STATUS_SUCCESS;
@@ -324,6 +465,12 @@ cuphyStatus_t slot_decode(const cuphyHipPuschSlotTb_t* d_tbs, int n_tbs, const u
     void* args[6] = {const_cast<cuphyHipPuschSlotTb_t**>(&d_tbs), &n_tbs, const_cast<uint32_t**>(&d_items),
                      &n_items, &max_lds_values, &ws};
     if (hipLaunchKernel(func, dim3((unsigned)grid), dim3(384), args, lds, strm) != hipSuccess)
+        return CUPHY_STATUS_INTERNAL_ERROR;
+    const uint32_t* shares = ws + FUSED_WS_QUEUE_WORDS;
+    void* fargs[5] = {const_cast<cuphyHipPuschSlotTb_t**>(&d_tbs), &n_tbs, const_cast<uint32_t**>(&d_items), &n_items,
+                      const_cast<uint32_t**>(&shares)};
+    if (hipLaunchKernel((void*)&pusch_tb_crc_finish, dim3((unsigned)((n_items + 255) / 256)), dim3(256), fargs, 0,
+                        strm) != hipSuccess)
         return CUPHY_STATUS_INTERNAL_ERROR;
     return CUPHY_STATUS_SUCCESS;
 }

@@ -8,8 +8,7 @@
 namespace nrldpc {
 
 // Workspace: the queue state (first FUSED_WS_QUEUE_WORDS words, of which 2
-// are used) followed by a completion counter per TB of the launch and a TB
-// CRC accumulator per TB.
+// are used) followed by the TB-CRC share slots (one per CB).
 constexpr int FUSED_WS_QUEUE_WORDS = 64;
 
 // Enqueue one persistent fused launch (de-rate-match + HARQ combine +

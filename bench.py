@@ -44,6 +44,8 @@ def parse():
                     help="comma-separated subset of: main, config1, config3, fused, serving; or 'copyprobe' "
                          "alone: three 1 GiB device copies (calibrates HBM byte counters)")
     ap.add_argument("--serve-ms", type=float, default=40.0, help="arrival window of each serving load point")
+    ap.add_argument("--serve-inflight", type=int, default=3, help="fused launches in flight in the serving engine")
+    ap.add_argument("--serve-loads", default="0.6,0.9,1.2", help="offered loads as fractions of capacity")
     return ap.parse_args()
 
 
@@ -67,23 +69,29 @@ def fused_slot_section(dec, steps):
         p, mb = tb.params(0, 1)
         harq = torch.zeros(tb.C * p.Ncb_padded, dtype=torch.float16, device="cuda")
         slot.append(SlotTb(p, llr, harq, mb, max_iterations=8, early_term=True, llr_scale=0.25))
-    sd = SlotDecoder(dec).build(slot)
-    sd.launch()
-    torch.cuda.synchronize()
-    sd.capture()
     stream = torch.cuda.current_stream()
-    for _ in range(2):
-        sd.replay()
-    torch.cuda.synchronize()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    n = max(3, steps // 2)
-    e0.record(stream)
-    for _ in range(n):
-        sd.replay()
-    e1.record(stream)
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / n
+
+    def time_slot(concurrent):
+        sd = SlotDecoder(dec, concurrent=concurrent).build(slot)
+        sd.launch()
+        torch.cuda.synchronize()
+        sd.capture()
+        for _ in range(2):
+            sd.replay()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        n = max(3, steps // 2)
+        e0.record(stream)
+        for _ in range(n):
+            sd.replay()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / n
+
+    ms_conc = time_slot(True)
+    ms_seq = time_slot(False)
+    ms = min(ms_conc, ms_seq)
     info_bits = sum(tb.info_bits for tb in tbs)
     ncb = sum(tb.C for tb in tbs)
     cb_fail = sum(int((st.cb_crc != 0).sum().item()) for st in slot)
@@ -104,6 +112,7 @@ def fused_slot_section(dec, steps):
                         "(4 steps/unit, +-127), consistent-Gaussian mu = SNR/5 at SNR U[18,24] dB, 8 iterations "
                         "with early termination, first transmission",
             "code_blocks": ncb, "tbs": len(tbs), "info_bits": info_bits, "ms_per_slot": round(ms, 4),
+            "ms_variants_concurrent_sequential": [round(ms_conc, 4), round(ms_seq, 4)],
             "gbps": round(info_bits / (ms * 1e-3) / 1e9, 3), "slot_budget_ms": 0.5,
             "mean_iterations": round(float(iters.float().mean().item()), 3),
             "cb_crc_failures": cb_fail, "tb_crc_failures": tb_fail, "tb_payload_mismatches": payload_err,
@@ -111,7 +120,7 @@ def fused_slot_section(dec, steps):
             "hbm_gbs": round(hbm / (ms * 1e-3) / 1e9, 1), "hbm_frac": round(hbm / (ms * 1e-3) / 8e12, 4)}
 
 
-def serving_section(dec, serve_ms, config3_gbps=None):
+def serving_section(dec, serve_ms, config3_gbps=None, inflight=3, loads=(0.6, 0.9, 1.2)):
     """Serving loop: Poisson TB arrivals from 32 cells (TB shapes as in the
     heterogeneous slot, binary16 BPSK-AWGN LLRs), HARQ with rv 0, 2, 3, 1
     and up to 4 transmissions, retransmissions released one HARQ round trip
@@ -124,7 +133,7 @@ def serving_section(dec, serve_ms, config3_gbps=None):
     from aerial_amd.server import HarqServer, TraceTemplates, build_trace, poisson_arrivals, serving_specs, trace_report
     n_templates = 256
     templates = TraceTemplates(serving_specs(n_templates, seed=41), seed=41)
-    srv = HarqServer(dec, pool_bytes=8 << 30, max_inflight=3, max_tbs=2048, max_cbs=16384)
+    srv = HarqServer(dec, pool_bytes=8 << 30, max_inflight=inflight, max_tbs=2048, max_cbs=16384)
     rtt = 2000.0
     mean_bits = float(np.mean([tb.info_bits for tb in templates.tbs]))
     n_sat = 1500
@@ -136,7 +145,7 @@ def serving_section(dec, serve_ms, config3_gbps=None):
     cap_tbs = n_sat / (first_tx_span_us * 1e-6)
     out = {"saturation": rep, "capacity_first_tx_tbs_per_s": round(cap_tbs, 1),
            "capacity_first_tx_gbps": round(cap_tbs * mean_bits / 1e9, 3), "loads": []}
-    for x in (0.6, 0.9, 1.2):
+    for x in loads:
         arr = poisson_arrivals(32, x * cap_tbs, serve_ms * 1e3, n_templates, seed=int(100 * x))
         entries, outs, meta = build_trace(srv, templates, arr)
         done = srv.serve_trace(entries, rtt)
@@ -151,7 +160,8 @@ def serving_section(dec, serve_ms, config3_gbps=None):
     srv.close()
     out["workload"] = ("32 cells, Poisson TB arrivals, TB shapes as the heterogeneous slot (256 templates), "
                        "rv 0,2,3,1 up to 4 transmissions, HARQ RTT 2 ms, fp16 HARQ buffers in an 8 GiB pool, "
-                       "early termination <= 10 iterations, 3 in-flight fused launches; %.0f ms per load" % serve_ms)
+                       "early termination <= 10 iterations, %d in-flight fused launches; %.0f ms per load"
+                       % (inflight, serve_ms))
     return out
 
 
@@ -340,7 +350,8 @@ def main():
         fused = fused_slot_section(dec, a.steps)
     serving = None
     if "serving" in sections:
-        serving = serving_section(dec, a.serve_ms, config3["batched_gbps"] if config3 else None)
+        serving = serving_section(dec, a.serve_ms, config3["batched_gbps"] if config3 else None,
+                                  inflight=a.serve_inflight, loads=[float(x) for x in a.serve_loads.split(",")])
 
     cpu = None if a.no_cpu else cpu_baseline(tbs, a.cpu_seconds)
     return line_of(a, locals())

@@ -405,16 +405,20 @@ typedef struct {
                                         (int8 LLRs with 4 steps per unit LLR: 0.25) */
     cuphyLDPCNormalization_t norm;   /* f16x2: binary16 normalisation */
     float clamp_value;               /* decoder input clamp */
+    uint32_t share_offset;           /* first of the TB's num_CBs TB-CRC share slots in the workspace */
 } cuphyHipPuschSlotTb_t;
 
-/* Workspace of cuphyHipPuschSlotDecode for up to num_tbs TBs: queue state
- * and per-TB completion counters.  Zero-fill once; every completed launch
- * leaves it zeroed (graph-replayable).  One workspace per running launch. */
-cuphyStatus_t CUPHYWINAPI cuphyHipPuschSlotDecodeGetWorkspaceSize(int num_tbs, size_t* sizeInBytes);
+/* Workspace of cuphyHipPuschSlotDecode: queue state and num_cbs TB-CRC
+ * share slots (TB t uses slots share_offset .. share_offset + num_CBs - 1;
+ * the TBs of one launch must not overlap).  Zero-fill once; the queue state
+ * returns to zero after every launch (graph-replayable).  One workspace per
+ * running launch. */
+cuphyStatus_t CUPHYWINAPI cuphyHipPuschSlotDecodeGetWorkspaceSize(int num_cbs, size_t* sizeInBytes);
 
 /* Fused de-rate-match + HARQ combine + layered NMS decode (binary16) + CB
  * CRC + TB assembly + TB CRC for num_cb_items code blocks in one persistent
- * launch.  d_cb_items[i] = (TB index << 16) | CB index within the TB;
+ * launch (followed by a small kernel that combines each TB's CRC shares).
+ * d_cb_items[i] = (TB index << 16) | CB index within the TB;
  * every CB of a TB must be in the same launch.  llr_type: CUPHY_R_8I,
  * CUPHY_R_16F or CUPHY_R_32F.  max_lds_values: max over TBs of
  * max(2Zc + Ncb, (nsys + mb) * Zc) (sizes LDS); max_parity_nodes: max mb

@@ -111,7 +111,7 @@ def run_slot(dec, dtype, early_term, specs=SLOT):
             tbs.append(SlotTb(p, dev, c.harq, c.mb, max_iterations=ITERS, early_term=early_term, llr_scale=scale))
         sd.build(tbs).launch()
         torch.cuda.synchronize()
-        assert all(int(w.abs().sum().item()) == 0 for w in sd.workspaces), "workspace not returned to zero"
+        assert all(int(w[:2].abs().sum().item()) == 0 for w in sd.workspaces), "queue not returned to zero"
         for c, tb in zip(cases, tbs):
             bg, Zc, C, Kp = c.spec[:4]
             alpha = half_to_float(sd._alpha_bits(tb))
@@ -242,7 +242,7 @@ def test_slot_graph_replay(dec):
         for (b, cc, tc, by, h), tb, c in zip(ref, tbs, cases):
             assert torch.equal(bits_region(c, tb), b) and torch.equal(tb.cb_crc, cc) and torch.equal(tb.tb_crc, tc)
             assert torch.equal(tb.tb_bytes, by) and torch.equal(harq_region(c), h)
-        assert all(int(w.abs().sum().item()) == 0 for w in sd.workspaces)
+        assert all(int(w[:2].abs().sum().item()) == 0 for w in sd.workspaces)
         sd.launch()
         torch.cuda.synchronize()
 
@@ -267,4 +267,4 @@ def test_slot_rejects_cb_beyond_variant(dec):
     torch.cuda.synchronize()
     assert (tb.cb_crc.cpu().numpy().view(np.uint32) == L.SLOT_CB_ERROR).all()
     assert int(tb.tb_crc.cpu().numpy().view(np.uint32)[0]) == L.SLOT_CB_ERROR
-    assert int(ln.workspace.abs().sum().item()) == 0
+    assert int(ln.workspace[:2].abs().sum().item()) == 0
