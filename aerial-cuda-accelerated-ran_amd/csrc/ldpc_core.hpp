@@ -285,7 +285,11 @@ __device__ __forceinline__ int run_iterations(T* __restrict__ app, const cb_job&
         asm volatile("" : "+v"(zl));
         static_for<NR>([&](auto Rc) {
             constexpr int R = decltype(Rc)::value;
-            if (R < mb) {
+            // opaque row count: the R < mb tests stay scalar compares at
+            // each row instead of 46 hoisted (and spilled) lane masks
+            int mbr = mb;
+            asm volatile("" : "+s"(mbr));
+            if (R < mbr) {
                 shift_ptr shr = sh;
                 uint32_t Zr = Z;
                 asm volatile("" : "+s"(shr), "+s"(Zr));
@@ -298,14 +302,16 @@ __device__ __forceinline__ int run_iterations(T* __restrict__ app, const cb_job&
                 // barrier before the next row only where it shares a column
                 // with the rows since the last barrier, and at the iteration end
                 constexpr bool next_needs = (R + 1 >= NR) || ((kBarrierMask<BG> >> (R + 1)) & 1ull);
-                if (next_needs || R + 1 == mb) __syncthreads();
+                if (next_needs || R + 1 == mbr) __syncthreads();
             }
         });
         if (job.early_term) {
             uint32_t fail = 0;
             static_for<NR>([&](auto Rc) {
                 constexpr int R = decltype(Rc)::value;
-                if (R < mb) {
+                int mbr = mb;
+                asm volatile("" : "+s"(mbr));
+                if (R < mbr) {
                     shift_ptr shr = sh;
                     uint32_t Zr = Z;
                     asm volatile("" : "+s"(shr), "+s"(Zr));

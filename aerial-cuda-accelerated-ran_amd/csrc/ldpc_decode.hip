@@ -65,7 +65,9 @@ __global__ __launch_bounds__(384) void ldpc_layered_persistent(const cuphyHipLDP
     for (;;) {
         if (threadIdx.x == 0) s_item = (int)atomicAdd(&queue[0], 1u);
         __syncthreads();
-        const int i = s_item;
+        // uniform: the item's fields then come in through scalar loads and
+        // the per-row bounds (R < mb) are scalar branches
+        const int i = __builtin_amdgcn_readfirstlane(s_item);
         __syncthreads();
         if (i >= n_items) break;
         const cuphyHipLDPCWorkItem_t& w = items[i];
