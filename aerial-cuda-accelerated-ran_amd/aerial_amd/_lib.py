@@ -200,6 +200,7 @@ SIGNATURES = {
     "cuphyHipLDPCEncodeCodeBlocks": ([_I, _I, _VP, _I, _I, _VP, _I, _VP], _I),
     "cuphyHipLDPCBatchDecodeGetWorkspaceSize": ([_P(C.c_size_t)], _I),
     "cuphyHipLDPCBatchDecode": ([_VP, _VP, _I, _I, _I, _I, _I, _VP, _VP], _I),
+    "cuphyHipLDPCBatchDecodeGroups": ([_VP, _VP, _VP, _I, _I, _I, _I, _I, _I, _VP, _VP], _I),
     "cuphyHipPuschSlotDecodeGetWorkspaceSize": ([_I, _P(C.c_size_t)], _I),
     "cuphyHipPuschSlotDecode": ([_VP, _VP, _I, _VP, _I, _I, _I, _I, _VP, _VP], _I),
     "cuphyHipHarqServerCreate": ([_VP, _P(HarqServerConfig), _P(_VP)], _I),

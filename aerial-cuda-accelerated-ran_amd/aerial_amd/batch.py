@@ -1,9 +1,19 @@
-"""Heterogeneous batched decode: one persistent launch per slot.
+"""Heterogeneous batched decode: persistent launches per slot.
 
-Builds the device work-item array for cuphyHipLDPCBatchDecode from a list of
-transport blocks (any BG, Z, mb per TB), orders items longest-first (a
-longest-processing-time list schedule over the device work queue), and can
-capture the launch in a HIP graph (torch.cuda.CUDAGraph) for replay per slot.
+Builds the device work-item array for cuphyHipLDPCBatchDecode /
+cuphyHipLDPCBatchDecodeGroups from a list of transport blocks (any BG, Z, mb
+per TB), orders work longest-first (a longest-processing-time list schedule
+over the device work queue), and can capture the launches in a HIP graph
+(torch.cuda.CUDAGraph) for replay per slot.
+
+Two regimes:
+  * latency (fewer code blocks than `split_min`): every code block gets a
+    384-thread workgroup of its own, so each one runs at its own speed;
+  * throughput (at least `split_min` code blocks): code blocks of one
+    (BG, Z, mb) are packed several to a workgroup and workgroup sizes follow
+    Z, as the tuning table chooses per (BG, Z, mb class)
+    (aerial_amd.tuning); one persistent launch per (mb class, workgroup
+    size), run concurrently on separate streams.
 """
 import ctypes as C
 
@@ -11,50 +21,69 @@ import numpy as np
 import torch
 
 from . import _lib as L
+from . import tuning
 from .decoder import _stream_handle
 from .workloads import num_edges, nsys
 
-
-def _half_bits(x):
-    return int(torch.tensor([x], dtype=torch.float16).view(torch.int16).item()) & 0xFFFF
-
-
 SMALL_MAXR = 8  # mb class boundary of the register-lean kernel variant
 
+# numpy view of cuphyHipLDPCWorkItem_t (48 bytes)
+ITEM_DTYPE = np.dtype([("llr", "<u8"), ("out", "<u8"), ("soft", "<u8"), ("BG", "<i2"), ("Z", "<i2"),
+                       ("num_parity_nodes", "<i2"), ("Kb", "<i2"), ("max_iterations", "<i2"), ("flags", "<u2"),
+                       ("norm", "<u4"), ("clamp_value", "<f4"), ("reserved", "<u4")])
+assert ITEM_DTYPE.itemsize == C.sizeof(L.LDPCWorkItem)
 
-class _SubBatch:
-    """Work items of one kernel variant (mb class) with their own queue."""
 
-    def __init__(self, lib, ws_words):
+def _half_bits(x):
+    return int(np.array([x], dtype=np.float16).view(np.uint16)[0])
+
+
+def group_values(bg, Z, mb, count):
+    """LDS values of a group of `count` code blocks (regions of whole 16 B)."""
+    nval = (nsys(bg) + mb) * Z
+    return count * ((nval + 7) & ~7)
+
+
+class _Launch:
+    """Work of one persistent launch (one kernel variant and workgroup size)
+    with its own device queue."""
+
+    def __init__(self, ws_words, block, grouped):
         self.workspace = torch.zeros(ws_words, dtype=torch.int32, device="cuda")
+        self.block = block
+        self.grouped = grouped
         self.items = None
+        self.group_first = None
         self.n_items = 0
+        self.n_groups = 0
         self.max_values = 0
         self.max_mb = 4
 
 
 class BatchDecoder:
-    """Heterogeneous batch over one or two persistent launches.
+    """Heterogeneous batch over one or more persistent launches.
 
-    Code blocks with mb <= 8 (high code rates) go to the register-lean
-    kernel variant when there are at least `split_min` of them (enough to
-    fill the GPU on their own), the rest to the full one; the launches run
-    on separate streams (fork/join) and share the GPU concurrently.  A
-    smaller high-rate class joins the full variant's launch: measured on the
-    16-UE heterogeneous slot, one launch is faster than two there."""
+    Latency regime (< split_min code blocks): code blocks with mb <= 8 go to
+    the register-lean kernel variant when all of them are high-rate, else
+    everything shares the full variant's launch (measured on the 16-UE
+    heterogeneous slot: one launch is faster than two there).  Throughput
+    regime: groups and workgroup sizes from the tuning table, one launch per
+    (mb class, workgroup size) on separate streams (fork/join)."""
 
-    def __init__(self, decoder, split_by_mb=True, split_min=256):
+    def __init__(self, decoder, split_by_mb=True, split_min=256, pack=True, table=None):
         self.dec = decoder
         self._lib = L.lib()
         self.split_by_mb = split_by_mb
         self.split_min = split_min
+        self.pack = pack
+        self.table = table if table is not None else tuning.load_table()
         sz = C.c_size_t()
         L.check("cuphyHipLDPCBatchDecodeGetWorkspaceSize",
                 self._lib.cuphyHipLDPCBatchDecodeGetWorkspaceSize(C.byref(sz)))
         self._ws_words = sz.value // 4
         self.subs = []
         self.graph = None
-        self.aux_streams = [torch.cuda.Stream()]
+        self.aux_streams = []
 
     @property
     def n_items(self):
@@ -64,6 +93,12 @@ class BatchDecoder:
     def workspaces(self):
         return [b.workspace for b in self.subs]
 
+    def _choice(self, bg, Z, mb, regime_throughput):
+        """(code blocks per workgroup, workgroup size) of a configuration."""
+        if not (self.pack and regime_throughput):
+            return 1, 384
+        return tuning.choose(self.table, bg, Z, mb)
+
     def build(self, tbs, llr_type=L.CUPHY_R_16F, fp16_math=True, max_iterations=10, alpha=0.8125,
               clamp_value=32.0, flags=0, soft=None, order="lpt"):
         """tbs: objects with bg, Z, mb, ncb, llr (ncb, stride) and out (ncb, words)
@@ -71,54 +106,74 @@ class BatchDecoder:
         list of (ncb, >= Kb*Z) float16 tensors per TB."""
         n_small = sum(tb.ncb for tb in tbs if tb.mb <= SMALL_MAXR)
         n_all = sum(tb.ncb for tb in tbs)
+        throughput = n_all >= self.split_min
         split = self.split_by_mb and (n_small >= self.split_min or n_small == n_all)
-        classes = {}
+        f16 = fp16_math or llr_type == L.CUPHY_R_16F
+        h = _half_bits(alpha)
+        norm = (h | (h << 16)) if f16 else int(np.array([alpha], dtype=np.float32).view(np.uint32)[0])
+
+        # code blocks -> groups of one configuration -> launches
+        launches = {}
         for t, tb in enumerate(tbs):
             Kb = getattr(tb, "Kb", None) or nsys(tb.bg)
-            cost = tb.Z * num_edges(tb.bg, tb.mb)
-            key = (tb.mb <= SMALL_MAXR) if split else False
-            for c in range(tb.ncb):
-                classes.setdefault(key, []).append((cost, t, c, Kb))
-        h = _half_bits(alpha)
+            P, block = self._choice(tb.bg, tb.Z, tb.mb, throughput)
+            small = (tb.mb <= SMALL_MAXR) if split else False
+            key = (small, block)
+            # estimated group time: waves per SIMD x edges per row sweep
+            waves = -(-P * tb.Z // 64)
+            cost = -(-waves // 4) * num_edges(tb.bg, tb.mb)
+            for c0 in range(0, tb.ncb, P):
+                launches.setdefault(key, []).append((cost, t, c0, min(P, tb.ncb - c0), Kb))
         self.subs = []
         self.order = []
-        for key in sorted(classes, key=lambda k: not k):
-            rows = classes[key]
+        for key in sorted(launches, key=lambda k: (not k[0], -k[1])):
+            groups = launches[key]
             if order == "lpt":
-                rows.sort(key=lambda r: -r[0])
-            sb = _SubBatch(self._lib, self._ws_words)
-            arr = (L.LDPCWorkItem * len(rows))()
-            for i, (cost, t, c, Kb) in enumerate(rows):
+                groups.sort(key=lambda g: -g[0])
+            n_items = sum(g[3] for g in groups)
+            arr = np.zeros(n_items, dtype=ITEM_DTYPE)
+            first = np.zeros(len(groups) + 1, dtype=np.uint32)
+            sb = _Launch(self._ws_words, key[1], True)
+            i = 0
+            for gi, (cost, t, c0, cnt, Kb) in enumerate(groups):
                 tb = tbs[t]
-                w = arr[i]
-                w.llr = tb.llr.data_ptr() + c * tb.llr.stride(0) * tb.llr.element_size()
-                w.out = tb.out.data_ptr() + c * tb.out.stride(0) * 4
+                first[gi] = i
+                cs = np.arange(c0, c0 + cnt, dtype=np.uint64)
+                sl = slice(i, i + cnt)
+                arr["llr"][sl] = tb.llr.data_ptr() + cs * (tb.llr.stride(0) * tb.llr.element_size())
+                arr["out"][sl] = tb.out.data_ptr() + cs * (tb.out.stride(0) * 4)
                 if soft is not None and soft[t] is not None:
-                    w.soft = soft[t].data_ptr() + c * soft[t].stride(0) * 2
-                w.BG, w.Z, w.num_parity_nodes, w.Kb = tb.bg, tb.Z, tb.mb, Kb
-                w.max_iterations = max_iterations
-                w.flags = flags
-                if fp16_math or llr_type == L.CUPHY_R_16F:
-                    w.norm.f16x2 = h | (h << 16)
-                else:
-                    w.norm.f32 = alpha
-                w.clamp_value = clamp_value
-                sb.max_values = max(sb.max_values, (nsys(tb.bg) + tb.mb) * tb.Z)
+                    arr["soft"][sl] = soft[t].data_ptr() + cs * (soft[t].stride(0) * 2)
+                arr["BG"][sl] = tb.bg
+                arr["Z"][sl] = tb.Z
+                arr["num_parity_nodes"][sl] = tb.mb
+                arr["Kb"][sl] = Kb
+                arr["max_iterations"][sl] = max_iterations
+                arr["flags"][sl] = flags
+                arr["norm"][sl] = norm
+                arr["clamp_value"][sl] = clamp_value
+                sb.max_values = max(sb.max_values, group_values(tb.bg, tb.Z, tb.mb, cnt))
                 sb.max_mb = max(sb.max_mb, tb.mb)
-            raw = np.frombuffer(bytes(arr), dtype=np.uint8)
-            sb.items = torch.from_numpy(raw.copy()).cuda()
-            sb.n_items = len(rows)
+                self.order += [(t, c) for c in range(c0, c0 + cnt)]
+                i += cnt
+            first[len(groups)] = i
+            sb.items = torch.from_numpy(arr.view(np.uint8).copy()).cuda()
+            sb.group_first = torch.from_numpy(first.view(np.int32).copy()).cuda()
+            sb.n_items = n_items
+            sb.n_groups = len(groups)
             self.subs.append(sb)
-            self.order += [(t, c) for _, t, c, _ in rows]
+        while len(self.aux_streams) < len(self.subs) - 1:
+            self.aux_streams.append(torch.cuda.Stream())
         self.llr_type = llr_type
         self.fp16_math = fp16_math
         return self
 
     def _launch_one(self, sb, stream):
-        st = self._lib.cuphyHipLDPCBatchDecode(self.dec.dec, C.c_void_p(sb.items.data_ptr()), sb.n_items,
-                                               self.llr_type, int(self.fp16_math), sb.max_values, sb.max_mb,
-                                               C.c_void_p(sb.workspace.data_ptr()), _stream_handle(stream))
-        return L.check("cuphyHipLDPCBatchDecode", st)
+        st = self._lib.cuphyHipLDPCBatchDecodeGroups(
+            self.dec.dec, C.c_void_p(sb.items.data_ptr()), C.c_void_p(sb.group_first.data_ptr()), sb.n_groups,
+            self.llr_type, int(self.fp16_math), sb.max_values, sb.max_mb, sb.block,
+            C.c_void_p(sb.workspace.data_ptr()), _stream_handle(stream))
+        return L.check("cuphyHipLDPCBatchDecodeGroups", st)
 
     def launch(self, stream=None):
         main = stream if stream is not None else torch.cuda.current_stream()
@@ -126,7 +181,7 @@ class BatchDecoder:
             return
         if len(self.subs) == 1:
             return self._launch_one(self.subs[0], main)
-        # fork: the other variants run on auxiliary streams, then join
+        # fork: the other launches run on auxiliary streams, then join
         aux = self.aux_streams[:len(self.subs) - 1]
         for a in aux:
             a.wait_stream(main)

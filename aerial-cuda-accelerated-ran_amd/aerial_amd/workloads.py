@@ -103,11 +103,12 @@ def fill_tb(tb, gen, llr_dtype=torch.float16, sigma2=None):
     return tb
 
 
-def heterogeneous_slot(seed=2, n_ue=16, llr_dtype=torch.float16):
-    """16 UEs: BG in {1,2}, Zc uniform over lifting sizes in [64, 384], code
-    rate U[0.2, 0.93], 1-24 CBs per TB, Es/N0 U[-1, 8] dB, BPSK-AWGN LLRs."""
+def heterogeneous_slot(seed=2, n_ue=16, llr_dtype=torch.float16, z_min=64):
+    """16 UEs: BG in {1,2}, Zc uniform over lifting sizes in [z_min, 384],
+    code rate U[0.2, 0.93], 1-24 CBs per TB, Es/N0 U[-1, 8] dB, BPSK-AWGN
+    LLRs."""
     rng = np.random.RandomState(seed)
-    zs = [z for z in LIFTING_SIZES if 64 <= z <= 384]
+    zs = [z for z in LIFTING_SIZES if z_min <= z <= 384]
     gen = torch.Generator(device="cuda")
     gen.manual_seed(seed)
     tbs = []

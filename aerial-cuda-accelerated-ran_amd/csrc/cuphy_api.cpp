@@ -575,8 +575,23 @@ cuphyStatus_t CUPHYWINAPI cuphyHipLDPCBatchDecode(cuphyLDPCDecoder_t decoder, co
                                                   hipStream_t strm)
 {
     if (!decoder || !d_workspace || max_parity_nodes < 4 || max_parity_nodes > 46) return CUPHY_STATUS_INVALID_ARGUMENT;
-    return nrldpc::decode_batch(d_items, num_items, llr_type, fp16_math != 0, max_values_per_cb, max_parity_nodes,
-                                static_cast<uint32_t*>(d_workspace), decoder->cu_count, strm);
+    if (max_values_per_cb <= 0 || max_values_per_cb > 68 * 384) return CUPHY_STATUS_INVALID_ARGUMENT;
+    return nrldpc::decode_batch(d_items, nullptr, num_items, llr_type, fp16_math != 0, max_values_per_cb,
+                                max_parity_nodes, 384, static_cast<uint32_t*>(d_workspace), decoder->cu_count, strm);
+}
+
+cuphyStatus_t CUPHYWINAPI cuphyHipLDPCBatchDecodeGroups(cuphyLDPCDecoder_t decoder,
+                                                        const cuphyHipLDPCWorkItem_t* d_items,
+                                                        const uint32_t* d_group_first, int num_groups,
+                                                        cuphyDataType_t llr_type, int fp16_math,
+                                                        int max_values_per_group, int max_parity_nodes,
+                                                        int block_threads, void* d_workspace, hipStream_t strm)
+{
+    if (!decoder || !d_workspace || !d_group_first || max_parity_nodes < 4 || max_parity_nodes > 46)
+        return CUPHY_STATUS_INVALID_ARGUMENT;
+    return nrldpc::decode_batch(d_items, d_group_first, num_groups, llr_type, fp16_math != 0, max_values_per_group,
+                                max_parity_nodes, block_threads, static_cast<uint32_t*>(d_workspace),
+                                decoder->cu_count, strm);
 }
 
 cuphyStatus_t CUPHYWINAPI cuphyHipPuschSlotDecodeGetWorkspaceSize(int num_cbs, size_t* sizeInBytes)

@@ -93,6 +93,20 @@ __device__ __forceinline__ uint32_t rot_index(uint32_t z, uint32_t s, uint32_t Z
     return q < p ? q : p;
 }
 
+// Value at byte offset `off` of a lane's LDS region.
+template <typename T>
+__device__ __forceinline__ T& lds_at(unsigned char* base, uint32_t off)
+{
+    return *reinterpret_cast<T*>(base + off);
+}
+
+// ---------------------------------------------------------------------------
+// Row addressing: lane z of a code block whose APP values start at byte
+// `base` reads APP[col][(z + s) mod Z] at base + col*Zb + rot_index(zb, sb, Zb),
+// with zb, sb, Zb the lane, shift and lifting size in bytes (x sizeof(T)):
+// add, subtract, min and one three-operand add per edge.
+// ---------------------------------------------------------------------------
+
 // ---------------------------------------------------------------------------
 // binary32 rows: compressed C2V per (row, lane) -- normalised min1/min2 and a
 // word holding the argmin (bits 27..31) and the C2V sign of every edge
@@ -100,8 +114,9 @@ __device__ __forceinline__ uint32_t rot_index(uint32_t z, uint32_t s, uint32_t Z
 // subtract 1, min/min2/parity 3, new message + write-back 6 VALU ops.
 // ---------------------------------------------------------------------------
 template <int BG, int R, int NR>
-__device__ __forceinline__ void process_row(float* __restrict__ app, shift_ptr sh, uint32_t z, uint32_t Z, float alpha,
-                                            float (&st_n1)[NR], float (&st_n2)[NR], uint32_t (&st_meta)[NR])
+__device__ __forceinline__ void process_row(float*, unsigned char* __restrict__ base, shift_ptr sh, uint32_t zb,
+                                            uint32_t Zb, float alpha, float (&st_n1)[NR], float (&st_n2)[NR],
+                                            uint32_t (&st_meta)[NR])
 {
     using BGT = bg_traits<BG>;
     constexpr int E0 = BGT::row_start(R);
@@ -113,10 +128,10 @@ __device__ __forceinline__ void process_row(float* __restrict__ app, shift_ptr s
     const uint32_t oidx = om >> 27;
     static_for<D>([&](auto J) {
         constexpr int j = decltype(J)::value;
-        off[j] = (uint32_t)BGT::col(E0 + j) * Z + rot_index(z, sh[E0 + j], Z);
+        off[j] = (uint32_t)BGT::col(E0 + j) * Zb + rot_index(zb, sh[E0 + j] * 4u, Zb);
         const float mag = (oidx == (uint32_t)j) ? on2 : on1;
         const float old = u2f(f2u(mag) | ((om << (32 - D + j)) & 0x80000000u));
-        t[j] = app[off[j]] - old;
+        t[j] = lds_at<float>(base, off[j]) - old;
     });
     float m1 = __builtin_huge_valf(), m2 = __builtin_huge_valf();
     uint32_t par = 0;
@@ -138,7 +153,7 @@ __device__ __forceinline__ void process_row(float* __restrict__ app, shift_ptr s
         const float mag = is_min ? a2 : a1;
         idx = is_min ? (uint32_t)j : idx;
         const uint32_t cb = f2u(mag) | ((f2u(t[j]) ^ par) & 0x80000000u);
-        app[off[j]] = t[j] + u2f(cb);
+        lds_at<float>(base, off[j]) = t[j] + u2f(cb);
         fs = __builtin_amdgcn_alignbit(fs, cb, 31);  // (fs << 1) | sign(c_j)
     });
     st_n1[R] = a1;
@@ -152,8 +167,8 @@ __device__ __forceinline__ void process_row(float* __restrict__ app, shift_ptr s
 // 1, min/min2/parity 3, new message + write-back 5 VALU ops; no argmin.
 // ---------------------------------------------------------------------------
 template <int BG, int R, int NW>
-__device__ __forceinline__ void process_row(_Float16* __restrict__ app, shift_ptr sh, uint32_t z, uint32_t Z,
-                                            _Float16 alpha, half2_t (&st)[NW])
+__device__ __forceinline__ void process_row(_Float16*, unsigned char* __restrict__ base, shift_ptr sh, uint32_t zb,
+                                            uint32_t Zb, _Float16 alpha, half2_t (&st)[NW])
 {
     using BGT = bg_traits<BG>;
     constexpr int E0 = BGT::row_start(R);
@@ -163,9 +178,9 @@ __device__ __forceinline__ void process_row(_Float16* __restrict__ app, shift_pt
     static_for<D>([&](auto J) {
         constexpr int j = decltype(J)::value;
         constexpr int e = E0 + j;
-        off[j] = (uint32_t)BGT::col(e) * Z + rot_index(z, sh[e], Z);
+        off[j] = (uint32_t)BGT::col(e) * Zb + rot_index(zb, sh[e] * 2u, Zb);
         const _Float16 old = (e & 1) ? st[e >> 1].y : st[e >> 1].x;
-        t[j] = app[off[j]] - old;
+        t[j] = lds_at<_Float16>(base, off[j]) - old;
     });
     _Float16 m1 = (_Float16)__builtin_huge_valf(), m2 = (_Float16)__builtin_huge_valf();
     uint32_t par = 0;
@@ -183,7 +198,7 @@ __device__ __forceinline__ void process_row(_Float16* __restrict__ app, shift_pt
         constexpr int e = E0 + j;
         const _Float16 mag = (__builtin_fabsf16(t[j]) == m1) ? a2 : a1;
         const _Float16 c = u2h((uint16_t)(h2u(mag) | ((h2u(t[j]) ^ par) & 0x8000u)));
-        app[off[j]] = t[j] + c;
+        lds_at<_Float16>(base, off[j]) = t[j] + c;
         if (e & 1)
             st[e >> 1].y = c;
         else
@@ -193,7 +208,7 @@ __device__ __forceinline__ void process_row(_Float16* __restrict__ app, shift_pt
 
 // Parity of check row R, lane z: hard bit = APP < 0 (syndrome_check.m:38-40).
 template <int BG, int R, typename T>
-__device__ __forceinline__ uint32_t row_syndrome(const T* __restrict__ app, shift_ptr sh, uint32_t z, uint32_t Z)
+__device__ __forceinline__ uint32_t row_syndrome(unsigned char* __restrict__ base, shift_ptr sh, uint32_t zb, uint32_t Zb)
 {
     using BGT = bg_traits<BG>;
     constexpr int E0 = BGT::row_start(R);
@@ -201,7 +216,8 @@ __device__ __forceinline__ uint32_t row_syndrome(const T* __restrict__ app, shif
     uint32_t par = 0;
     static_for<D>([&](auto J) {
         constexpr int j = decltype(J)::value;
-        par ^= (uint32_t)(app[(uint32_t)BGT::col(E0 + j) * Z + rot_index(z, sh[E0 + j], Z)] < (T)0);
+        const uint32_t off = (uint32_t)BGT::col(E0 + j) * Zb + rot_index(zb, sh[E0 + j] * (uint32_t)sizeof(T), Zb);
+        par ^= (uint32_t)(lds_at<T>(base, off) < (T)0);
     });
     return par;
 }
@@ -223,7 +239,8 @@ struct cb_job {
 // (type T), converted, clamped to +-clamp (clamp > 0) and with a signed zero
 // made +0.  Every thread of the workgroup calls it; ends with a barrier.
 template <typename T, typename TIn>
-__device__ __forceinline__ void load_llrs(T* __restrict__ app, const TIn* __restrict__ src, int nval, float clamp)
+__device__ __forceinline__ void load_llrs_nosync(T* __restrict__ app, const TIn* __restrict__ src, int nval,
+                                                 float clamp)
 {
     const T clampv = (T)clamp;
     const bool do_clamp = clamp > 0.0f;
@@ -232,8 +249,17 @@ __device__ __forceinline__ void load_llrs(T* __restrict__ app, const TIn* __rest
         if (do_clamp) v = num<T>::vmax(num<T>::vmin(v, clampv), -clampv);
         app[n] = (v == (T)0) ? (T)0 : v;
     }
+}
+template <typename T, typename TIn>
+__device__ __forceinline__ void load_llrs(T* __restrict__ app, const TIn* __restrict__ src, int nval, float clamp)
+{
+    load_llrs_nosync<T, TIn>(app, src, nval, clamp);
     __syncthreads();
 }
+
+// Values between the APP regions of consecutive code blocks of a group
+// (whole 16-byte units).
+__device__ __forceinline__ int group_stride(int nval) { return (nval + 7) & ~7; }
 
 template <typename T> __device__ __forceinline__ T job_alpha(const cb_job& job);
 template <> __device__ __forceinline__ float job_alpha<float>(const cb_job& job) { return job.alpha_f32; }
@@ -244,11 +270,18 @@ template <> __device__ __forceinline__ _Float16 job_alpha<_Float16>(const cb_job
 
 // Phase 2: the layered iterations over the APP values in LDS (already
 // loaded).  Returns the number of iterations run (< max_iter only with
-// early termination).  Every thread of the workgroup calls it.
+// early termination; the largest over the group).  Every thread of the
+// workgroup calls it.
 // MAXR: rows the register state is sized for (mb <= MAXR); a small MAXR
 // (high code rates) needs far fewer VGPRs and so allows more resident waves.
+// Groups: `count` code blocks of the same (BG, Z, mb, iterations) decoded
+// together, lanes [c*Z, (c+1)*Z) on code block c, whose APP values start at
+// app + c*stride.  With early termination every code block stops on its own
+// syndrome (its lanes freeze, exactly as if decoded alone); `cb_flags` is
+// LDS scratch of `count` words for that (unused when count == 1).
 template <int BG, typename T, int MAXR>
-__device__ __forceinline__ int run_iterations(T* __restrict__ app, const cb_job& job)
+__device__ __forceinline__ int run_iterations(T* __restrict__ app, const cb_job& job, int count = 1, int stride = 0,
+                                              uint32_t* cb_flags = nullptr)
 {
     using BGT = bg_traits<BG>;
     constexpr int NR = (MAXR < BGT::NROWS) ? MAXR : BGT::NROWS;
@@ -272,17 +305,25 @@ __device__ __forceinline__ int run_iterations(T* __restrict__ app, const cb_job&
         static_for<NW>([&](auto W) { st_h[W] = half2_t{(_Float16)0, (_Float16)0}; });
     }
 
-    const bool active = tid < Z;
+    // lane -> (code block, check row)
+    const uint32_t cb = (count == 1) ? (tid < Z ? 0u : 1u) : tid / Z;
+    const uint32_t zlane = tid - (cb < (uint32_t)count ? cb : 0u) * Z;
+    const bool in_group = cb < (uint32_t)count;
+    bool active = in_group;
+    unsigned char* const lane_base =
+        reinterpret_cast<unsigned char*>(app) + (size_t)(in_group ? cb : 0u) * (size_t)stride * sizeof(T);
+    const uint32_t Zb = Z * (uint32_t)sizeof(T);
+
     shift_ptr sh = (shift_ptr)(const void*)&g_shift_tables.v[BG - 1][z_index((int)Z)][0];
     int it = 0;
     while (it < job.max_iter) {
         ++it;
-        // Opaque copies of the lane index and of the shift pointer, renewed
+        // Opaque copies of the lane offset and of the shift pointer, renewed
         // per row: the 316 (BG1) per-edge shifts and addresses are reloaded
         // and recomputed where used instead of being hoisted out of the
         // loops into (spilled) registers.
-        uint32_t zl = tid;
-        asm volatile("" : "+v"(zl));
+        uint32_t zb = zlane * (uint32_t)sizeof(T);
+        asm volatile("" : "+v"(zb));
         static_for<NR>([&](auto Rc) {
             constexpr int R = decltype(Rc)::value;
             // opaque row count: the R < mb tests stay scalar compares at
@@ -291,13 +332,13 @@ __device__ __forceinline__ int run_iterations(T* __restrict__ app, const cb_job&
             asm volatile("" : "+s"(mbr));
             if (R < mbr) {
                 shift_ptr shr = sh;
-                uint32_t Zr = Z;
+                uint32_t Zr = Zb;
                 asm volatile("" : "+s"(shr), "+s"(Zr));
                 if (active) {
                     if constexpr (std::is_same<T, float>::value)
-                        process_row<BG, R, NR>(app, shr, zl, Zr, alpha, st_n1, st_n2, st_meta);
+                        process_row<BG, R, NR>(app, lane_base, shr, zb, Zr, alpha, st_n1, st_n2, st_meta);
                     else
-                        process_row<BG, R, NW>(app, shr, zl, Zr, alpha, st_h);
+                        process_row<BG, R, NW>(app, lane_base, shr, zb, Zr, alpha, st_h);
                 }
                 // barrier before the next row only where it shares a column
                 // with the rows since the last barrier, and at the iteration end
@@ -313,12 +354,22 @@ __device__ __forceinline__ int run_iterations(T* __restrict__ app, const cb_job&
                 asm volatile("" : "+s"(mbr));
                 if (R < mbr) {
                     shift_ptr shr = sh;
-                    uint32_t Zr = Z;
+                    uint32_t Zr = Zb;
                     asm volatile("" : "+s"(shr), "+s"(Zr));
-                    if (active) fail |= row_syndrome<BG, R, T>(app, shr, zl, Zr);
+                    if (active) fail |= row_syndrome<BG, R, T>(lane_base, shr, zb, Zr);
                 }
             });
-            if (!__syncthreads_or((int)fail)) break;
+            if (count == 1) {
+                if (!__syncthreads_or((int)fail)) break;
+            } else {
+                // per code block: any failing check row keeps it iterating
+                if (tid < (uint32_t)count) cb_flags[tid] = 0u;
+                __syncthreads();
+                if (active && fail) cb_flags[cb] = 1u;
+                __syncthreads();
+                if (active && cb_flags[cb] == 0u) active = false;
+                if (!__syncthreads_or((int)active)) break;
+            }
         }
     }
     return it;
@@ -381,36 +432,62 @@ __device__ __forceinline__ void store_hard_words(const T* __restrict__ app, int 
     __syncthreads();
 }
 
-// Decode one code block with the whole workgroup; LDS `app` holds
-// (NSYS + mb) * Z values of type T.  Every thread of the workgroup calls it.
-template <int BG, typename T, typename TIn, int MAXR>
-__device__ __forceinline__ int decode_one(T* __restrict__ app, const cb_job& job)
+// Buffers of one code block of a group.
+struct cb_buffers {
+    const void* llr;
+    uint32_t* out;
+    _Float16* soft;  // nullptr: no soft output
+};
+
+// Decode `count` code blocks of one (BG, Z, mb) configuration with the whole
+// workgroup (count * Z <= blockDim.x); LDS `app` holds count regions of
+// group_stride((NSYS + mb) * Z) values of type T.  buf(c) gives code block
+// c's buffers.  Every thread of the workgroup calls it.
+template <int BG, typename T, typename TIn, int MAXR, typename Buf>
+__device__ __forceinline__ int decode_group(T* __restrict__ app, const cb_job& job, int count, const Buf& buf,
+                                            uint32_t* cb_flags)
 {
     const int nval = (bg_traits<BG>::NSYS + job.mb) * job.Z;
-    load_llrs<T, TIn>(app, static_cast<const TIn*>(job.llr), nval, job.clamp);
-    const int iters = run_iterations<BG, T, MAXR>(app, job);
+    const int stride = group_stride(nval);
+    for (int c = 0; c < count; ++c)
+        load_llrs_nosync<T, TIn>(app + c * stride, static_cast<const TIn*>(buf(c).llr), nval, job.clamp);
+    __syncthreads();
+    const int iters = run_iterations<BG, T, MAXR>(app, job, count, stride, cb_flags);
     const int nbits = job.Kb * job.Z;
-    store_hard<T>(app, nbits, job.out);
-    // soft outputs: Kb*Z binary16 APP values
-    if (job.soft)
-        for (int n = threadIdx.x; n < nbits; n += blockDim.x) job.soft[n] = (_Float16)app[n];
+    for (int c = 0; c < count; ++c) {
+        const cb_buffers b = buf(c);
+        store_hard<T>(app + c * stride, nbits, b.out);
+        // soft outputs: Kb*Z binary16 APP values
+        if (b.soft)
+            for (int n = threadIdx.x; n < nbits; n += blockDim.x) b.soft[n] = (_Float16)app[c * stride + n];
+    }
     return iters;
 }
 
-// The code block of a transport-block launch handled by this workgroup.
+// Code blocks per workgroup of a transport-block launch: small lifting
+// sizes share a wave (64 / Z code blocks for Z <= 32), larger ones get a
+// workgroup each.  Host and device use the same rule, so a launch
+// descriptor needs nothing beyond cuphyLDPCDecodeDesc_t.
+__host__ __device__ constexpr int tb_group_size(int Z) { return (Z >= 2 && Z <= 32) ? 64 / Z : 1; }
+constexpr int TB_MAX_GROUP = 32;
+
+// Number of code words of a transport-block launch.
+__host__ __device__ inline int desc_codewords(const cuphyLDPCDecodeDesc_t& d)
+{
+    int n = 0;
+    for (int i = 0; i < d.num_tbs; ++i) n += d.llr_input[i].num_codewords;
+    return n;
+}
+
+// Shared configuration of a transport-block launch.
 template <typename TIn>
 __device__ __forceinline__ cb_job tb_job(const cuphyLDPCDecodeDesc_t& desc)
 {
     const cuphyLDPCDecodeConfigDesc_t& cfg = desc.config;
-    const cw_location loc = locate_cw(desc, blockIdx.x);
     cb_job job;
-    job.llr = static_cast<const TIn*>(desc.llr_input[loc.tb].addr) +
-              (size_t)loc.idx * (size_t)desc.llr_input[loc.tb].stride_elements;
-    job.out = desc.tb_output[loc.tb].addr + (size_t)loc.idx * (size_t)desc.tb_output[loc.tb].stride_words;
-    job.soft = ((cfg.flags & CUPHY_LDPC_DECODE_WRITE_SOFT_OUTPUTS) && desc.llr_output[loc.tb].addr)
-                   ? static_cast<_Float16*>(desc.llr_output[loc.tb].addr) +
-                         (size_t)loc.idx * (size_t)desc.llr_output[loc.tb].stride_elements
-                   : nullptr;
+    job.llr = nullptr;
+    job.out = nullptr;
+    job.soft = nullptr;
     job.Z = cfg.Z;
     job.mb = cfg.num_parity_nodes;
     job.Kb = cfg.Kb;
@@ -422,6 +499,23 @@ __device__ __forceinline__ cb_job tb_job(const cuphyLDPCDecodeDesc_t& desc)
     return job;
 }
 
+// Buffers of code word `cw` of a transport-block launch.
+template <typename TIn>
+__device__ __forceinline__ cb_buffers tb_buffers(const cuphyLDPCDecodeDesc_t& desc, int cw)
+{
+    const cw_location loc = locate_cw(desc, cw);
+    cb_buffers b;
+    b.llr = static_cast<const TIn*>(desc.llr_input[loc.tb].addr) +
+            (size_t)loc.idx * (size_t)desc.llr_input[loc.tb].stride_elements;
+    b.out = desc.tb_output[loc.tb].addr + (size_t)loc.idx * (size_t)desc.tb_output[loc.tb].stride_words;
+    b.soft = ((desc.config.flags & CUPHY_LDPC_DECODE_WRITE_SOFT_OUTPUTS) && desc.llr_output[loc.tb].addr)
+                 ? static_cast<_Float16*>(desc.llr_output[loc.tb].addr) +
+                       (size_t)loc.idx * (size_t)desc.llr_output[loc.tb].stride_elements
+                 : nullptr;
+    return b;
+}
+
+// Shared configuration of a batch work item (the first of its group).
 __device__ __forceinline__ cb_job item_job(const cuphyHipLDPCWorkItem_t& w)
 {
     cb_job job;

@@ -43,39 +43,67 @@
 namespace nrldpc {
 
 
+// Workgroup b decodes code words [b*P, b*P + P) of the launch, P =
+// tb_group_size(Z) (several small-Z code blocks share a wave).
 template <int BG, typename T, typename TIn, int MAXR>
 __global__ __launch_bounds__(384) void ldpc_layered_tb(cuphyLDPCDecodeDesc_t desc)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    decode_one<BG, T, TIn, MAXR>(reinterpret_cast<T*>(lds_raw), tb_job<TIn>(desc));
+    __shared__ uint32_t s_cb_flags[TB_MAX_GROUP];
+    const int P = tb_group_size(desc.config.Z);
+    const int first = (int)blockIdx.x * P;
+    const int count = min(P, desc_codewords(desc) - first);
+    decode_group<BG, T, TIn, MAXR>(
+        reinterpret_cast<T*>(lds_raw), tb_job<TIn>(desc), count,
+        [&](int c) { return tb_buffers<TIn>(desc, first + c); }, s_cb_flags);
 }
 
 // Persistent heterogeneous decoder: each workgroup repeatedly takes the
-// next work item from a device queue (atomic counter) and decodes it, so
-// code blocks of any (BG, Z, mb) share one launch.  The queue state
-// (2 words: next item, finished workgroups) is returned to zero by the
-// last workgroup to finish, so the launch can be replayed from a graph.
+// next group of work items from a device queue (atomic counter) and decodes
+// it, so code blocks of any (BG, Z, mb) share one launch.  Group g is items
+// [group_first[g], group_first[g + 1]) (one item per group when group_first
+// is null): code blocks of one configuration decoded side by side, lanes
+// [c*Z, (c+1)*Z) on item c.  A group that does not fit the workgroup (count
+// * Z > blockDim.x) or the LDS (count regions > lds_values) is cut to what
+// fits; the host never builds one.  The queue state (2 words: next group,
+// finished workgroups) is returned to zero by the last workgroup to finish,
+// so the launch can be replayed from a graph.
+constexpr int PERSISTENT_MAX_GROUP = 192;  // 384 lanes / Z = 2
+
 template <typename T, typename TIn, int MAXR>
 __global__ __launch_bounds__(384) void ldpc_layered_persistent(const cuphyHipLDPCWorkItem_t* __restrict__ items,
-                                                               int n_items, uint32_t* __restrict__ queue)
+                                                               const uint32_t* __restrict__ group_first,
+                                                               int n_groups, int lds_values,
+                                                               uint32_t* __restrict__ queue)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     __shared__ int s_item;
+    __shared__ uint32_t s_cb_flags[PERSISTENT_MAX_GROUP];
     T* app = reinterpret_cast<T*>(lds_raw);
     for (;;) {
         if (threadIdx.x == 0) s_item = (int)atomicAdd(&queue[0], 1u);
         __syncthreads();
         // uniform: the item's fields then come in through scalar loads and
         // the per-row bounds (R < mb) are scalar branches
-        const int i = __builtin_amdgcn_readfirstlane(s_item);
+        const int g = __builtin_amdgcn_readfirstlane(s_item);
         __syncthreads();
-        if (i >= n_items) break;
-        const cuphyHipLDPCWorkItem_t& w = items[i];
-        const cb_job job = item_job(w);
-        if (w.BG == 1)
-            decode_one<1, T, TIn, MAXR>(app, job);
-        else
-            decode_one<2, T, TIn, MAXR>(app, job);
+        if (g >= n_groups) break;
+        const int first = group_first ? (int)group_first[g] : g;
+        int count = group_first ? (int)group_first[g + 1] - first : 1;
+        const cuphyHipLDPCWorkItem_t* grp = items + first;
+        const cb_job job = item_job(grp[0]);
+        const int nval = ((grp[0].BG == 1) ? 22 : 10) * job.Z + job.mb * job.Z;
+        count = min(count, min((int)blockDim.x / max(job.Z, 1), lds_values / group_stride(nval)));
+        count = min(count, PERSISTENT_MAX_GROUP);
+        if (count >= 1) {
+            auto buf = [&](int c) {
+                return cb_buffers{grp[c].llr, grp[c].out, static_cast<_Float16*>(grp[c].soft)};
+            };
+            if (grp[0].BG == 1)
+                decode_group<1, T, TIn, MAXR>(app, job, count, buf, s_cb_flags);
+            else
+                decode_group<2, T, TIn, MAXR>(app, job, count, buf, s_cb_flags);
+        }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
@@ -123,12 +151,6 @@ void* select_persistent_kernel(cuphyDataType_t llr_type, bool fp16_math)
     return nullptr;
 }
 
-int total_codewords(const cuphyLDPCDecodeDesc_t& d)
-{
-    int n = 0;
-    for (int i = 0; i < d.num_tbs; ++i) n += d.llr_input[i].num_codewords;
-    return n;
-}
 
 }  // namespace
 
@@ -161,9 +183,11 @@ cuphyStatus_t fill_launch(const cuphyLDPCDecodeDesc_t& desc, launch_geometry& g)
     g.func = select_kernel(c.BG, c.llr_type, fp16, c.num_parity_nodes);
     if (!g.func) return CUPHY_STATUS_UNSUPPORTED_CONFIG;
     const int nsys = (c.BG == 1) ? 22 : 10;
-    g.lds_bytes = (size_t)(nsys + c.num_parity_nodes) * (size_t)c.Z * (fp16 ? 2u : 4u);
-    g.block = dim3((unsigned)((c.Z + 63) / 64 * 64));
-    g.grid = dim3((unsigned)total_codewords(desc));
+    const int P = tb_group_size(c.Z);
+    const size_t stride = (size_t)(((nsys + c.num_parity_nodes) * c.Z + 7) & ~7);
+    g.lds_bytes = (size_t)P * stride * (fp16 ? 2u : 4u);
+    g.block = dim3((unsigned)((P * c.Z + 63) / 64 * 64));
+    g.grid = dim3((unsigned)((desc_codewords(desc) + P - 1) / P));
     return CUPHY_STATUS_SUCCESS;
 }
 
@@ -201,27 +225,32 @@ cuphyStatus_t decode_tb(const cuphyLDPCDecodeDesc_t& desc, hipStream_t strm)
     return CUPHY_STATUS_SUCCESS;
 }
 
-cuphyStatus_t decode_batch(const cuphyHipLDPCWorkItem_t* d_items, int n_items, cuphyDataType_t llr_type,
-                           bool fp16_math, int max_values_per_cb, int max_mb, uint32_t* d_queue, int cu_count,
-                           hipStream_t strm)
+cuphyStatus_t decode_batch(const cuphyHipLDPCWorkItem_t* d_items, const uint32_t* d_group_first, int n_groups,
+                           cuphyDataType_t llr_type, bool fp16_math, int max_values_per_group, int max_mb,
+                           int block_threads, uint32_t* d_queue, int cu_count, hipStream_t strm)
 {
-    if (n_items < 0 || !d_queue || (n_items > 0 && !d_items)) return CUPHY_STATUS_INVALID_ARGUMENT;
-    if (max_values_per_cb <= 0 || max_values_per_cb > 68 * 384) return CUPHY_STATUS_INVALID_ARGUMENT;
-    if (n_items == 0) return CUPHY_STATUS_SUCCESS;
-    if (llr_type != CUPHY_R_32F && llr_type != CUPHY_R_16F) return CUPHY_STATUS_UNSUPPORTED_TYPE;
+    if (n_groups < 0 || !d_queue || (n_groups > 0 && !d_items)) return CUPHY_STATUS_INVALID_ARGUMENT;
+    if (block_threads < 64 || block_threads > 384 || block_threads % 64) return CUPHY_STATUS_INVALID_ARGUMENT;
     const bool f16 = fp16_math || llr_type == CUPHY_R_16F;
+    const size_t elem = f16 ? 2 : 4;
+    // a group's LDS regions are at most 384 lanes' worth of code blocks
+    if (max_values_per_group <= 0 || (size_t)max_values_per_group * elem > 160u * 1024u)
+        return CUPHY_STATUS_INVALID_ARGUMENT;
+    if (n_groups == 0) return CUPHY_STATUS_SUCCESS;
+    if (llr_type != CUPHY_R_32F && llr_type != CUPHY_R_16F) return CUPHY_STATUS_UNSUPPORTED_TYPE;
     void* func = max_mb <= SMALL_MAXR ? select_persistent_kernel<SMALL_MAXR>(llr_type, f16)
                                       : select_persistent_kernel<46>(llr_type, f16);
-    const unsigned block = 384;
-    const size_t elem = f16 ? 2 : 4;
-    const size_t lds = (size_t)max_values_per_cb * elem;
+    const unsigned block = (unsigned)block_threads;
+    const size_t lds = (size_t)max_values_per_group * elem;
     cuphyStatus_t s = ensure_lds_attr(func, lds);
     if (s != CUPHY_STATUS_SUCCESS) return s;
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, func, (int)block, lds) != hipSuccess || per_cu < 1)
         per_cu = 1;
-    const int grid = std::min(n_items, std::max(1, cu_count) * per_cu);
-    void* args[3] = {const_cast<cuphyHipLDPCWorkItem_t**>(&d_items), &n_items, &d_queue};
+    const int grid = std::min(n_groups, std::max(1, cu_count) * per_cu);
+    int lds_values = max_values_per_group;
+    void* args[5] = {const_cast<cuphyHipLDPCWorkItem_t**>(&d_items), const_cast<uint32_t**>(&d_group_first),
+                     &n_groups, &lds_values, &d_queue};
     if (hipLaunchKernel(func, dim3((unsigned)grid), dim3(block), args, lds, strm) != hipSuccess)
         return CUPHY_STATUS_INTERNAL_ERROR;
     return CUPHY_STATUS_SUCCESS;

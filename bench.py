@@ -40,8 +40,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU work budget of the baseline leg")
-    ap.add_argument("--sections", default="main,config1,config3,fused,serving",
-                    help="comma-separated subset of: main, config1, config3, fused, serving; or 'copyprobe' "
+    ap.add_argument("--sections", default="main,config1,config3,throughput,fused,serving",
+                    help="comma-separated subset of: main, config1, config3, throughput, fused, serving; or 'copyprobe' "
                          "alone: three 1 GiB device copies (calibrates HBM byte counters)")
     ap.add_argument("--serve-ms", type=float, default=40.0, help="arrival window of each serving load point")
     ap.add_argument("--serve-inflight", type=int, default=3, help="fused launches in flight in the serving engine")
@@ -345,6 +345,9 @@ def main():
     config3 = None
     if "config3" in sections:
         config3 = config3_section(dec, stream, a.steps)
+    throughput = None
+    if "throughput" in sections:
+        throughput = throughput_section(dec, stream, a.steps)
     fused = None
     if "fused" in sections:
         fused = fused_slot_section(dec, a.steps)
@@ -420,6 +423,56 @@ def config3_section(dec, stream, steps):
     return config3
 
 
+def throughput_section(dec, stream, steps, n_slots=16):
+    """Batched throughput with many slots in flight: n_slots heterogeneous
+    slots decoded as one batch, (a) Zc over 64..384 as in the heterogeneous
+    slot, (b) Zc over all 51 lifting sizes 2..384.  Each batch is timed with
+    one code block per 384-thread workgroup and with the tuned packing
+    (several code blocks per workgroup, workgroup size by Zc); the decoded
+    bits of the two must be identical."""
+    import torch
+    from aerial_amd import _lib as L
+    from aerial_amd import workloads as W
+    from aerial_amd.batch import BatchDecoder
+    out = {}
+    for name, z_min in (("zc_64_384", 64), ("zc_2_384", 2)):
+        tbs = []
+        for k in range(n_slots):
+            tbs += W.heterogeneous_slot(seed=100 + k, z_min=z_min)
+        bits = sum(tb.ncb * tb.info_bits_per_cb for tb in tbs)
+        res = {}
+        outs = {}
+        for pack in (False, True):
+            for tb in tbs:
+                tb.out.zero_()
+            b = BatchDecoder(dec, pack=pack).build(tbs, llr_type=L.CUPHY_R_16F, fp16_math=True,
+                                                   max_iterations=ITERS, alpha=0.8125, clamp_value=32.0)
+            b.capture()
+            for _ in range(2):
+                b.replay()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(max(2, steps // 2)):
+                b.replay()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / max(2, steps // 2)
+            res["packed" if pack else "one_per_workgroup"] = {
+                "ms": round(ms, 4), "gbps": round(bits / (ms * 1e-3) / 1e9, 3), "launches": len(b.subs)}
+            outs[pack] = [tb.out.clone() for tb in tbs]
+            del b
+        res["packed_over_one_per_workgroup"] = round(res["one_per_workgroup"]["ms"] / res["packed"]["ms"], 3)
+        res["bit_identical"] = all(torch.equal(x, y) for x, y in zip(outs[False], outs[True]))
+        res["code_blocks"] = sum(tb.ncb for tb in tbs)
+        res["info_bits"] = bits
+        out[name] = res
+        del tbs, outs
+    out["workload"] = ("%d heterogeneous slots (16 UEs each, BG1/BG2, R 0.2..0.93, 1-24 CBs/TB, fp16 LLRs, "
+                       "10 iterations) in one batch; packing from aerial_amd/tuning_gfx950.json" % n_slots)
+    return out
+
+
 def line_of(a, v):
     """The one JSON line of the run (v: main()'s locals)."""
     info_bits, ncb, tbs = v["info_bits"], v["ncb"], v["tbs"]
@@ -453,6 +506,7 @@ def line_of(a, v):
         "config1_plain_fp32": v["config1"],
         "config3_full_load_20_cells": v["config3"],
         "fused_slot_64_cells_int8": v["fused"],
+        "batched_throughput_16_slots": v["throughput"],
         "serving": v["serving"],
     }
     return line

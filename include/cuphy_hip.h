@@ -382,6 +382,25 @@ cuphyStatus_t CUPHYWINAPI cuphyHipLDPCBatchDecode(cuphyLDPCDecoder_t decoder, co
                                                   int max_values_per_cb, int max_parity_nodes, void* d_workspace,
                                                   hipStream_t strm);
 
+/* Grouped form of cuphyHipLDPCBatchDecode: the device queue hands out groups
+ * of items, group g being items [d_group_first[g], d_group_first[g + 1])
+ * (d_group_first: num_groups + 1 device words).  The items of a group share
+ * BG, Z, num_parity_nodes, Kb, max_iterations, flags, norm and clamp_value
+ * and are decoded side by side by one workgroup of block_threads threads (a
+ * multiple of 64, <= 384, >= count * Z), lanes [c*Z, (c+1)*Z) on item c, each
+ * with its own early termination; results equal the items decoded one by
+ * one.  max_values_per_group: max over groups of count * roundup8((nsys + mb)
+ * * Z) (sizes LDS).  The reference packs small-Z code blocks of one TB per
+ * CTA (ldpc2_reg_index_fp_desc_dyn_small.cu:115-618); here any configuration
+ * can be packed, with the group size chosen per (BG, Z) by the tuning tables
+ * of the host layer.  Same workspace contract as cuphyHipLDPCBatchDecode. */
+cuphyStatus_t CUPHYWINAPI cuphyHipLDPCBatchDecodeGroups(cuphyLDPCDecoder_t decoder,
+                                                        const cuphyHipLDPCWorkItem_t* d_items,
+                                                        const uint32_t* d_group_first, int num_groups,
+                                                        cuphyDataType_t llr_type, int fp16_math,
+                                                        int max_values_per_group, int max_parity_nodes,
+                                                        int block_threads, void* d_workspace, hipStream_t strm);
+
 /* One transport block (one transmission) of a fused slot decode.  The
  * reference runs de-rate-matching, LDPC decoding and CRC checking as three
  * kernels per TB group (pusch_rx.cpp:3007-3021); cuphyHipPuschSlotDecode

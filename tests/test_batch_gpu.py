@@ -86,7 +86,8 @@ def test_batch_order_independent(dec):
     check(tbs, refs)
     for tb in tbs:
         tb.out.fill_(-1)
-    b = BatchDecoder(dec, split_min=1).build(tbs, llr_type=L.CUPHY_R_16F, fp16_math=True, max_iterations=7)
+    b = BatchDecoder(dec, split_min=1, pack=False).build(tbs, llr_type=L.CUPHY_R_16F, fp16_math=True,
+                                                          max_iterations=7)
     assert len(b.subs) == 2  # high-rate CBs on the register-lean variant, concurrently
     b.launch()
     torch.cuda.synchronize()
@@ -126,3 +127,67 @@ def test_batch_single_and_empty(dec):
     b.subs = []
     b.launch()  # no-op
     torch.cuda.synchronize()
+
+
+# Packed groups: several code blocks of one (BG, Z, mb) per workgroup.
+PACK_MIX = [(1, 2, 20, 70), (2, 15, 4, 30), (1, 36, 8, 12), (2, 64, 42, 9), (1, 72, 4, 7), (2, 104, 10, 5),
+            (1, 176, 12, 4), (1, 384, 6, 2), (2, 7, 42, 25)]
+
+
+def _max_pack_table(mix):
+    t = {}
+    for bg, Z, mb, _ in mix:
+        P = 384 // Z
+        t["%d,%d,%s" % (bg, Z, "small" if mb <= 8 else "full")] = {"P": P, "block": 64 * (-(-P * Z // 64))}
+    return t
+
+
+def make_packed(seed, et, n_iter):
+    tbs, refs = [], []
+    for k, (bg, Z, mb, ncb) in enumerate(PACK_MIX):
+        # SNR near the waterfall so that code blocks stop at different iterations
+        llr, _ = V.make_cbs(bg, Z, mb, ncb, seed=seed + k, snr_db=2.0)
+        llr = V.to_half_rn(llr).astype(np.float32)
+        stride = (llr.shape[1] + 7) // 8 * 8
+        host = np.zeros((ncb, stride), np.float32)
+        host[:, :llr.shape[1]] = llr
+        kbz = V.nsys(bg) * Z
+        nw = (kbz + 31) // 32
+        tb = SimpleNamespace(bg=bg, Z=Z, mb=mb, ncb=ncb, llr=torch.from_numpy(host).cuda().half(),
+                             out=torch.full((ncb, nw), -1, dtype=torch.int32, device="cuda"),
+                             soft=torch.zeros((ncb, kbz), dtype=torch.float16, device="cuda"))
+        cfg = O.dec_cfg(bg, Z, mb, n_iter=n_iter, alpha=0.8125, clamp=32.0, mode=O.FP16, early_term=et)
+        hs, ss, its = [], [], []
+        for i in range(ncb):
+            h, it, app = O.decode_cb(cfg, llr[i], want_app=True)
+            hs.append(h)
+            ss.append(app[:kbz].astype(np.float16))
+            its.append(it)
+        tbs.append(tb)
+        refs.append((np.stack(hs), np.stack(ss), its))
+    return tbs, refs
+
+
+@pytest.mark.parametrize("et", [False, True])
+@pytest.mark.parametrize("table", ["max_pack", "tuned"])
+def test_batch_packed_bit_exact(dec, table, et):
+    from aerial_amd import _lib as L
+    from aerial_amd.batch import BatchDecoder
+    print("[case] packed batch table=%s et=%s fp16 seed=300" % (table, et), flush=True)
+    n_iter = 12
+    tbs, refs = make_packed(300, et, n_iter)
+    if et:
+        spread = [len(set(r[2])) for r in refs]
+        assert max(spread) > 1, "early termination did not vary within a TB"
+    b = BatchDecoder(dec, split_min=1, table=_max_pack_table(PACK_MIX) if table == "max_pack" else None)
+    b.build(tbs, llr_type=L.CUPHY_R_16F, fp16_math=True, max_iterations=n_iter,
+            flags=L.CUPHY_LDPC_DECODE_EARLY_TERM if et else 0, soft=[tb.soft for tb in tbs])
+    if table == "max_pack":
+        assert any(sb.n_items > sb.n_groups for sb in b.subs), "no code blocks were packed"
+    b.launch()
+    torch.cuda.synchronize()
+    for tb, (h, sft, _) in zip(tbs, refs):
+        assert np.array_equal(tb.out.cpu().numpy().view(np.uint32), h), (tb.bg, tb.Z, tb.mb)
+        got = tb.soft.cpu().numpy()
+        assert np.array_equal(got.view(np.uint16), sft.view(np.uint16)), (tb.bg, tb.Z, tb.mb)
+    assert all(int(w.abs().sum().item()) == 0 for w in b.workspaces)

@@ -132,6 +132,29 @@ def test_sweep_bit_exact(dec, bg, Z, mb, dtype):
     assert_soft_equal(s_gpu, s_ref)
 
 
+@pytest.mark.parametrize("bg,Z,mb,ncb", [(1, 2, 46, 70), (2, 3, 42, 45), (1, 16, 10, 9), (2, 30, 6, 5)])
+def test_small_z_packed_tb(dec, bg, Z, mb, ncb):
+    # 64 // Z code blocks share one wave (tb_group_size): full and partial groups
+    _log("packed small-Z TB BG%d Z=%d mb=%d %d CBs fp16 seed=21" % (bg, Z, mb, ncb))
+    llr, _ = V.make_cbs(bg, Z, mb, ncb, seed=21, snr_db=1.5)
+    llr = V.to_half_rn(llr).astype(np.float32)
+    h_gpu, s_gpu = run_gpu(dec, bg, Z, mb, llr, n_iter=8, dtype="fp16")
+    h_ref, s_ref, _ = run_oracle(bg, Z, mb, llr, n_iter=8, mode=O.FP16)
+    assert np.array_equal(h_gpu, h_ref)
+    assert_soft_equal(s_gpu, s_ref)
+
+
+def test_small_z_packed_tb_early_termination(dec):
+    _log("packed small-Z TB BG2 Z=8 mb=20 40 CBs fp32 ET seed=23")
+    from aerial_amd import _lib as L
+    llr, _ = V.make_cbs(2, 8, 20, 40, seed=23, snr_db=2.0)
+    h_gpu, s_gpu = run_gpu(dec, 2, 8, 20, llr, n_iter=25, flags=L.CUPHY_LDPC_DECODE_EARLY_TERM)
+    h_ref, s_ref, iters = run_oracle(2, 8, 20, llr, n_iter=25, et=True)
+    assert len(set(iters)) > 1, iters
+    assert np.array_equal(h_gpu, h_ref)
+    assert_soft_equal(s_gpu, s_ref)
+
+
 @pytest.mark.parametrize("Kb", [6, 8, 9, 10])
 def test_bg2_kb_output_width(dec, Kb):
     _log("BG2 Z=64 mb=10 Kb=%d" % Kb)
