@@ -16,6 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "lib")
 OBJ_DIR = os.path.join(HERE, "lib", "obj")
 LIB = os.path.join(OUT_DIR, "libcuphy_hip.so")
+BENCH = os.path.join(OUT_DIR, "decode_bench")  # native C-ABI driver (csrc/decode_bench.cpp)
 ARCH = os.environ.get("NRLDPC_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = [
@@ -56,8 +57,15 @@ def build(verbose=True):
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
     os.replace(tmp, LIB)
+    cmd = ["hipcc", "-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-O2", "-std=c++17",
+           "-I" + os.path.join(ROOT, "include"), os.path.join(CSRC, "decode_bench.cpp"), "-o", BENCH + ".tmp",
+           "-L" + OUT_DIR, "-lcuphy_hip", "-Wl,-rpath,$ORIGIN", "-L/opt/rocm/lib", "-lamdhip64"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("decode_bench build failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
+    os.replace(BENCH + ".tmp", BENCH)
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", BENCH)
     return LIB
 
 
