@@ -40,6 +40,8 @@ CUPHY_MAX_N_CSI2_REPORTS_PER_UE = 16
 MAX_WORDS_PER_CODE_BLOCK = 264
 SLOT_CB_ERROR = 0xFFFFFFFF  # CRC value of a CB/TB the fused slot kernel rejected
 CUPHY_HIP_HARQ_LAST_TX = 0x100
+CUPHY_HIP_SLOT_TB_NOTIFY = 0x200
+CUPHY_HIP_HARQ_SERVER_RESIDENT = 0x1
 
 
 class Norm(C.Union):
@@ -126,7 +128,7 @@ class PuschSlotTb(C.Structure):
 
 class HarqServerConfig(C.Structure):
     _fields_ = [("harq_pool_bytes", C.c_size_t), ("max_inflight", C.c_int), ("max_tbs_per_launch", C.c_int),
-                ("max_cbs_per_launch", C.c_int), ("llr_type", C.c_int)]
+                ("max_cbs_per_launch", C.c_int), ("llr_type", C.c_int), ("flags", C.c_uint32)]
 
 
 class HarqRequest(C.Structure):

@@ -36,11 +36,16 @@ class HarqServer:
     """Owner of a cuphyHipHarqServer_t."""
 
     def __init__(self, decoder, pool_bytes=2 << 30, max_inflight=3, max_tbs=1024, max_cbs=8192,
-                 llr_type=L.CUPHY_R_16F):
+                 llr_type=L.CUPHY_R_16F, resident=False):
+        """resident: one resident launch takes code blocks as they are
+        admitted (max_tbs / max_cbs then bound the TBs / code blocks in
+        flight); otherwise admitted transmissions are batched into up to
+        max_inflight launches."""
         self.dec = decoder
         self._lib = L.lib()
         cfg = L.HarqServerConfig(harq_pool_bytes=pool_bytes, max_inflight=max_inflight, max_tbs_per_launch=max_tbs,
-                                 max_cbs_per_launch=max_cbs, llr_type=llr_type)
+                                 max_cbs_per_launch=max_cbs, llr_type=llr_type,
+                                 flags=L.CUPHY_HIP_HARQ_SERVER_RESIDENT if resident else 0)
         self.h = C.c_void_p()
         L.check("cuphyHipHarqServerCreate", self._lib.cuphyHipHarqServerCreate(decoder.dec, C.byref(cfg),
                                                                                C.byref(self.h)))

@@ -597,7 +597,7 @@ cuphyStatus_t CUPHYWINAPI cuphyHipLDPCBatchDecodeGroups(cuphyLDPCDecoder_t decod
 cuphyStatus_t CUPHYWINAPI cuphyHipPuschSlotDecodeGetWorkspaceSize(int num_cbs, size_t* sizeInBytes)
 {
     if (!sizeInBytes || num_cbs < 0) return CUPHY_STATUS_INVALID_ARGUMENT;
-    *sizeInBytes = (size_t)(nrldpc::FUSED_WS_QUEUE_WORDS + (size_t)num_cbs) * sizeof(uint32_t);
+    *sizeInBytes = (size_t)(nrldpc::FUSED_WS_QUEUE_WORDS + 2 * (size_t)num_cbs) * sizeof(uint32_t);
     return CUPHY_STATUS_SUCCESS;
 }
 

@@ -19,9 +19,22 @@
 //     previous one): requests that arrive while launches run are batched
 //     into the next free slot, so launches overlap on the device.
 //   * per launch: one host->device copy of the PerTbParams, job and item
-//     arrays, one or two persistent fused launches (the register-lean
+//     arrays, then one or two persistent fused launches (the register-lean
 //     variant for mb <= 8 when at least one CU's worth of such CBs is
-//     present), one device->host copy of the TB CRCs, an event.
+//     present; the two on two streams, concurrently), an event per stream.
+//   * resident mode (CUPHY_HIP_HARQ_SERVER_RESIDENT): instead of batched
+//     launches, one resident launch (pusch_serve_resident) takes code
+//     blocks from a ring in host-mapped memory as admitted TBs are
+//     published; TBs occupy descriptor slots until their CRC appears.  Its
+//     workgroups leave after 20 ms without work (handing their tickets to
+//     the next launch) and the engine starts it again when work is open.
+//   * per-TB completion: each TB's CRC is written by the kernel straight
+//     into the slot's host-mapped pinned array (CUPHY_HIP_SLOT_TB_NOTIFY:
+//     after all of the TB's outputs are in device memory) as soon as the
+//     TB's own code blocks are decoded (pre-set to a value no CRC takes);
+//     Poll() completes a TB when its value appears, so a TB does not wait
+//     for the longest code block of its launch.  The slot, and the HARQ
+//     buffers its TBs used, are released once its launches have ended.
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -44,6 +57,7 @@ using clk = std::chrono::steady_clock;
 
 constexpr size_t POOL_ALIGN = 256;
 constexpr uint32_t CB_REJECTED = 0xFFFFFFFFu;
+constexpr uint32_t CRC_PENDING = 0xFFFFFFFEu;  // TB CRC slot before the kernel writes it (CRCs are < 2^24)
 constexpr int SMALL_MB = 8;
 
 size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
@@ -109,15 +123,18 @@ struct queued {
 };
 
 struct launch_slot {
-    hipStream_t stream = nullptr;
-    hipEvent_t done = nullptr;
+    hipStream_t stream[2] = {nullptr, nullptr};  // high-rate class, full class
+    hipEvent_t done[2] = {nullptr, nullptr};
+    hipEvent_t staged = nullptr;       // descriptors copied (second stream waits on it)
     bool busy = false;
     unsigned char* h_stage = nullptr;  // pinned: params | jobs | items
     unsigned char* d_stage = nullptr;
-    uint32_t* h_crc = nullptr;         // pinned TB CRCs of the launch
-    uint32_t* d_crc = nullptr;
+    uint32_t* h_crc = nullptr;         // pinned, device-mapped: TB CRCs written by the kernel
+    uint32_t* dv_crc = nullptr;        // device view of h_crc
     uint32_t* d_ws[2] = {nullptr, nullptr};
     std::vector<queued> reqs;
+    std::vector<uint8_t> taken;        // TB already completed
+    int n_open = 0;                    // TBs not yet completed
     double t_launch = 0.0;
 };
 
@@ -151,6 +168,33 @@ struct cuphyHipHarqServer {
     clk::time_point t0;
     size_t ws_bytes = 0, stage_bytes = 0;
     uint64_t submitted = 0, completed = 0, launches = 0, admission_waits = 0;
+
+    // resident mode
+    bool resident = false;
+    nrldpc::serve_ring_hdr* h_ring = nullptr;  // pinned, device-mapped
+    nrldpc::serve_ring_hdr* dv_ring = nullptr;
+    uint32_t* h_items = nullptr;
+    uint32_t* dv_items = nullptr;
+    uint32_t ring_size = 0, published = 0, items_open = 0;
+    cuphyHipPuschSlotTb_t* h_jobs = nullptr;
+    cuphyHipPuschSlotTb_t* dv_jobs = nullptr;
+    PerTbParams* h_params = nullptr;
+    PerTbParams* dv_params = nullptr;
+    uint32_t* h_tbcrc = nullptr;
+    uint32_t* dv_tbcrc = nullptr;
+    nrldpc::serve_state* d_state = nullptr;
+    uint32_t* d_tbw = nullptr;
+    hipStream_t r_stream = nullptr;
+    hipEvent_t r_done = nullptr;
+    bool r_running = false;
+    uint64_t idle_ticks = 0, life_ticks = 0;
+    uint32_t* d_abandoned = nullptr;  // one ticket word per workgroup of the resident grid
+    int r_grid = 0;
+    std::vector<int> free_tb_slots;
+    std::vector<queued> slot_req;
+    std::vector<uint8_t> slot_open;
+    std::vector<double> slot_t_publish;
+    int n_open_slots = 0;
 
     double now_us() const { return std::chrono::duration<double, std::micro>(clk::now() - t0).count(); }
     size_t params_off() const { return 0; }
@@ -187,50 +231,241 @@ struct cuphyHipHarqServer {
     cuphyStatus_t retire();
     cuphyStatus_t admit_and_launch();
     cuphyStatus_t launch(launch_slot& s);
+    cuphyStatus_t retire_resident();
+    cuphyStatus_t admit_resident();
+    cuphyStatus_t start_resident();
+    bool idle() const;
     void destroy();
 };
+
+bool cuphyHipHarqServer::idle() const
+{
+    if (resident) return n_open_slots == 0;
+    for (auto& sl : slots)
+        if (sl.busy) return false;
+    return true;
+}
 
 void cuphyHipHarqServer::destroy()
 {
     // teardown: errors are not actionable here, the handles are dropped either way
     for (auto& s : slots) {
-        if (s.stream) {
-            (void)hipStreamSynchronize(s.stream);
-            (void)hipStreamDestroy(s.stream);
+        for (int k = 0; k < 2; ++k) {
+            if (s.stream[k]) {
+                (void)hipStreamSynchronize(s.stream[k]);
+                (void)hipStreamDestroy(s.stream[k]);
+            }
+            if (s.done[k]) (void)hipEventDestroy(s.done[k]);
         }
-        if (s.done) (void)hipEventDestroy(s.done);
+        if (s.staged) (void)hipEventDestroy(s.staged);
         if (s.h_stage) (void)hipHostFree(s.h_stage);
         if (s.h_crc) (void)hipHostFree(s.h_crc);
         if (s.d_stage) (void)hipFree(s.d_stage);
-        if (s.d_crc) (void)hipFree(s.d_crc);
         for (auto w : s.d_ws)
             if (w) (void)hipFree(w);
     }
     slots.clear();
+    if (r_stream) {
+        // resident launch: leave once the ring is drained (bounded by its
+        // idle and lifetime limits in any case)
+        if (h_ring) __atomic_store_n(&h_ring->stop, 1u, __ATOMIC_RELEASE);
+        (void)hipStreamSynchronize(r_stream);
+        (void)hipStreamDestroy(r_stream);
+    }
+    if (r_done) (void)hipEventDestroy(r_done);
+    for (void* h : {(void*)h_ring, (void*)h_items, (void*)h_jobs, (void*)h_params, (void*)h_tbcrc})
+        if (h) (void)hipHostFree(h);
+    if (d_state) (void)hipFree(d_state);
+    if (d_abandoned) (void)hipFree(d_abandoned);
+    if (d_tbw) (void)hipFree(d_tbw);
     if (d_pool) (void)hipFree(d_pool);
     d_pool = nullptr;
 }
 
-// Retire every finished launch: TB CRCs -> completions, HARQ buffers of
-// decoded (or abandoned) TBs back to the pool.
+// Resident mode: start the resident launch (none running).  The device
+// state (tickets, tail seen, abandoned tickets) carries over from the
+// previous launch.
+cuphyStatus_t cuphyHipHarqServer::start_resident()
+{
+    if (nrldpc::serve_resident_launch(dv_ring, dv_items, ring_size - 1, dv_jobs, (int)slot_open.size(), d_state, d_tbw,
+                                      d_abandoned, r_grid, cfg.llr_type, idle_ticks, life_ticks, r_stream) !=
+            CUPHY_STATUS_SUCCESS ||
+        hipEventRecord(r_done, r_stream) != hipSuccess)
+        return CUPHY_STATUS_INTERNAL_ERROR;
+    r_running = true;
+    ++launches;
+    return CUPHY_STATUS_SUCCESS;
+}
+
+// Resident mode: complete every TB whose CRC has appeared; start the
+// resident launch again if it left (idle) while work is still open.
+cuphyStatus_t cuphyHipHarqServer::retire_resident()
+{
+    if (r_running) {
+        const hipError_t q = hipEventQuery(r_done);
+        if (q == hipSuccess)
+            r_running = false;
+        else if (q != hipErrorNotReady)
+            return CUPHY_STATUS_INTERNAL_ERROR;
+    }
+    const double t = now_us();
+    bool open_work = false;
+    for (int k = 0; k < (int)slot_open.size() && n_open_slots > 0; ++k) {
+        if (!slot_open[k]) continue;
+        const uint32_t crc = __atomic_load_n(&h_tbcrc[k], __ATOMIC_ACQUIRE);
+        if (crc == CRC_PENDING) {
+            open_work = true;
+            continue;
+        }
+        const queued& r = slot_req[k];
+        auto it = procs.find(r.req.harq_id);
+        if (it != procs.end()) it->second.busy = false;
+        // the kernel released every output of the TB (HARQ buffer included)
+        // before its CRC: the buffer may go to another TB now
+        if (crc == 0 || crc == CB_REJECTED || (r.req.flags & CUPHY_HIP_HARQ_LAST_TX)) free_proc(r.req.harq_id);
+        complete(r, crc, crc == CB_REJECTED ? CUPHY_STATUS_UNSUPPORTED_CONFIG : CUPHY_STATUS_SUCCESS,
+                 slot_t_publish[k], t);
+        items_open -= r.req.prm.num_CBs;
+        slot_open[k] = 0;
+        free_tb_slots.push_back(k);
+        --n_open_slots;
+    }
+    if (open_work && !r_running) return start_resident();
+    return CUPHY_STATUS_SUCCESS;
+}
+
+// Resident mode: admit pending requests in arrival order (same rules as the
+// batched mode) into free descriptor slots and publish their code blocks.
+cuphyStatus_t cuphyHipHarqServer::admit_resident()
+{
+    if (pending.empty()) return CUPHY_STATUS_SUCCESS;
+    std::deque<queued> rest;
+    bool new_blocked = false;
+    const uint32_t first = published;
+    while (!pending.empty()) {
+        queued q = pending.front();
+        pending.pop_front();
+        const PerTbParams& p = q.req.prm;
+        const uint32_t C = p.num_CBs;
+        if (free_tb_slots.empty() || items_open + C > ring_size ||
+            items_open + C > (uint32_t)cfg.max_cbs_per_launch) {
+            rest.push_back(q);
+            continue;
+        }
+        auto it = procs.find(q.req.harq_id);
+        if (it != procs.end() && it->second.busy) {  // previous transmission still running
+            rest.push_back(q);
+            continue;
+        }
+        if (p.ndi) {
+            const size_t need = harq_bytes(p);
+            if (it != procs.end() && it->second.bytes < need) {
+                free_proc(q.req.harq_id);
+                it = procs.end();
+            }
+            if (it == procs.end()) {
+                size_t off = 0;
+                if (new_blocked || !pool.alloc(need, off)) {
+                    if (!new_blocked) ++admission_waits;
+                    new_blocked = true;
+                    rest.push_back(q);
+                    continue;
+                }
+                harq_proc hp;
+                hp.off = off;
+                hp.bytes = need;
+                it = procs.emplace(q.req.harq_id, hp).first;
+            }
+        } else if (it == procs.end()) {
+            complete(q, CB_REJECTED, CUPHY_STATUS_INVALID_ARGUMENT, now_us(), now_us());
+            continue;
+        }
+        it->second.busy = true;
+        const int k = free_tb_slots.back();
+        free_tb_slots.pop_back();
+        const cuphyHipHarqRequest_t& r = q.req;
+        h_params[k] = r.prm;
+        cuphyHipPuschSlotTb_t& j = h_jobs[k];
+        j.prm = dv_params + k;
+        j.llr = r.llr;
+        j.harq = d_pool + it->second.off;
+        j.cb_bits = r.cb_bits;
+        j.cb_crc = r.cb_crc;
+        j.tb_bytes = r.tb_bytes;
+        j.tb_crc = dv_tbcrc + k;
+        j.cb_iterations = r.cb_iterations;
+        j.num_parity_nodes = r.num_parity_nodes;
+        j.max_iterations = r.max_iterations;
+        j.flags = (uint16_t)((r.flags & CUPHY_LDPC_DECODE_EARLY_TERM) | CUPHY_HIP_SLOT_TB_NOTIFY);
+        j.reserved = 0;
+        j.llr_scale = r.llr_scale;
+        j.norm = r.norm;
+        j.clamp_value = r.clamp_value;
+        j.share_offset = (uint32_t)k;  // accumulator pair of slot k
+        h_tbcrc[k] = CRC_PENDING;
+        for (uint32_t c = 0; c < C; ++c) h_items[(published + c) & (ring_size - 1)] = ((uint32_t)k << 16) | c;
+        published += C;
+        items_open += C;
+        slot_req[k] = q;
+        slot_open[k] = 1;
+        slot_t_publish[k] = -1.0;
+        ++n_open_slots;
+    }
+    pending.swap(rest);
+    if (published == first) return CUPHY_STATUS_SUCCESS;
+    const double t = now_us();
+    for (int k = 0; k < (int)slot_open.size(); ++k)
+        if (slot_open[k] && slot_t_publish[k] < 0.0) slot_t_publish[k] = t;
+    // descriptors and items before the tail that publishes them
+    __atomic_store_n(&h_ring->tail, published, __ATOMIC_RELEASE);
+    if (!r_running) return start_resident();
+    return CUPHY_STATUS_SUCCESS;
+}
+
+// Complete every TB whose CRC the kernel has written (HARQ buffers of
+// decoded or abandoned TBs back to the pool), and free every slot whose
+// launches have ended.
 cuphyStatus_t cuphyHipHarqServer::retire()
 {
     for (auto& s : slots) {
         if (!s.busy) continue;
-        const hipError_t q = hipEventQuery(s.done);
-        if (q == hipErrorNotReady) continue;
-        if (q != hipSuccess) return CUPHY_STATUS_INTERNAL_ERROR;
+        bool ended = true;
+        for (int k = 0; k < 2; ++k) {
+            const hipError_t q = hipEventQuery(s.done[k]);
+            if (q == hipErrorNotReady) {
+                ended = false;
+                continue;
+            }
+            if (q != hipSuccess) return CUPHY_STATUS_INTERNAL_ERROR;
+        }
         const double t = now_us();
+        for (size_t i = 0; i < s.reqs.size() && s.n_open > 0; ++i) {
+            if (s.taken[i]) continue;
+            const uint32_t crc = __atomic_load_n(&s.h_crc[i], __ATOMIC_ACQUIRE);
+            if (crc == CRC_PENDING && !ended) continue;
+            int32_t status = CUPHY_STATUS_SUCCESS;
+            if (crc == CRC_PENDING)
+                status = CUPHY_STATUS_INTERNAL_ERROR;  // launches ended without this TB's result
+            else if (crc == CB_REJECTED)
+                status = CUPHY_STATUS_UNSUPPORTED_CONFIG;
+            complete(s.reqs[i], crc, status, s.t_launch, t);
+            s.taken[i] = 1;
+            --s.n_open;
+        }
+        if (!ended) continue;
+        // HARQ buffers: another launch may use them only once this one has
+        // ended (its writes are then out of every L2)
         for (size_t i = 0; i < s.reqs.size(); ++i) {
             const queued& r = s.reqs[i];
             const uint32_t crc = s.h_crc[i];
             auto it = procs.find(r.req.harq_id);
             if (it != procs.end()) it->second.busy = false;
-            if (crc == 0 || crc == CB_REJECTED || (r.req.flags & CUPHY_HIP_HARQ_LAST_TX)) free_proc(r.req.harq_id);
-            complete(r, crc, crc == CB_REJECTED ? CUPHY_STATUS_UNSUPPORTED_CONFIG : CUPHY_STATUS_SUCCESS,
-                     s.t_launch, t);
+            if (crc == 0 || crc == CB_REJECTED || crc == CRC_PENDING || (r.req.flags & CUPHY_HIP_HARQ_LAST_TX))
+                free_proc(r.req.harq_id);
         }
         s.reqs.clear();
+        s.taken.clear();
+        s.n_open = 0;
         s.busy = false;
     }
     return CUPHY_STATUS_SUCCESS;
@@ -317,11 +552,12 @@ cuphyStatus_t cuphyHipHarqServer::launch(launch_slot& s)
         j.cb_bits = r.cb_bits;
         j.cb_crc = r.cb_crc;
         j.tb_bytes = r.tb_bytes;
-        j.tb_crc = s.d_crc + t;
+        s.h_crc[t] = CRC_PENDING;
+        j.tb_crc = s.dv_crc + t;
         j.cb_iterations = r.cb_iterations;
         j.num_parity_nodes = r.num_parity_nodes;
         j.max_iterations = r.max_iterations;
-        j.flags = (uint16_t)(r.flags & CUPHY_LDPC_DECODE_EARLY_TERM);
+        j.flags = (uint16_t)((r.flags & CUPHY_LDPC_DECODE_EARLY_TERM) | CUPHY_HIP_SLOT_TB_NOTIFY);
         j.reserved = 0;
         j.llr_scale = r.llr_scale;
         j.norm = r.norm;
@@ -356,21 +592,23 @@ cuphyStatus_t cuphyHipHarqServer::launch(launch_slot& s)
     for (auto& c : small) h_items[k++] = c.id;
     for (auto& c : big) h_items[k++] = c.id;
     const size_t bytes = items_off() + k * sizeof(uint32_t);
-    if (hipMemcpyAsync(s.d_stage, s.h_stage, bytes, hipMemcpyHostToDevice, s.stream) != hipSuccess)
+    if (hipMemcpyAsync(s.d_stage, s.h_stage, bytes, hipMemcpyHostToDevice, s.stream[0]) != hipSuccess ||
+        hipEventRecord(s.staged, s.stream[0]) != hipSuccess || hipStreamWaitEvent(s.stream[1], s.staged, 0) != hipSuccess)
         return CUPHY_STATUS_INTERNAL_ERROR;
     const cuphyHipPuschSlotTb_t* d_jobs = reinterpret_cast<const cuphyHipPuschSlotTb_t*>(s.d_stage + jobs_off());
     const uint32_t* d_items = reinterpret_cast<const uint32_t*>(s.d_stage + items_off());
     cuphyStatus_t st = CUPHY_STATUS_SUCCESS;
     if (!small.empty())
         st = nrldpc::slot_decode(d_jobs, n, d_items, (int)small.size(), cfg.llr_type, lds_small, mb_small, s.d_ws[0],
-                                 cu_count, s.stream);
+                                 cu_count, s.stream[0]);
     if (st == CUPHY_STATUS_SUCCESS && !big.empty())
         st = nrldpc::slot_decode(d_jobs, n, d_items + small.size(), (int)big.size(), cfg.llr_type, lds_big, mb_big,
-                                 s.d_ws[1], cu_count, s.stream);
+                                 s.d_ws[1], cu_count, s.stream[1]);
     if (st != CUPHY_STATUS_SUCCESS) return st;
-    if (hipMemcpyAsync(s.h_crc, s.d_crc, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost, s.stream) != hipSuccess)
-        return CUPHY_STATUS_INTERNAL_ERROR;
-    if (hipEventRecord(s.done, s.stream) != hipSuccess) return CUPHY_STATUS_INTERNAL_ERROR;
+    for (int q = 0; q < 2; ++q)
+        if (hipEventRecord(s.done[q], s.stream[q]) != hipSuccess) return CUPHY_STATUS_INTERNAL_ERROR;
+    s.taken.assign((size_t)n, 0);
+    s.n_open = n;
     s.t_launch = now_us();
     s.busy = true;
     ++launches;
@@ -384,8 +622,11 @@ cuphyStatus_t CUPHYWINAPI cuphyHipHarqServerCreate(cuphyLDPCDecoder_t decoder, c
 {
     if (!decoder || !cfg || !server) return CUPHY_STATUS_INVALID_ARGUMENT;
     if (cfg->max_inflight < 1 || cfg->max_inflight > 8 || cfg->max_tbs_per_launch < 1 ||
-        cfg->max_tbs_per_launch > 65536 || cfg->max_cbs_per_launch < 1 || cfg->harq_pool_bytes == 0)
+        cfg->max_tbs_per_launch > 65536 || cfg->max_cbs_per_launch < 1 || cfg->harq_pool_bytes == 0 ||
+        (cfg->flags & ~CUPHY_HIP_HARQ_SERVER_RESIDENT) != 0)
         return CUPHY_STATUS_INVALID_ARGUMENT;
+    if ((cfg->flags & CUPHY_HIP_HARQ_SERVER_RESIDENT) && cfg->max_tbs_per_launch > 65535)
+        return CUPHY_STATUS_INVALID_ARGUMENT;  // descriptor slot in 16 bits of an item
     if (cfg->llr_type != CUPHY_R_8I && cfg->llr_type != CUPHY_R_16F && cfg->llr_type != CUPHY_R_32F)
         return CUPHY_STATUS_UNSUPPORTED_TYPE;
     cuphyHipHarqServer* s = new (std::nothrow) cuphyHipHarqServer();
@@ -400,16 +641,54 @@ cuphyStatus_t CUPHYWINAPI cuphyHipHarqServerCreate(cuphyLDPCDecoder_t decoder, c
     s->stage_bytes = s->items_off() + (size_t)cfg->max_cbs_per_launch * sizeof(uint32_t);
     bool ok = hipMalloc(&s->d_pool, cfg->harq_pool_bytes) == hipSuccess;
     s->pool.init(cfg->harq_pool_bytes);
-    s->slots.resize(cfg->max_inflight);
+    s->resident = (cfg->flags & CUPHY_HIP_HARQ_SERVER_RESIDENT) != 0;
+    if (s->resident) {
+        const int ns = cfg->max_tbs_per_launch;
+        uint32_t rs = 4096;
+        while (rs < 4u * (uint32_t)cfg->max_cbs_per_launch && rs < (1u << 30)) rs <<= 1;  // slack: no slot is rewritten before it is read
+        s->ring_size = rs;
+        auto host = [&](void** h, void** dv, size_t bytes) {
+            return hipHostMalloc(h, bytes, hipHostMallocMapped | hipHostMallocCoherent) == hipSuccess &&
+                   (std::memset(*h, 0, bytes), hipHostGetDevicePointer(dv, *h, 0) == hipSuccess);
+        };
+        int khz = 0;
+        s->r_grid = nrldpc::serve_resident_grid(s->cu_count);
+        ok = ok && s->r_grid > 0 && host((void**)&s->h_ring, (void**)&s->dv_ring, sizeof(nrldpc::serve_ring_hdr)) &&
+             hipMalloc((void**)&s->d_abandoned, (size_t)s->r_grid * sizeof(uint32_t)) == hipSuccess &&
+             hipMemset(s->d_abandoned, 0xFF, (size_t)s->r_grid * sizeof(uint32_t)) == hipSuccess &&
+             host((void**)&s->h_items, (void**)&s->dv_items, (size_t)rs * sizeof(uint32_t)) &&
+             host((void**)&s->h_jobs, (void**)&s->dv_jobs, (size_t)ns * sizeof(cuphyHipPuschSlotTb_t)) &&
+             host((void**)&s->h_params, (void**)&s->dv_params, (size_t)ns * sizeof(PerTbParams)) &&
+             host((void**)&s->h_tbcrc, (void**)&s->dv_tbcrc, (size_t)ns * sizeof(uint32_t)) &&
+             hipMalloc((void**)&s->d_state, sizeof(nrldpc::serve_state)) == hipSuccess &&
+             hipMemset(s->d_state, 0, sizeof(nrldpc::serve_state)) == hipSuccess &&
+             hipMalloc((void**)&s->d_tbw, (size_t)ns * 2 * sizeof(uint32_t)) == hipSuccess &&
+             hipMemset(s->d_tbw, 0, (size_t)ns * 2 * sizeof(uint32_t)) == hipSuccess &&
+             hipStreamCreateWithFlags(&s->r_stream, hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&s->r_done, hipEventDisableTiming) == hipSuccess &&
+             hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, s->device) == hipSuccess && khz > 0 &&
+             hipDeviceSynchronize() == hipSuccess;
+        // a resident workgroup leaves after 20 ms without work, and never
+        // runs longer than 60 s
+        s->idle_ticks = (uint64_t)khz * 20u;
+        s->life_ticks = (uint64_t)khz * 60000u;
+        s->slot_req.resize(ns);
+        s->slot_open.assign(ns, 0);
+        s->slot_t_publish.assign(ns, -1.0);
+        for (int k = ns - 1; k >= 0; --k) s->free_tb_slots.push_back(k);
+    }
+    s->slots.resize(s->resident ? 0 : cfg->max_inflight);
     for (auto& sl : s->slots) {
         if (!ok) break;
-        ok = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) == hipSuccess &&
-             hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) == hipSuccess &&
+        for (int k = 0; k < 2 && ok; ++k)
+            ok = hipStreamCreateWithFlags(&sl.stream[k], hipStreamNonBlocking) == hipSuccess &&
+                 hipEventCreateWithFlags(&sl.done[k], hipEventDisableTiming) == hipSuccess;
+        ok = ok && hipEventCreateWithFlags(&sl.staged, hipEventDisableTiming) == hipSuccess &&
              hipHostMalloc(reinterpret_cast<void**>(&sl.h_stage), s->stage_bytes, hipHostMallocDefault) == hipSuccess &&
-             hipHostMalloc(reinterpret_cast<void**>(&sl.h_crc), (size_t)cfg->max_tbs_per_launch * 4, hipHostMallocDefault) ==
-                 hipSuccess &&
-             hipMalloc(reinterpret_cast<void**>(&sl.d_stage), s->stage_bytes) == hipSuccess &&
-             hipMalloc(reinterpret_cast<void**>(&sl.d_crc), (size_t)cfg->max_tbs_per_launch * 4) == hipSuccess;
+             hipHostMalloc(reinterpret_cast<void**>(&sl.h_crc), (size_t)cfg->max_tbs_per_launch * 4,
+                           hipHostMallocMapped | hipHostMallocCoherent) == hipSuccess &&
+             hipHostGetDevicePointer(reinterpret_cast<void**>(&sl.dv_crc), sl.h_crc, 0) == hipSuccess &&
+             hipMalloc(reinterpret_cast<void**>(&sl.d_stage), s->stage_bytes) == hipSuccess;
         for (auto& w : sl.d_ws)
             ok = ok && hipMalloc(reinterpret_cast<void**>(&w), ws) == hipSuccess && hipMemset(w, 0, ws) == hipSuccess;
     }
@@ -456,8 +735,8 @@ cuphyStatus_t CUPHYWINAPI cuphyHipHarqServerPoll(cuphyHipHarqServer_t server, cu
                                                  int max_out, int* n_out)
 {
     if (!server || !n_out || max_out < 0 || (max_out > 0 && !out)) return CUPHY_STATUS_INVALID_ARGUMENT;
-    cuphyStatus_t st = server->retire();
-    if (st == CUPHY_STATUS_SUCCESS) st = server->admit_and_launch();
+    cuphyStatus_t st = server->resident ? server->retire_resident() : server->retire();
+    if (st == CUPHY_STATUS_SUCCESS) st = server->resident ? server->admit_resident() : server->admit_and_launch();
     int n = 0;
     while (n < max_out && !server->ready.empty()) {
         out[n++] = server->ready.front();
@@ -541,6 +820,11 @@ cuphyStatus_t CUPHYWINAPI cuphyHipHarqServeTrace(cuphyHipHarqServer_t server, co
             const int nx = trace[i].next;
             if (e.status == CUPHY_STATUS_SUCCESS && e.tb_crc != 0 && nx >= 0) due.emplace(e.t_done_us + harq_rtt_us, nx);
         }
+    }
+    // let the last launches end: their HARQ buffers go back to the pool then
+    while (!server->idle()) {
+        const cuphyStatus_t st = server->resident ? server->retire_resident() : server->retire();
+        if (st != CUPHY_STATUS_SUCCESS) return st;
     }
     *n_done = got;
     return CUPHY_STATUS_SUCCESS;

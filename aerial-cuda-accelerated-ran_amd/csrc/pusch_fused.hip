@@ -23,10 +23,16 @@
 //
 // The TB CRC24A is assembled from per-CB shares (CRC linearity): each CB's
 // workgroup computes the remainder of its payload bytes shifted to their
-// place in the TB and stores it in the workspace; a small kernel after the
-// persistent one XORs each TB's shares (no cross-workgroup synchronisation
-// inside the persistent kernel).  The queue is returned to 0 by the last
-// workgroup to leave, so the two launches are replayable from a graph.
+// place in the TB, XORs it into the TB's accumulator word with a device
+// atomic and then counts the CB done on the TB's counter word; the
+// workgroup that finishes the TB's last CB takes (and zeroes) the
+// accumulator, zeroes the counter and writes the TB CRC.  No workgroup ever
+// waits for another.  With CUPHY_HIP_SLOT_TB_NOTIFY each CB's outputs are
+// first written back from the XCD's L2 (agent-scope release), so the TB CRC
+// appears only after every output of the TB is in memory: a host polling
+// the TB CRC in host-mapped memory can take each TB as soon as its own code
+// blocks are decoded.  The queue is returned to 0 by the last workgroup to
+// leave, so the launch replays from a graph.
 #include "crc_core.hpp"
 #include "derate_core.hpp"
 #include "ldpc_core.hpp"
@@ -199,10 +205,11 @@ __device__ __forceinline__ void combine_symbols(_Float16* __restrict__ cbuf, con
     __syncthreads();
 }
 
-// One CB.  Returns this CB's share of the TB CRC24A (the remainder of its
-// payload bytes shifted to their place in the TB), or TB_ACC_ERROR when
-// the CB's parameters do not fit the launch (then only the error marker is
-// written, into its CRC and into the TB CRC of a one-CB TB).
+// One CB.  Returns, in every thread, this CB's share of the TB CRC24A (the
+// remainder of its payload bytes shifted to their place in the TB; for a
+// one-CB TB its CRC, which is the TB CRC), or TB_ACC_ERROR when the CB's
+// parameters do not fit the launch (then only the error marker is written,
+// into its CRC).  The TB CRC itself is written by complete_cb.
 template <typename TI, int MAXR>
 __device__ __forceinline__ uint32_t fused_cb(_Float16* __restrict__ hb, fused_lds& sh,
                                              const cuphyHipPuschSlotTb_t& job, const PerTbParams& p, int cb_in_tb,
@@ -224,10 +231,7 @@ __device__ __forceinline__ uint32_t fused_cb(_Float16* __restrict__ hb, fused_ld
                     mb <= (bg == 1 ? 46 : 42) && mb <= MAXR && nh <= lds_values && nval <= lds_values &&
                     Kb * Zc <= 32 * (int)SLOT_CB_WORDS && g.L > 0 && g.E > 0;
     if (!ok) {
-        if (threadIdx.x == 0) {
-            job.cb_crc[r] = FUSED_CB_ERROR;
-            if (C == 1) *job.tb_crc = FUSED_CB_ERROR;
-        }
+        if (threadIdx.x == 0) job.cb_crc[r] = FUSED_CB_ERROR;
         return TB_ACC_ERROR;
     }
     _Float16* harq = static_cast<_Float16*>(job.harq) + (size_t)r * ncb_padded;
@@ -334,12 +338,11 @@ __device__ __forceinline__ uint32_t fused_cb(_Float16* __restrict__ hb, fused_ld
     if (tid == 0) {
         job.cb_crc[r] = crc;
         if (cb_in_tb == 0) {
-            if (C == 1) *job.tb_crc = crc;
             const int bytes = C * data_bytes;
             for (int b = bytes; b < (bytes + 3) / 4 * 4; ++b) job.tb_bytes[b] = 0;
         }
     }
-    if (C == 1) return 0;
+    if (C == 1) return crc;
     // ---- 6. this CB's share of the TB CRC24A --------------------------------
     // TB CRC = XOR over CBs of crc(payload bytes of CB c) * x^(8 * TB bytes
     // after them), the TB zero-padded to whole words (CRC linearity)
@@ -351,36 +354,43 @@ __device__ __forceinline__ uint32_t fused_cb(_Float16* __restrict__ hb, fused_ld
     return part;
 }
 
-// TB CRCs of a launch's multi-CB TBs from the per-CB shares in the
-// workspace (one thread per TB, found as its item with CB index 0): XOR of
-// the shares, or the rejection marker if a CB was rejected.
-__global__ __launch_bounds__(256) void pusch_tb_crc_finish(const cuphyHipPuschSlotTb_t* __restrict__ tbs, int n_tbs,
-                                                           const uint32_t* __restrict__ items, int n_items,
-                                                           const uint32_t* __restrict__ shares)
+// Workspace words of a TB (after the queue words): TB-CRC accumulator at
+// 2 * share_offset, count of finished CBs at 2 * share_offset + 1.  Called
+// by one thread once the whole workgroup has finished CB `cb` (all of its
+// outputs issued).  v: the CB's share (TB_ACC_ERROR: rejected CB).
+__device__ __forceinline__ void complete_cb(const cuphyHipPuschSlotTb_t& job, uint32_t C, uint32_t v,
+                                            uint32_t* __restrict__ tbw)
 {
-    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i >= n_items) return;
-    const uint32_t it = items[i];
-    const int t = (int)(it >> 16);
-    if ((it & 0xFFFFu) != 0 || t >= n_tbs) return;
-    const cuphyHipPuschSlotTb_t& job = tbs[t];
-    const uint32_t C = job.prm->num_CBs;
-    if (C <= 1) return;
-    uint32_t acc = 0;
-    bool rejected = false;
-    for (uint32_t c = 0; c < C; ++c) {
-        const uint32_t v = shares[job.share_offset + c];
-        rejected |= v == TB_ACC_ERROR;
-        acc ^= v;
+    const bool notify = (job.flags & CUPHY_HIP_SLOT_TB_NOTIFY) != 0;
+    // every output of this CB out of the XCD's L2 before it is counted
+    if (notify) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    uint32_t tb_crc;
+    if (C == 1) {
+        tb_crc = (v == TB_ACC_ERROR) ? FUSED_CB_ERROR : v;
+    } else {
+        uint32_t* acc = tbw + 2u * job.share_offset;
+        uint32_t prev = (v == TB_ACC_ERROR)
+                            ? __hip_atomic_fetch_or(acc, 0x80000000u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                            : __hip_atomic_fetch_xor(acc, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // the share is in the accumulator (its old value returned) before
+        // the count is raised
+        asm volatile("" : : "v"(prev) : "memory");
+        if (__hip_atomic_fetch_add(acc + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != C - 1u) return;
+        const uint32_t a = __hip_atomic_exchange(acc, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(acc + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        tb_crc = (a & 0x80000000u) ? FUSED_CB_ERROR : a;
     }
-    *job.tb_crc = rejected ? FUSED_CB_ERROR : acc;
+    if (notify)
+        __hip_atomic_store(job.tb_crc, tb_crc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    else
+        *job.tb_crc = tb_crc;
 }
 
 // Persistent fused slot kernel.  Workspace: queue state (2 words used of
-// FUSED_WS_QUEUE_WORDS), then one TB-CRC share per CB at the TB's
-// share_offset + CB index, combined by pusch_tb_crc_finish after the
-// launch.  The next item index is fetched while the current CB is worked
-// on.  The queue is returned to 0 by the last workgroup to leave.
+// FUSED_WS_QUEUE_WORDS), then two words per CB slot of the launch, of which
+// each TB uses the pair at its first slot (TB-CRC accumulator and count of
+// finished CBs; see complete_cb).  The next item index is fetched while the current CB is
+// worked on.  The queue is returned to 0 by the last workgroup to leave.
 template <typename TI, int MAXR>
 __global__ __launch_bounds__(384) void pusch_slot_fused(const cuphyHipPuschSlotTb_t* __restrict__ tbs, int n_tbs,
                                                         const uint32_t* __restrict__ items, int n_items,
@@ -390,7 +400,7 @@ __global__ __launch_bounds__(384) void pusch_slot_fused(const cuphyHipPuschSlotT
     __shared__ fused_lds sh;
     _Float16* hb = reinterpret_cast<_Float16*>(lds_raw);
     uint32_t* queue = ws;
-    uint32_t* shares = ws + FUSED_WS_QUEUE_WORDS;
+    uint32_t* tbw = ws + FUSED_WS_QUEUE_WORDS;
     for (int i = threadIdx.x; i < 3 * 256; i += blockDim.x) (&sh.crc_t[0][0])[i] = (&g_crc_tables.byte[0][0])[i];
     for (int i = threadIdx.x; i < 3 * CRC_SHIFT_LO; i += blockDim.x) (&sh.shift[0][0])[i] = (&g_crc_shift.lo[0][0])[i];
     if (threadIdx.x == 0) sh.item = (int)atomicAdd(&queue[0], 1u);
@@ -407,7 +417,8 @@ __global__ __launch_bounds__(384) void pusch_slot_fused(const cuphyHipPuschSlotT
             const uint32_t C = (uint32_t)__builtin_amdgcn_readfirstlane((int)p.num_CBs);
             if ((uint32_t)cb < C) {
                 const uint32_t part = fused_cb<TI, MAXR>(hb, sh, job, p, cb, lds_values);
-                if (C > 1 && threadIdx.x == 0) shares[job.share_offset + (uint32_t)cb] = part;
+                __syncthreads();  // every output of the CB issued by every thread
+                if (threadIdx.x == 0) complete_cb(job, C, part, tbw);
             }
         }
         __syncthreads();
@@ -421,6 +432,109 @@ __global__ __launch_bounds__(384) void pusch_slot_fused(const cuphyHipPuschSlotT
             atomicExch(&queue[0], 0u);
             atomicExch(&queue[1], 0u);
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Resident serving kernel: one persistent launch that takes code blocks from
+// a ring in host-mapped memory as the host publishes them (no launch per
+// batch).  Work item i is ring items[i & mask] = (TB slot << 16) | CB; the
+// TB's job descriptor (and, through it, its PerTbParams) is in host-mapped
+// memory too and is copied to LDS per CB.  A workgroup takes a ticket (the
+// next item index: one device atomic add, no retries) and waits until the
+// host has published that item; one waiting workgroup at a time refreshes
+// the device copy of the host's tail.  Every wait is bounded: a workgroup
+// leaves after idle_ticks of the wall clock without work, when the host
+// sets `stop`, or after life_ticks in all, and then leaves its ticket in
+// abandoned[workgroup]; the same workgroup of the next launch (same grid)
+// starts with it, so no item is lost or taken twice.  The host launches the
+// kernel again whenever work is open and none is running.  Each CB starts
+// with an agent-scope acquire (inputs and HARQ buffers written since may
+// sit behind stale lines of this XCD's L2) and ends with the TB-CRC
+// protocol of complete_cb in notify mode.
+// ---------------------------------------------------------------------------
+constexpr uint32_t NO_TICKET = 0xFFFFFFFFu;
+
+__device__ __forceinline__ int serve_next(const serve_ring_hdr* ring, serve_state* st, uint32_t* abandoned,
+                                          bool first, uint64_t t_start, uint64_t idle_ticks, uint64_t life_ticks)
+{
+    uint32_t i = NO_TICKET;
+    if (first) {
+        i = __hip_atomic_exchange(&abandoned[blockIdx.x], NO_TICKET, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (i == NO_TICKET) i = __hip_atomic_fetch_add(&st->head, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t t_wait = wall_clock64();
+    for (;;) {
+        const uint32_t t = __hip_atomic_load(&st->tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((int32_t)(t - i) > 0) return (int)(i & 0x7FFFFFFFu);
+        // not published yet: one workgroup at a time reads the host's tail
+        if (__hip_atomic_exchange(&st->token, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
+            const uint32_t ht = __hip_atomic_load(&ring->tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            const uint32_t hs = __hip_atomic_load(&ring->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            const uint32_t cur = __hip_atomic_load(&st->tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if ((int32_t)(ht - cur) > 0) __hip_atomic_store(&st->tail, ht, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&st->stop, hs ? 1u : 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&st->token, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if ((int32_t)(ht - i) > 0) return (int)(i & 0x7FFFFFFFu);
+        }
+        const uint64_t now = wall_clock64();
+        if (__hip_atomic_load(&st->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) || now - t_wait > idle_ticks ||
+            now - t_start > life_ticks) {
+            __hip_atomic_store(&abandoned[blockIdx.x], i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return -1;
+        }
+        __builtin_amdgcn_s_sleep(32);
+    }
+}
+
+template <typename TI>
+__global__ __launch_bounds__(384) void pusch_serve_resident(const serve_ring_hdr* ring,
+                                                            const uint32_t* __restrict__ ring_items, uint32_t ring_mask,
+                                                            const cuphyHipPuschSlotTb_t* jobs, int n_slots,
+                                                            serve_state* st, uint32_t* abandoned,
+                                                            uint32_t* __restrict__ tbw, int lds_values,
+                                                            uint64_t idle_ticks, uint64_t life_ticks)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __shared__ fused_lds sh;
+    __shared__ cuphyHipPuschSlotTb_t s_job;
+    __shared__ PerTbParams s_prm;
+    _Float16* hb = reinterpret_cast<_Float16*>(lds_raw);
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    for (int i = tid; i < 3 * 256; i += nthr) (&sh.crc_t[0][0])[i] = (&g_crc_tables.byte[0][0])[i];
+    for (int i = tid; i < 3 * CRC_SHIFT_LO; i += nthr) (&sh.shift[0][0])[i] = (&g_crc_shift.lo[0][0])[i];
+    const uint64_t t_start = wall_clock64();
+    bool first = true;
+    for (;;) {
+        if (tid == 0) sh.item = serve_next(ring, st, abandoned, first, t_start, idle_ticks, life_ticks);
+        first = false;
+        __syncthreads();
+        const int i = sh.item;
+        if (i < 0) break;
+        const uint32_t it = __hip_atomic_load(&ring_items[(uint32_t)i & ring_mask], __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_SYSTEM);
+        const int t = (int)(it >> 16), cb = (int)(it & 0xFFFFu);
+        if (t < n_slots) {
+            // the TB's descriptor and parameters from host memory into LDS
+            const uint32_t* js = reinterpret_cast<const uint32_t*>(&jobs[t]);
+            for (int k = tid; k < (int)(sizeof(cuphyHipPuschSlotTb_t) / 4); k += nthr)
+                reinterpret_cast<uint32_t*>(&s_job)[k] =
+                    __hip_atomic_load(js + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __syncthreads();
+            const uint32_t* ps = reinterpret_cast<const uint32_t*>(s_job.prm);
+            for (int k = tid; k < (int)(sizeof(PerTbParams) / 4); k += nthr)
+                reinterpret_cast<uint32_t*>(&s_prm)[k] =
+                    __hip_atomic_load(ps + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            if (tid == 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            __syncthreads();
+            const uint32_t C = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_prm.num_CBs);
+            if ((uint32_t)cb < C) {
+                const uint32_t part = fused_cb<TI, 46>(hb, sh, s_job, s_prm, cb, lds_values);
+                __syncthreads();  // every output of the CB issued by every thread
+                if (tid == 0) complete_cb(s_job, C, part, tbw);
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -444,6 +558,40 @@ namespace nrldpc {
 
 cuphyStatus_t ensure_lds_attr(void* func, size_t bytes);  // ldpc_decode.hip
 
+int serve_resident_grid(int cu_count)
+{
+    void* func = (void*)&pusch_serve_resident<_Float16>;
+    const size_t lds = ((size_t)68 * 384 * 2 + 15) / 16 * 16 + 64u;
+    if (ensure_lds_attr(func, lds) != CUPHY_STATUS_SUCCESS) return 0;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, func, 384, lds) != hipSuccess || per_cu < 1) per_cu = 1;
+    return std::max(1, cu_count) * per_cu;
+}
+
+cuphyStatus_t serve_resident_launch(const serve_ring_hdr* ring, const uint32_t* ring_items, uint32_t ring_mask,
+                                    const cuphyHipPuschSlotTb_t* jobs, int n_slots, serve_state* st, uint32_t* tbw,
+                                    uint32_t* abandoned, int grid, cuphyDataType_t llr_type, uint64_t idle_ticks,
+                                    uint64_t life_ticks, hipStream_t strm)
+{
+    void* func = nullptr;
+    switch (llr_type) {
+    case CUPHY_R_8I: func = (void*)&pusch_serve_resident<int8_t>; break;
+    case CUPHY_R_16F: func = (void*)&pusch_serve_resident<_Float16>; break;
+    case CUPHY_R_32F: func = (void*)&pusch_serve_resident<float>; break;
+    default: return CUPHY_STATUS_UNSUPPORTED_TYPE;
+    }
+    int lds_values = 68 * 384;  // any TB
+    const size_t lds = ((size_t)lds_values * 2 + 15) / 16 * 16 + 64u;
+    cuphyStatus_t s = ensure_lds_attr(func, lds);
+    if (s != CUPHY_STATUS_SUCCESS) return s;
+    void* args[11] = {const_cast<serve_ring_hdr**>(&ring), const_cast<uint32_t**>(&ring_items), &ring_mask,
+                      const_cast<cuphyHipPuschSlotTb_t**>(&jobs), &n_slots, &st, &abandoned, &tbw, &lds_values,
+                      &idle_ticks, &life_ticks};
+    if (hipLaunchKernel(func, dim3((unsigned)grid), dim3(384), args, lds, strm) != hipSuccess)
+        return CUPHY_STATUS_INTERNAL_ERROR;
+    return CUPHY_STATUS_SUCCESS;
+}
+
 cuphyStatus_t slot_decode(const cuphyHipPuschSlotTb_t* d_tbs, int n_tbs, const uint32_t* d_items, int n_items,
                           cuphyDataType_t llr_type, int max_lds_values, int max_parity_nodes, void* d_ws,
                           int cu_count, hipStream_t strm)
@@ -465,12 +613,6 @@ cuphyStatus_t slot_decode(const cuphyHipPuschSlotTb_t* d_tbs, int n_tbs, const u
     void* args[6] = {const_cast<cuphyHipPuschSlotTb_t**>(&d_tbs), &n_tbs, const_cast<uint32_t**>(&d_items),
                      &n_items, &max_lds_values, &ws};
     if (hipLaunchKernel(func, dim3((unsigned)grid), dim3(384), args, lds, strm) != hipSuccess)
-        return CUPHY_STATUS_INTERNAL_ERROR;
-    const uint32_t* shares = ws + FUSED_WS_QUEUE_WORDS;
-    void* fargs[5] = {const_cast<cuphyHipPuschSlotTb_t**>(&d_tbs), &n_tbs, const_cast<uint32_t**>(&d_items), &n_items,
-                      const_cast<uint32_t**>(&shares)};
-    if (hipLaunchKernel((void*)&pusch_tb_crc_finish, dim3((unsigned)((n_items + 255) / 256)), dim3(256), fargs, 0,
-                        strm) != hipSuccess)
         return CUPHY_STATUS_INTERNAL_ERROR;
     return CUPHY_STATUS_SUCCESS;
 }

@@ -8,8 +8,37 @@
 namespace nrldpc {
 
 // Workspace: the queue state (first FUSED_WS_QUEUE_WORDS words, of which 2
-// are used) followed by the TB-CRC share slots (one per CB).
+// are used) followed by two words per CB slot, of which each TB uses the
+// pair at its first slot: TB-CRC accumulator and count of finished CBs.
 constexpr int FUSED_WS_QUEUE_WORDS = 64;
+
+// Resident serving launch (pusch_serve_resident): the host-written ring
+// header (host-mapped pinned memory; items follow in their own array) and
+// the device-side claim state (device memory, zero before the first launch
+// and kept across launches).
+struct serve_ring_hdr {
+    uint32_t tail;  // items published (host: release store)
+    uint32_t stop;  // host: 1 = leave once no work is left
+    uint32_t pad[62];
+};
+struct serve_state {       // one 256-byte line per word: tickets do not contend with polls
+    uint32_t head;         // next ticket (item index)
+    uint32_t pad0[63];
+    uint32_t tail;         // last host tail seen
+    uint32_t pad1[63];
+    uint32_t token;        // a workgroup is reading the host tail
+    uint32_t stop;         // host stop seen
+    uint32_t pad2[62];
+};
+
+// Workgroups of a resident launch (the same for every launch of a server:
+// abandoned tickets are handed over by workgroup index); 0 on failure.
+int serve_resident_grid(int cu_count);
+// abandoned: `grid` words, all 0xFFFFFFFF before the first launch.
+cuphyStatus_t serve_resident_launch(const serve_ring_hdr* ring, const uint32_t* ring_items, uint32_t ring_mask,
+                                    const cuphyHipPuschSlotTb_t* jobs, int n_slots, serve_state* st, uint32_t* tbw,
+                                    uint32_t* abandoned, int grid, cuphyDataType_t llr_type, uint64_t idle_ticks,
+                                    uint64_t life_ticks, hipStream_t strm);
 
 // Enqueue one persistent fused launch (de-rate-match + HARQ combine +
 // decode + CB/TB CRC) over the CB items of a slot.

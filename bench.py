@@ -120,20 +120,23 @@ def fused_slot_section(dec, steps):
             "hbm_gbs": round(hbm / (ms * 1e-3) / 1e9, 1), "hbm_frac": round(hbm / (ms * 1e-3) / 8e12, 4)}
 
 
-def serving_section(dec, serve_ms, config3_gbps=None, inflight=3, loads=(0.6, 0.9, 1.2)):
+def serving_section(dec, serve_ms, config3_gbps=None, inflight=3, loads=(0.6, 0.9, 1.2), resident=True,
+                    templates=None):
     """Serving loop: Poisson TB arrivals from 32 cells (TB shapes as in the
     heterogeneous slot, binary16 BPSK-AWGN LLRs), HARQ with rv 0, 2, 3, 1
     and up to 4 transmissions, retransmissions released one HARQ round trip
     (2 ms) after a failing TB CRC and soft-combined into device-resident
     HARQ buffers (8 GiB pool, admission by that budget), syndrome early
-    termination within 10 iterations, continuous batching over 3 in-flight
-    fused launches.  Saturation capacity first (all TBs at t = 0), then
-    offered loads of 60/90/120% of it; latency = completion seen by the
-    engine - release of the transmission."""
+    termination within 10 iterations.  resident: one resident launch takes
+    code blocks as they are admitted; otherwise continuous batching over
+    `inflight` fused launches.  Saturation capacity first (all TBs at
+    t = 0), then offered loads as fractions of it; latency = completion seen
+    by the engine - release of the transmission."""
     from aerial_amd.server import HarqServer, TraceTemplates, build_trace, poisson_arrivals, serving_specs, trace_report
     n_templates = 256
-    templates = TraceTemplates(serving_specs(n_templates, seed=41), seed=41)
-    srv = HarqServer(dec, pool_bytes=8 << 30, max_inflight=inflight, max_tbs=2048, max_cbs=16384)
+    if templates is None:
+        templates = TraceTemplates(serving_specs(n_templates, seed=41), seed=41)
+    srv = HarqServer(dec, pool_bytes=8 << 30, max_inflight=inflight, max_tbs=2048, max_cbs=16384, resident=resident)
     rtt = 2000.0
     mean_bits = float(np.mean([tb.info_bits for tb in templates.tbs]))
     n_sat = 1500
@@ -156,12 +159,15 @@ def serving_section(dec, serve_ms, config3_gbps=None, inflight=3, loads=(0.6, 0.
             r["offered_vs_config3"] = round(x * cap_tbs * mean_bits / 1e9 / config3_gbps, 4)
         r["engine"] = srv.stats()
         out["loads"].append(r)
-        print("serving load %.1f: %s" % (x, json.dumps(r)), file=sys.stderr, flush=True)
+        print("serving (%s) load %.1f: %s" % ("resident" if resident else "batched", x, json.dumps(r)),
+              file=sys.stderr, flush=True)
     srv.close()
     out["workload"] = ("32 cells, Poisson TB arrivals, TB shapes as the heterogeneous slot (256 templates), "
                        "rv 0,2,3,1 up to 4 transmissions, HARQ RTT 2 ms, fp16 HARQ buffers in an 8 GiB pool, "
-                       "early termination <= 10 iterations, %d in-flight fused launches; %.0f ms per load"
-                       % (inflight, serve_ms))
+                       "early termination <= 10 iterations, %s; %.0f ms per load"
+                       % ("one resident launch taking code blocks as admitted" if resident else
+                          "%d in-flight batched fused launches" % inflight, serve_ms))
+    out["templates"] = templates
     return out
 
 
@@ -353,8 +359,13 @@ def main():
         fused = fused_slot_section(dec, a.steps)
     serving = None
     if "serving" in sections:
-        serving = serving_section(dec, a.serve_ms, config3["batched_gbps"] if config3 else None,
-                                  inflight=a.serve_inflight, loads=[float(x) for x in a.serve_loads.split(",")])
+        c3g = config3["batched_gbps"] if config3 else None
+        serving = serving_section(dec, a.serve_ms, c3g, loads=[float(x) for x in a.serve_loads.split(",")],
+                                  resident=True)
+        batched = serving_section(dec, a.serve_ms, c3g, inflight=a.serve_inflight, loads=(0.6,), resident=False,
+                                  templates=serving.pop("templates"))
+        batched.pop("templates")
+        serving["batched_launches_for_comparison"] = batched
 
     cpu = None if a.no_cpu else cpu_baseline(tbs, a.cpu_seconds)
     return line_of(a, locals())

@@ -418,7 +418,7 @@ typedef struct {
     uint8_t* cb_iterations;          /* optional: iterations run per CB r, NULL for none */
     int16_t num_parity_nodes;        /* mb of the decoder, 4..46 (BG2: ..42) */
     int16_t max_iterations;
-    uint16_t flags;                  /* CUPHY_LDPC_DECODE_EARLY_TERM */
+    uint16_t flags;                  /* CUPHY_LDPC_DECODE_EARLY_TERM | CUPHY_HIP_SLOT_TB_NOTIFY */
     uint16_t reserved;
     float llr_scale;                 /* combined LLR = stored value * llr_scale, rounded to binary16
                                         (int8 LLRs with 4 steps per unit LLR: 0.25) */
@@ -427,18 +427,29 @@ typedef struct {
     uint32_t share_offset;           /* first of the TB's num_CBs TB-CRC share slots in the workspace */
 } cuphyHipPuschSlotTb_t;
 
-/* Workspace of cuphyHipPuschSlotDecode: queue state and num_cbs TB-CRC
- * share slots (TB t uses slots share_offset .. share_offset + num_CBs - 1;
- * the TBs of one launch must not overlap).  Zero-fill once; the queue state
- * returns to zero after every launch (graph-replayable).  One workspace per
- * running launch. */
+/* cuphyHipPuschSlotTb_t.flags: publish the TB CRC as soon as the TB is decoded
+ * (see cuphyHipPuschSlotDecode) */
+#define CUPHY_HIP_SLOT_TB_NOTIFY 0x200u
+
+/* Workspace of cuphyHipPuschSlotDecode: queue state and num_cbs CB slots of
+ * two words (TB t owns slots share_offset .. share_offset + num_CBs - 1 and
+ * uses the pair at its first slot as TB-CRC accumulator and completion
+ * count; the TBs of one launch must not overlap).  Zero-fill once; all of
+ * it returns to zero after every launch (graph-replayable).  One workspace
+ * per running launch. */
 cuphyStatus_t CUPHYWINAPI cuphyHipPuschSlotDecodeGetWorkspaceSize(int num_cbs, size_t* sizeInBytes);
 
 /* Fused de-rate-match + HARQ combine + layered NMS decode (binary16) + CB
  * CRC + TB assembly + TB CRC for num_cb_items code blocks in one persistent
- * launch (followed by a small kernel that combines each TB's CRC shares).
- * d_cb_items[i] = (TB index << 16) | CB index within the TB;
- * every CB of a TB must be in the same launch.  llr_type: CUPHY_R_8I,
+ * launch.  d_cb_items[i] = (TB index << 16) | CB index within the TB;
+ * every CB of a TB must be in the launch exactly once.  A TB's CRC
+ * (*tb_crc) is written as soon as its last CB is decoded.  With
+ * CUPHY_HIP_SLOT_TB_NOTIFY in the TB's flags it is written only after every
+ * other output of the TB has reached device memory: a caller that points
+ * tb_crc into host-mapped pinned memory, pre-set to a value no CRC takes
+ * (e.g. 0xFFFFFFFE), can take each TB as soon as that value changes, before
+ * the launch ends (the TB's HARQ buffer must still not be reused by another
+ * launch before this one ends).  llr_type: CUPHY_R_8I,
  * CUPHY_R_16F or CUPHY_R_32F.  max_lds_values: max over TBs of
  * max(2Zc + Ncb, (nsys + mb) * Zc) (sizes LDS); max_parity_nodes: max mb
  * (<= 8 selects the register-lean high-code-rate variant; a CB with a
@@ -463,12 +474,17 @@ typedef struct cuphyHipHarqServer* cuphyHipHarqServer_t;
 
 #define CUPHY_HIP_HARQ_LAST_TX 0x100u   /* request flag: free the HARQ buffer after this transmission */
 
+/* cuphyHipHarqServerConfig_t.flags: run one resident launch that takes code
+ * blocks as they are admitted (see cuphyHipHarqServerCreate) */
+#define CUPHY_HIP_HARQ_SERVER_RESIDENT 0x1u
+
 typedef struct {
     size_t harq_pool_bytes;          /* device bytes for HARQ soft buffers */
-    int max_inflight;                /* launches in flight at once, 1..8 */
-    int max_tbs_per_launch;          /* 1..65536 */
-    int max_cbs_per_launch;          /* >= 1 */
+    int max_inflight;                /* launches in flight at once, 1..8 (batched mode) */
+    int max_tbs_per_launch;          /* 1..65536; resident mode: TBs in flight at once (<= 65535) */
+    int max_cbs_per_launch;          /* >= 1; resident mode: code blocks in flight at once */
     cuphyDataType_t llr_type;        /* element type of every request's LLRs */
+    uint32_t flags;                  /* 0 (batched launches) or CUPHY_HIP_HARQ_SERVER_RESIDENT */
 } cuphyHipHarqServerConfig_t;
 
 typedef struct {

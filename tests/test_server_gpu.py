@@ -65,10 +65,11 @@ def check_against_alone(dec, templates, done, meta, outs, arrivals, max_iteratio
         assert done[last].tb_crc == int(ref[3].cpu().numpy().view(np.uint32)[0]), (a, ti)
 
 
-def test_results_equal_alone_and_in_the_mix(dec, templates):
+@pytest.mark.parametrize("resident", [False, True])
+def test_results_equal_alone_and_in_the_mix(dec, templates, resident):
     from aerial_amd.server import HarqServer, build_trace, poisson_arrivals
-    print("[case] server: 40 overlapping TBs, rtt 300us", flush=True)
-    srv = HarqServer(dec, pool_bytes=1 << 30, max_inflight=3, max_tbs=64, max_cbs=1024)
+    print("[case] server: 40 overlapping TBs, rtt 300us, resident=%s" % resident, flush=True)
+    srv = HarqServer(dec, pool_bytes=1 << 30, max_inflight=3, max_tbs=64, max_cbs=1024, resident=resident)
     arrivals = poisson_arrivals(n_cells=8, rate_per_s=200000.0, duration_us=600.0, n_templates=len(templates.tbs),
                                 seed=5)
     arrivals = arrivals[:40]
@@ -77,7 +78,7 @@ def test_results_equal_alone_and_in_the_mix(dec, templates):
     done = srv.serve_trace(entries, harq_rtt_us=300.0)
     st = srv.stats()
     assert st["pending"] == 0 and st["pool_bytes_in_use"] == 0, st
-    assert st["launches"] >= 2
+    assert st["launches"] >= (1 if resident else 2)
     check_against_alone(dec, templates, done, meta, outs, arrivals, 8)
     # some TBs needed retransmissions, and overlapping launches happened
     ks = [k for i, (a, k, ti) in enumerate(meta) if done[i].status == 0]
@@ -85,17 +86,52 @@ def test_results_equal_alone_and_in_the_mix(dec, templates):
     srv.close()
 
 
-def test_admission_by_hbm_budget(dec, templates):
+@pytest.mark.parametrize("resident", [False, True])
+def test_tb_completes_before_its_launch_ends(dec, resident):
+    # a one-CB TB and a 24-CB low-SNR TB in the same launch: the small TB's
+    # CRC is written by the kernel as soon as its own CB is decoded, and the
+    # engine completes it while the large TB is still decoding
+    from aerial_amd.server import HarqServer, TraceTemplates
+    from aerial_amd.slot import SlotTb
+    print("[case] server: per-TB completion, BG2 Z=64 1 CB + BG1 Z=384 24 CBs in one launch, resident=%s"
+          % resident, flush=True)
+    specs = [(2, 64, 1, 640, 2, 1, 1280, 8.0), (1, 384, 24, 8448, 2, 1, 42240, -1.0)]
+    tt = TraceTemplates(specs, seed=3)
+    srv = HarqServer(dec, pool_bytes=1 << 30, max_inflight=2, max_tbs=16, max_cbs=256, resident=resident)
+    outs = []
+    for i, tb in enumerate(tt.tbs):
+        p, _ = tb.params(0, 1)
+        o = SlotTb.alloc_outputs(p)
+        outs.append(o)
+        srv.submit(srv.request(100 + i, p, tb.llrs[0], o, tt.mbs[i][0], max_iterations=10, early_term=True,
+                               last=True, tag=i))
+    done = {}
+    while len(done) < 2:
+        for c in srv.poll():
+            done[c.tag] = c
+    st = srv.stats()
+    srv.close()
+    assert st["launches"] == 1, st
+    small, big = done[0], done[1]
+    assert small.status == 0 and big.status == 0
+    assert small.t_launch_us == big.t_launch_us
+    assert small.t_done_us < big.t_done_us, (small.t_done_us, big.t_done_us)
+    # the one-CB TB's CB CRC is its TB CRC
+    assert int(outs[0][1].cpu()[0]) == small.tb_crc == 0
+
+
+@pytest.mark.parametrize("resident", [False, True])
+def test_admission_by_hbm_budget(dec, templates, resident):
     """A pool that holds about two TBs still serves a burst of ten; its peak
     use never exceeds its size and new TBs waited for space."""
     from aerial_amd.server import HarqServer, build_trace
-    print("[case] server: burst of 10 TBs into a small pool", flush=True)
+    print("[case] server: burst of 10 TBs into a small pool, resident=%s" % resident, flush=True)
     sizes = []
     for tb in templates.tbs[:10]:
         p, _ = tb.params(0, 1)
         sizes.append((p.firstCodeBlockIndex + p.num_CBs) * p.Ncb_padded * 2)
     pool = (max(sizes) * 2 + 255) // 256 * 256 + 512
-    srv = HarqServer(dec, pool_bytes=pool, max_inflight=2, max_tbs=64, max_cbs=1024)
+    srv = HarqServer(dec, pool_bytes=pool, max_inflight=2, max_tbs=64, max_cbs=1024, resident=resident)
     arrivals = [(0.0, 0, i) for i in range(10)]
     entries, outs, meta = build_trace(srv, templates, arrivals, max_iterations=8)
     done = srv.serve_trace(entries, harq_rtt_us=50.0)
@@ -107,12 +143,13 @@ def test_admission_by_hbm_budget(dec, templates):
     srv.close()
 
 
-def test_retransmission_waits_for_its_predecessor(dec, templates):
+@pytest.mark.parametrize("resident", [False, True])
+def test_retransmission_waits_for_its_predecessor(dec, templates, resident):
     """Two transmissions of one HARQ process submitted back to back run in
     order (the second combines into the first's buffer)."""
     from aerial_amd.server import HarqServer, RV_SEQUENCE
     from aerial_amd.slot import SlotTb
-    print("[case] server: back-to-back submit of tx 0 and tx 1", flush=True)
+    print("[case] server: back-to-back submit of tx 0 and tx 1, resident=%s" % resident, flush=True)
     # a TB whose first transmission fails (a passing TB's buffer is freed
     # and its retransmission rejected, as the next test checks)
     ti = next((i for i in range(len(templates.tbs))
@@ -120,7 +157,7 @@ def test_retransmission_waits_for_its_predecessor(dec, templates):
     if ti is None:
         pytest.skip("no template fails its first transmission")
     tb = templates.tbs[ti]
-    srv = HarqServer(dec, pool_bytes=1 << 28, max_inflight=3, max_tbs=8, max_cbs=256)
+    srv = HarqServer(dec, pool_bytes=1 << 28, max_inflight=3, max_tbs=8, max_cbs=256, resident=resident)
     p0, _ = tb.params(0, 1)
     outs = SlotTb.alloc_outputs(p0)
     for k in range(2):
@@ -136,6 +173,7 @@ def test_retransmission_waits_for_its_predecessor(dec, templates):
     nw = (tb.Kp + 31) // 32
     assert torch.equal(outs[0][:, :nw], ref[0][:, :nw])
     assert got[1].tb_crc == int(ref[3].cpu().numpy().view(np.uint32)[0])
+    assert srv.poll() == []  # launches ended (synchronized): their buffers are released
     assert srv.stats()["pool_bytes_in_use"] == 0  # LAST_TX freed the buffer
     srv.close()
 

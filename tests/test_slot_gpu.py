@@ -112,6 +112,7 @@ def run_slot(dec, dtype, early_term, specs=SLOT):
         sd.build(tbs).launch()
         torch.cuda.synchronize()
         assert all(int(w[:2].abs().sum().item()) == 0 for w in sd.workspaces), "queue not returned to zero"
+        assert all(int(w.abs().sum().item()) == 0 for w in sd.workspaces), "TB words not returned to zero"
         for c, tb in zip(cases, tbs):
             bg, Zc, C, Kp = c.spec[:4]
             alpha = half_to_float(sd._alpha_bits(tb))
