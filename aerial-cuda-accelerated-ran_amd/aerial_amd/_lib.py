@@ -128,7 +128,8 @@ class PuschSlotTb(C.Structure):
 
 class HarqServerConfig(C.Structure):
     _fields_ = [("harq_pool_bytes", C.c_size_t), ("max_inflight", C.c_int), ("max_tbs_per_launch", C.c_int),
-                ("max_cbs_per_launch", C.c_int), ("llr_type", C.c_int), ("flags", C.c_uint32)]
+                ("max_cbs_per_launch", C.c_int), ("llr_type", C.c_int), ("flags", C.c_uint32),
+                ("harq_timeout_us", C.c_double)]
 
 
 class HarqRequest(C.Structure):
@@ -147,7 +148,7 @@ class HarqCompletion(C.Structure):
 class HarqServerStats(C.Structure):
     _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("launches", C.c_uint64),
                 ("pool_bytes_in_use", C.c_uint64), ("pool_bytes_peak", C.c_uint64), ("pending", C.c_uint64),
-                ("admission_waits", C.c_uint64)]
+                ("admission_waits", C.c_uint64), ("evictions", C.c_uint64)]
 
 
 class HarqTraceEntry(C.Structure):

@@ -68,10 +68,14 @@ def params_to_device(params):
 class DeRateMatch:
     """cuphyCreate/Setup/DestroyPuschRxRateMatch + launch of its kernel nodes."""
 
-    def __init__(self, fp_config=3):
+    def __init__(self, fp_config=3, descrambling=False):
+        """descrambling: negate LLR n of each TB where its Gold sequence
+        (PerTbParams.cinit) has c(n) = 1, as cuphyCreatePuschRxRateMatch's
+        descramblingOn."""
         self._lib = L.lib()
         self.h = C.c_void_p()
-        L.check("cuphyCreatePuschRxRateMatch", self._lib.cuphyCreatePuschRxRateMatch(C.byref(self.h), fp_config, 0))
+        L.check("cuphyCreatePuschRxRateMatch",
+                self._lib.cuphyCreatePuschRxRateMatch(C.byref(self.h), fp_config, int(descrambling)))
         sz, al = C.c_size_t(), C.c_size_t()
         L.check("cuphyPuschRxRateMatchGetDescrInfo",
                 self._lib.cuphyPuschRxRateMatchGetDescrInfo(C.byref(sz), C.byref(al)))

@@ -36,16 +36,18 @@ class HarqServer:
     """Owner of a cuphyHipHarqServer_t."""
 
     def __init__(self, decoder, pool_bytes=2 << 30, max_inflight=3, max_tbs=1024, max_cbs=8192,
-                 llr_type=L.CUPHY_R_16F, resident=False):
+                 llr_type=L.CUPHY_R_16F, resident=False, harq_timeout_us=0.0):
         """resident: one resident launch takes code blocks as they are
         admitted (max_tbs / max_cbs then bound the TBs / code blocks in
         flight); otherwise admitted transmissions are batched into up to
-        max_inflight launches."""
+        max_inflight launches.  harq_timeout_us > 0: a HARQ buffer not used
+        for that long is freed (a later retransmission of it is rejected)."""
         self.dec = decoder
         self._lib = L.lib()
         cfg = L.HarqServerConfig(harq_pool_bytes=pool_bytes, max_inflight=max_inflight, max_tbs_per_launch=max_tbs,
                                  max_cbs_per_launch=max_cbs, llr_type=llr_type,
-                                 flags=L.CUPHY_HIP_HARQ_SERVER_RESIDENT if resident else 0)
+                                 flags=L.CUPHY_HIP_HARQ_SERVER_RESIDENT if resident else 0,
+                                 harq_timeout_us=harq_timeout_us)
         self.h = C.c_void_p()
         L.check("cuphyHipHarqServerCreate", self._lib.cuphyHipHarqServerCreate(decoder.dec, C.byref(cfg),
                                                                                C.byref(self.h)))

@@ -41,7 +41,18 @@ struct cuphyTensorDescriptor {
 // ---------------------------------------------------------------------------
 struct rm_object : cuphyPuschRxRateMatch {
     int fpconfig = 0;
+    bool descrambling = false;
 };
+
+// Largest E_r of a TB (TS 38.212 5.4.2.1, as the kernel computes it).
+static uint32_t rm_max_E(const PerTbParams& p)
+{
+    const uint64_t nq = (uint64_t)p.Nl * p.Qm, C = p.num_CBs;
+    const uint64_t total = p.uciOnPuschFlag ? p.G : p.encodedSize;
+    if (nq == 0 || C == 0) return 0;
+    const uint64_t q = (total / nq) / C;
+    return (uint32_t)(nq * q + nq * ((q * nq * C < total) ? 1u : 0u));
+}
 struct crc_object : cuphyPuschRxCrcDecode {
     int reverse_bytes = 1;
 };
@@ -423,11 +434,11 @@ cuphyStatus_t CUPHYWINAPI cuphyCreatePuschRxRateMatch(cuphyPuschRxRateMatchHndl_
 {
     if (!pPuschRxRateMatchHndl) return CUPHY_STATUS_INVALID_ARGUMENT;
     *pPuschRxRateMatchHndl = nullptr;
-    if (descramblingOn) return CUPHY_STATUS_NOT_SUPPORTED;
-    if (!nrldpc::rm_kernel(FPconfig)) return CUPHY_STATUS_INVALID_ARGUMENT;
+    if (!nrldpc::rm_kernel(FPconfig, false)) return CUPHY_STATUS_INVALID_ARGUMENT;
     rm_object* o = new (std::nothrow) rm_object;
     if (!o) return CUPHY_STATUS_ALLOC_FAILED;
     o->fpconfig = FPconfig;
+    o->descrambling = descramblingOn != 0;
     *pPuschRxRateMatchHndl = o;
     return CUPHY_STATUS_SUCCESS;
 }
@@ -455,16 +466,18 @@ cuphyStatus_t CUPHYWINAPI cuphySetupPuschRxRateMatch(cuphyPuschRxRateMatchHndl_t
         else
             d.llr_in[i] = (p.nDmrsCdmGrpsNoData == 1 ? pTPrmCdm1RmIn : pTPrmRmIn)[p.userGroupIndex].pAddr;
         cmax = std::max(cmax, p.num_CBs);
+        if (o.descrambling && !p.uciOnPuschFlag && rm_max_E(p) > nrldpc::RM_MAX_DESCR_E)
+            return CUPHY_STATUS_UNSUPPORTED_CONFIG;  // more descrambling bits per CB than the kernel holds
     }
     d.out = ppRmOut;
     d.tb_prms = pTbPrmsGpu;
-    d.descrambling_on = 0;
+    d.descrambling_on = o.descrambling ? 1 : 0;
     if (enableCpuToGpuDescrAsyncCpy &&
         hipMemcpyAsync(pGpuDesc, pCpuDesc, sizeof(nrldpc::rm_descriptor), hipMemcpyHostToDevice, strm) != hipSuccess)
         return CUPHY_STATUS_MEMCPY_ERROR;
     pLaunchCfg->desc = pGpuDesc;
     pLaunchCfg->kernelArgs[0] = &pLaunchCfg->desc;
-    set_node(pLaunchCfg->kernelNodeParamsDriver, nrldpc::rm_kernel(o.fpconfig), dim3(std::max(1u, cmax), std::max<unsigned>(1, nSchUes), 1),
+    set_node(pLaunchCfg->kernelNodeParamsDriver, nrldpc::rm_kernel(o.fpconfig, o.descrambling), dim3(std::max(1u, cmax), std::max<unsigned>(1, nSchUes), 1),
              dim3(nrldpc::RM_BLOCK), pLaunchCfg->kernelArgs);
     set_node(pLaunchCfg->resetKernelNodeParamsDriver, nrldpc::rm_noop_kernel(), dim3(1), dim3(64), pLaunchCfg->kernelArgs);
     set_node(pLaunchCfg->clampKernelNodeParamsDriver, nrldpc::rm_noop_kernel(), dim3(1), dim3(64), pLaunchCfg->kernelArgs);

@@ -111,6 +111,15 @@ __device__ __forceinline__ int rm_input_index(const rm_geometry& g, const PerTbP
     return g.adj + kIdx + jIdx * QAM_STRIDE;
 }
 
+// Position in the TB's rate-matched (transmission-order) bit sequence of
+// de-interleaved bit i of the CB: symbol jIdx, bit kIdx -> start + jIdx*Qm
+// + kIdx (the descrambling index, rate_matching.cu:366-386).
+__device__ __forceinline__ int rm_tx_position(const rm_geometry& g, int i)
+{
+    const int kIdx = i / g.EQ, jIdx = i - kIdx * g.EQ;
+    return g.start + jIdx * g.Qm + kIdx;
+}
+
 // Number of passes over the selected bits; with ndi the first pass covers
 // all L positions (untransmitted ones are written 0).
 __device__ __forceinline__ int rm_num_passes(const rm_geometry& g)

@@ -115,6 +115,7 @@ private:
 struct harq_proc {
     size_t off = 0, bytes = 0;
     bool busy = false;
+    double t_last = 0.0;  // last admission or completion of a transmission using it
 };
 
 struct queued {
@@ -167,7 +168,8 @@ struct cuphyHipHarqServer {
     std::deque<cuphyHipHarqCompletion_t> ready;
     clk::time_point t0;
     size_t ws_bytes = 0, stage_bytes = 0;
-    uint64_t submitted = 0, completed = 0, launches = 0, admission_waits = 0;
+    uint64_t submitted = 0, completed = 0, launches = 0, admission_waits = 0, evictions = 0;
+    double t_last_evict = 0.0;
 
     // resident mode
     bool resident = false;
@@ -220,6 +222,24 @@ struct cuphyHipHarqServer {
         c.t_done_us = t_done;
         ready.push_back(c);
         ++completed;
+    }
+    // Free every idle HARQ buffer unused for harq_timeout_us (checked at
+    // most once per millisecond).
+    void evict_stale()
+    {
+        if (cfg.harq_timeout_us <= 0.0) return;
+        const double t = now_us();
+        if (t - t_last_evict < 1000.0) return;
+        t_last_evict = t;
+        for (auto it = procs.begin(); it != procs.end();) {
+            if (!it->second.busy && t - it->second.t_last > cfg.harq_timeout_us) {
+                pool.release(it->second.off, it->second.bytes);
+                it = procs.erase(it);
+                ++evictions;
+            } else {
+                ++it;
+            }
+        }
     }
     void free_proc(uint64_t id)
     {
@@ -319,7 +339,10 @@ cuphyStatus_t cuphyHipHarqServer::retire_resident()
         }
         const queued& r = slot_req[k];
         auto it = procs.find(r.req.harq_id);
-        if (it != procs.end()) it->second.busy = false;
+        if (it != procs.end()) {
+            it->second.busy = false;
+            it->second.t_last = t;
+        }
         // the kernel released every output of the TB (HARQ buffer included)
         // before its CRC: the buffer may go to another TB now
         if (crc == 0 || crc == CB_REJECTED || (r.req.flags & CUPHY_HIP_HARQ_LAST_TX)) free_proc(r.req.harq_id);
@@ -381,6 +404,7 @@ cuphyStatus_t cuphyHipHarqServer::admit_resident()
             continue;
         }
         it->second.busy = true;
+        it->second.t_last = now_us();
         const int k = free_tb_slots.back();
         free_tb_slots.pop_back();
         const cuphyHipHarqRequest_t& r = q.req;
@@ -459,7 +483,10 @@ cuphyStatus_t cuphyHipHarqServer::retire()
             const queued& r = s.reqs[i];
             const uint32_t crc = s.h_crc[i];
             auto it = procs.find(r.req.harq_id);
-            if (it != procs.end()) it->second.busy = false;
+            if (it != procs.end()) {
+                it->second.busy = false;
+                it->second.t_last = now_us();
+            }
             if (crc == 0 || crc == CB_REJECTED || crc == CRC_PENDING || (r.req.flags & CUPHY_HIP_HARQ_LAST_TX))
                 free_proc(r.req.harq_id);
         }
@@ -517,6 +544,7 @@ cuphyStatus_t cuphyHipHarqServer::admit_and_launch()
                 continue;
             }
             it->second.busy = true;
+            it->second.t_last = now_us();
             s.reqs.push_back(q);
             cbs += C;
         }
@@ -623,7 +651,7 @@ cuphyStatus_t CUPHYWINAPI cuphyHipHarqServerCreate(cuphyLDPCDecoder_t decoder, c
     if (!decoder || !cfg || !server) return CUPHY_STATUS_INVALID_ARGUMENT;
     if (cfg->max_inflight < 1 || cfg->max_inflight > 8 || cfg->max_tbs_per_launch < 1 ||
         cfg->max_tbs_per_launch > 65536 || cfg->max_cbs_per_launch < 1 || cfg->harq_pool_bytes == 0 ||
-        (cfg->flags & ~CUPHY_HIP_HARQ_SERVER_RESIDENT) != 0)
+        (cfg->flags & ~CUPHY_HIP_HARQ_SERVER_RESIDENT) != 0 || !(cfg->harq_timeout_us >= 0.0))
         return CUPHY_STATUS_INVALID_ARGUMENT;
     if ((cfg->flags & CUPHY_HIP_HARQ_SERVER_RESIDENT) && cfg->max_tbs_per_launch > 65535)
         return CUPHY_STATUS_INVALID_ARGUMENT;  // descriptor slot in 16 bits of an item
@@ -736,6 +764,7 @@ cuphyStatus_t CUPHYWINAPI cuphyHipHarqServerPoll(cuphyHipHarqServer_t server, cu
 {
     if (!server || !n_out || max_out < 0 || (max_out > 0 && !out)) return CUPHY_STATUS_INVALID_ARGUMENT;
     cuphyStatus_t st = server->resident ? server->retire_resident() : server->retire();
+    server->evict_stale();
     if (st == CUPHY_STATUS_SUCCESS) st = server->resident ? server->admit_resident() : server->admit_and_launch();
     int n = 0;
     while (n < max_out && !server->ready.empty()) {
@@ -766,6 +795,7 @@ cuphyStatus_t CUPHYWINAPI cuphyHipHarqServerGetStats(cuphyHipHarqServer_t server
     stats->pool_bytes_peak = server->pool.peak();
     stats->pending = server->pending.size();
     stats->admission_waits = server->admission_waits;
+    stats->evictions = server->evictions;
     return CUPHY_STATUS_SUCCESS;
 }
 

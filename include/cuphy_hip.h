@@ -485,6 +485,8 @@ typedef struct {
     int max_cbs_per_launch;          /* >= 1; resident mode: code blocks in flight at once */
     cuphyDataType_t llr_type;        /* element type of every request's LLRs */
     uint32_t flags;                  /* 0 (batched launches) or CUPHY_HIP_HARQ_SERVER_RESIDENT */
+    double harq_timeout_us;          /* > 0: free a HARQ buffer unused this long (the reference's
+                                        HarqPoolManager timeout, harq_pool.hpp); 0: never */
 } cuphyHipHarqServerConfig_t;
 
 typedef struct {
@@ -520,6 +522,7 @@ typedef struct {
     uint64_t pool_bytes_in_use, pool_bytes_peak;
     uint64_t pending;                /* submitted, not yet launched */
     uint64_t admission_waits;        /* polls at which a new TB waited for pool space */
+    uint64_t evictions;              /* HARQ buffers freed by harq_timeout_us */
 } cuphyHipHarqServerStats_t;
 
 cuphyStatus_t CUPHYWINAPI cuphyHipHarqServerCreate(cuphyLDPCDecoder_t decoder, const cuphyHipHarqServerConfig_t* cfg,

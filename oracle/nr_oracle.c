@@ -441,12 +441,37 @@ int nr_orc_rm_E(const nr_orc_rm_cfg* c, int r)
 
 #define LLR_MAX_ABS 10000.0f
 
+/* build_Gold_sequence.m:16-48: x1(0) = 1, x2 = bits of c_init (LSB first),
+ * x1(n+31) = x1(n+3) + x1(n), x2(n+31) = x2(n+3) + x2(n+2) + x2(n+1) + x2(n)
+ * (mod 2), c(n) = x1(n+1600) + x2(n+1600). */
+void nr_orc_gold(unsigned int cinit, int n0, int nbits, unsigned char* bits)
+{
+    const int NC = 1600, N = n0 + nbits;
+    unsigned char* x1 = (unsigned char*)calloc((size_t)N + NC + 31, 1);
+    unsigned char* x2 = (unsigned char*)calloc((size_t)N + NC + 31, 1);
+    x1[0] = 1;
+    for (int i = 0; i < 31; ++i) x2[i] = (unsigned char)((cinit >> i) & 1u);
+    for (int n = 0; n < N + NC; ++n) {
+        x1[n + 31] = x1[n + 3] ^ x1[n];
+        x2[n + 31] = x2[n + 3] ^ x2[n + 2] ^ x2[n + 1] ^ x2[n];
+    }
+    for (int n = 0; n < nbits; ++n) bits[n] = x1[n0 + n + NC] ^ x2[n0 + n + NC];
+    free(x1);
+    free(x2);
+}
+
 static void derate_match_impl(const nr_orc_rm_cfg* c, const float* llr_demap, float* harq, int fp16)
 {
     const int twoZ = 2 * c->Zc;
     const int Kp = c->K - c->F;
     const int fill_lo = Kp - twoZ, fill_hi = c->K - twoZ; /* filler [lo, hi) in circular-buffer coords */
     size_t start = 0;
+    /* descramble_bits.m:31: LLR x (1 - 2c) over the TB's G LLRs */
+    unsigned char* cseq = NULL;
+    if (c->descramble) {
+        cseq = (unsigned char*)malloc((size_t)c->G);
+        nr_orc_gold(c->cinit, 0, c->G, cseq);
+    }
     for (int r = 0; r < c->C; ++r) {
         float* out = harq + (size_t)r * c->Ncb_padded;
         const int E = nr_orc_rm_E(c, r);
@@ -460,7 +485,9 @@ static void derate_match_impl(const nr_orc_rm_cfg* c, const float* llr_demap, fl
             const int idx = (c->k0 + j) % c->Ncb;
             if (idx < fill_lo || idx >= fill_hi) {
                 const int kIdx = k / EQ, jIdx = k % EQ;
-                float v = llr_demap[start + (size_t)jIdx * c->Qm + kIdx];
+                const size_t n = start + (size_t)jIdx * c->Qm + kIdx;
+                float v = llr_demap[n];
+                if (cseq && cseq[n]) v = -v;
                 float* o = &out[twoZ + idx];
                 if (fp16) {
                     v = nr_orc_round_half(v);
@@ -486,6 +513,7 @@ static void derate_match_impl(const nr_orc_rm_cfg* c, const float* llr_demap, fl
         for (int n = 0; n < twoZ; ++n) out[n] = 0.0f;                        /* punctured */
         start += (size_t)E;
     }
+    free(cseq);
 }
 
 void nr_orc_derate_match(const nr_orc_rm_cfg* c, const float* llr_demap, float* harq)

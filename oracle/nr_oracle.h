@@ -13,6 +13,8 @@
  *       update_APP.m:37-42, LDPC_decode.m:16-96 (punctured-column insertion,
  *       alpha table :101-208)
  *   - syndrome (PCE) check:        syndrome_check.m:16-54, compute_check.m:16-52
+ *   - descrambling (Gold sequence): build_Gold_sequence.m:16-48 and
+ *                                  descramble_bits.m:16-33 (LLR x (1 - 2c))
  *   - de-rate-matching + HARQ:     derate_match.m:16-196; output layout and
  *       input clamp from cuPHY/src/cuphy/rate_matching/rate_matching.cu:112-177,
  *       622-683 (HARQ buffer = [2Zc punctured zeros | Ncb circular buffer])
@@ -115,7 +117,12 @@ typedef struct {
     int Ncb, Ncb_padded;     /* circular buffer length, HARQ stride per CB    */
     int k0;                  /* starting position for the rv                  */
     int ndi;                 /* 1 = new data (overwrite), 0 = combine         */
+    int descramble;          /* 1 = LLR n of the TB times (1 - 2 c(n))        */
+    unsigned int cinit;      /* Gold sequence initialisation c_init            */
 } nr_orc_rm_cfg;
+/* Gold sequence bits c(n0 .. n0+nbits-1) of TS 38.211 5.2.1 for c_init, bit
+ * per byte (build_Gold_sequence.m:16-48, bit-serial). */
+void nr_orc_gold(unsigned int cinit, int n0, int nbits, unsigned char* bits);
 /* E of code block r (TS 38.212 5.4.2.1) */
 int  nr_orc_rm_E(const nr_orc_rm_cfg* c, int r);
 /* llr_demap: G LLRs of the TB (layer-demapped, CB after CB, Qm-interleaved).

@@ -30,7 +30,8 @@ class DecCfg(C.Structure):
 class RmCfg(C.Structure):
     _fields_ = [("bg", C.c_int), ("Zc", C.c_int), ("C", C.c_int), ("Qm", C.c_int), ("Nl", C.c_int),
                 ("G", C.c_int), ("K", C.c_int), ("F", C.c_int), ("Ncb", C.c_int),
-                ("Ncb_padded", C.c_int), ("k0", C.c_int), ("ndi", C.c_int)]
+                ("Ncb_padded", C.c_int), ("k0", C.c_int), ("ndi", C.c_int), ("descramble", C.c_int),
+                ("cinit", C.c_uint)]
 
 
 _lib = None
@@ -63,6 +64,8 @@ def lib():
         L.nr_orc_crc.restype = C.c_uint32
         L.nr_orc_rm_E.argtypes = [P(RmCfg), C.c_int]
         L.nr_orc_derate_match.argtypes = [P(RmCfg), C.c_void_p, C.c_void_p]
+        L.nr_orc_gold.argtypes = [C.c_uint, C.c_int, C.c_int, C.c_void_p]
+        L.nr_orc_gold.restype = None
         L.nr_orc_derate_match_fp16.argtypes = [P(RmCfg), C.c_void_p, C.c_void_p]
         L.nr_orc_k0.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
         L.nr_orc_crc_desegment.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
@@ -190,6 +193,13 @@ def k0(bg, rv, Ncb, Zc):
 
 def rm_cfg(**kw):
     return RmCfg(**kw)
+
+
+def gold(cinit, n0, nbits):
+    """TS 38.211 5.2.1 Gold sequence bits c(n0 .. n0+nbits-1), uint8."""
+    out = np.zeros(nbits, np.uint8)
+    lib().nr_orc_gold(cinit, n0, nbits, out.ctypes.data)
+    return out
 
 
 def rm_E(cfg, r):

@@ -435,3 +435,71 @@ def test_encoder_matches_reference_goldens():
             assert np.array_equal(cw, enc[i]), (tag, i)
             n += 1
     assert n == 6
+
+
+# ---------------------------------------------------------------------------
+# Gold sequence (TS 38.211 5.2.1) and descrambling
+# ---------------------------------------------------------------------------
+def _gold_by_matrix_power(cinit, n):
+    """c(n) from 31x31 GF(2) matrix powers of the two LFSRs (a method
+    independent of the oracle's bit-serial recurrence)."""
+    def step(taps):
+        A = np.zeros((31, 31), np.int64)
+        for i in range(30):
+            A[i, i + 1] = 1
+        for t in taps:
+            A[30, t] = 1
+        return A
+
+    def power(A, e):
+        R = np.eye(31, dtype=np.int64)
+        while e:
+            if e & 1:
+                R = (R @ A) % 2
+            A = (A @ A) % 2
+            e >>= 1
+        return R
+
+    m = n + 1600
+    s1 = np.zeros(31, np.int64)
+    s1[0] = 1
+    s2 = np.array([(cinit >> i) & 1 for i in range(31)], np.int64)
+    x1 = (power(step([0, 3]), m) @ s1) % 2
+    x2 = (power(step([0, 1, 2, 3]), m) @ s2) % 2
+    return int(x1[0] ^ x2[0])
+
+
+def test_gold_sequence_matches_matrix_power():
+    rng = np.random.RandomState(7)
+    for cinit in (0, 1, 0x12345, 1507348, 0x7FFFFFFF):
+        ns = [0, 1, 30, 31, 32, 1000] + list(rng.randint(0, 2_000_000, 6))
+        for n in ns:
+            assert O.gold(cinit, int(n), 1)[0] == _gold_by_matrix_power(cinit, int(n)), (cinit, n)
+
+
+def test_gold_sequence_windows_and_linearity():
+    a, b = 0x1A2B3C, 0x0F0F0F
+    full = O.gold(a, 0, 5000)
+    assert np.array_equal(O.gold(a, 1234, 3000), full[1234:4234])  # any window of the same sequence
+    # x2 is linear in c_init and x1 does not depend on it
+    lhs = O.gold(a ^ b, 0, 4000) ^ O.gold(0, 0, 4000)
+    rhs = O.gold(a, 0, 4000) ^ O.gold(b, 0, 4000)
+    assert np.array_equal(lhs, rhs)
+    assert 0.45 < full.mean() < 0.55
+
+
+def test_descrambled_derate_match_is_derate_match_of_descrambled_llrs():
+    # descramble_bits.m (LLR x (1 - 2c) over the TB) before derate_match.m
+    bg, Zc, C, Qm, Nl, G = 1, 64, 3, 4, 2, 3 * 2400
+    K = 22 * Zc
+    cfg = O.rm_cfg(bg=bg, Zc=Zc, C=C, Qm=Qm, Nl=Nl, G=G, K=K, F=40, Ncb=66 * Zc, Ncb_padded=(68 * Zc + 7) // 8 * 8,
+                   k0=O.lib().nr_orc_k0(bg, 2, 66 * Zc, Zc), ndi=1, descramble=1, cinit=0x4321)
+    llr = np.random.RandomState(3).uniform(-30, 30, G).astype(np.float32)
+    a = np.zeros(C * cfg.Ncb_padded, np.float32)
+    O.derate_match(cfg, llr, a)
+    c = O.gold(0x4321, 0, G)
+    plain = O.rm_cfg(**{f: getattr(cfg, f) for f, _ in cfg._fields_})
+    plain.descramble = 0
+    b = np.zeros_like(a)
+    O.derate_match(plain, llr * (1.0 - 2.0 * c.astype(np.float32)), b)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))

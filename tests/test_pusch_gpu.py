@@ -46,27 +46,37 @@ def qam_stride_layout(llr_demap, Qm, Nl, n_bbu, layer_map, fill=123.0):
     return buf.reshape(-1)
 
 
+# descrambling cases: the RM_CASES plus one TB of 50 CBs (G = 600,000: the
+# Gold sequence far from its start)
+DESCR_CASES = RM_CASES + [(1, 384, 50, 8448, 8, 1, 50 * 12000, 1)]
+
+
+@pytest.mark.parametrize("descr", [False, True])
 @pytest.mark.parametrize("fp", [0, 1, 2, 3])
-@pytest.mark.parametrize("case", range(len(RM_CASES)))
-def test_derate_match_two_transmissions(case, fp):
+@pytest.mark.parametrize("case", range(len(DESCR_CASES)))
+def test_derate_match_two_transmissions(case, fp, descr):
     _need_gpu()
     from aerial_amd.pusch import DeRateMatch, make_tb_params
-    bg, Zc, C, Kp, Qm, Nl, G, rv = RM_CASES[case]
+    if case >= len(RM_CASES) and not descr:
+        pytest.skip("the long TB is a descrambling case")
+    bg, Zc, C, Kp, Qm, Nl, G, rv = DESCR_CASES[case]
     in16, out16 = fp in (1, 3), fp in (2, 3)
-    print("[case] derate BG%d Zc=%d C=%d K'=%d Qm=%d Nl=%d G=%d rv=%d fp=%d" % (bg, Zc, C, Kp, Qm, Nl, G, rv, fp),
-          flush=True)
+    cinit = (0x1234567 * (case + 1) + 0x9E37) & 0x7FFFFFFF if descr else 0
+    print("[case] derate BG%d Zc=%d C=%d K'=%d Qm=%d Nl=%d G=%d rv=%d fp=%d descr=%d cinit=%#x"
+          % (bg, Zc, C, Kp, Qm, Nl, G, rv, fp, descr, cinit), flush=True)
     n_bbu = Nl + (1 if case % 2 else 0)
     lmap = list(range(n_bbu))[::-1][:Nl] if case % 2 else list(range(Nl))
-    rm = DeRateMatch(fp_config=fp)
+    rm = DeRateMatch(fp_config=fp, descrambling=descr)
     p1, _ = make_tb_params(bg, Zc, C, Kp, Qm, Nl, G, rv, ndi=1, n_bbu_layers=n_bbu, layer_map=lmap)
     ocfg = O.rm_cfg(bg=bg, Zc=Zc, C=C, Qm=Qm, Nl=Nl, G=G, K=p1.K, F=p1.F, Ncb=p1.Ncb,
-                    Ncb_padded=p1.Ncb_padded, k0=p1.k0, ndi=1)
+                    Ncb_padded=p1.Ncb_padded, k0=p1.k0, ndi=1, descramble=int(descr), cinit=cinit)
     n_cmp = 2 * Zc + p1.Ncb
     ref = np.zeros(C * p1.Ncb_padded, np.float32)
     out_t = torch.float16 if out16 else torch.float32
     harq = torch.full((C * p1.Ncb_padded,), 77.0, dtype=out_t, device="cuda")
     for tx, (ndi, rv_) in enumerate([(1, rv), (0, (rv + 2) % 4)]):
         p, _ = make_tb_params(bg, Zc, C, Kp, Qm, Nl, G, rv_, ndi=ndi, n_bbu_layers=n_bbu, layer_map=lmap)
+        p.cinit = cinit
         llr = tb_llrs(G, 1000 * case + tx)
         if in16:
             llr = llr.astype(np.float16).astype(np.float32)

@@ -178,6 +178,48 @@ def test_retransmission_waits_for_its_predecessor(dec, templates, resident):
     srv.close()
 
 
+@pytest.mark.parametrize("resident", [False, True])
+def test_harq_buffer_timeout_eviction(dec, templates, resident):
+    """A HARQ buffer left unused for harq_timeout_us is freed; a later
+    retransmission of that process is rejected."""
+    import time
+    from aerial_amd import _lib as L
+    from aerial_amd.server import HarqServer, RV_SEQUENCE
+    from aerial_amd.slot import SlotTb
+    print("[case] server: HARQ buffer timeout 2 ms, resident=%s" % resident, flush=True)
+    ti = next((i for i in range(len(templates.tbs))
+               if int(alone(dec, templates, i, 1, 8)[3].cpu().numpy().view(np.uint32)[0]) != 0), None)
+    if ti is None:
+        pytest.skip("no template fails its first transmission")
+    tb = templates.tbs[ti]
+    srv = HarqServer(dec, pool_bytes=1 << 28, max_inflight=2, max_tbs=8, max_cbs=256, resident=resident,
+                     harq_timeout_us=2000.0)
+    p0, _ = tb.params(0, 1)
+    outs = SlotTb.alloc_outputs(p0)
+    srv.submit(srv.request(55, p0, tb.llrs[0], outs, templates.mbs[ti][0], max_iterations=8, tag=0))
+    got = []
+    while not got:
+        got += srv.poll()
+    assert got[0].tb_crc != 0
+    if not resident:  # batched: the launch's buffers are released when it has ended
+        torch.cuda.synchronize()
+        srv.poll()
+    assert srv.stats()["pool_bytes_in_use"] > 0  # kept for the retransmission
+    t_end = time.time() + 0.02
+    while time.time() < t_end:
+        srv.poll()
+        time.sleep(0.0005)
+    st = srv.stats()
+    assert st["evictions"] == 1 and st["pool_bytes_in_use"] == 0, st
+    p1, _ = tb.params(RV_SEQUENCE[1], 0)
+    srv.submit(srv.request(55, p1, tb.llrs[RV_SEQUENCE[1]], outs, templates.mbs[ti][1], max_iterations=8, tag=1))
+    got = []
+    while not got:
+        got += srv.poll()
+    assert got[0].tag == 1 and got[0].status == L.CUPHY_STATUS_INVALID_ARGUMENT
+    srv.close()
+
+
 def test_invalid_requests_rejected(dec, templates):
     from aerial_amd import _lib as L
     from aerial_amd.server import HarqServer
