@@ -178,6 +178,46 @@ def test_retransmission_waits_for_its_predecessor(dec, templates, resident):
     srv.close()
 
 
+def test_resident_launch_restarts_after_idle(dec, templates):
+    """The resident launch leaves after 20 ms without work, its workgroups
+    leaving their tickets behind; work submitted afterwards starts a new
+    launch that takes those tickets over, and every TB still decodes exactly
+    as alone."""
+    import time
+    from aerial_amd.server import HarqServer
+    from aerial_amd.slot import SlotTb
+    print("[case] server: resident launch idle exit and restart", flush=True)
+    srv = HarqServer(dec, pool_bytes=1 << 28, max_tbs=16, max_cbs=512, resident=True)
+    results, crcs = [], {}
+    for wave_, tis in enumerate(([0, 1, 2], [3, 4], [5])):
+        for ti in tis:
+            tb = templates.tbs[ti]
+            p0, _ = tb.params(0, 1)
+            outs = SlotTb.alloc_outputs(p0)
+            srv.submit(srv.request(200 + ti, p0, tb.llrs[0], outs, templates.mbs[ti][0], max_iterations=8,
+                                   last=True, tag=ti))
+            results.append((ti, outs))
+        got = []
+        while len(got) < len(tis):
+            got += srv.poll()
+        assert all(c.status == 0 for c in got)
+        crcs.update({c.tag: c.tb_crc for c in got})
+        t_end = time.time() + 0.06  # idle: the launch leaves after 20 ms
+        while time.time() < t_end:
+            srv.poll()
+            time.sleep(0.001)
+    st = srv.stats()
+    srv.close()
+    assert st["launches"] == 3, st
+    assert st["pool_bytes_in_use"] == 0
+    for ti, outs in results:
+        ref = alone(dec, templates, ti, 1, 8)
+        nw = (templates.tbs[ti].Kp + 31) // 32
+        assert torch.equal(outs[0][:, :nw], ref[0][:, :nw]), ti
+        assert torch.equal(outs[1], ref[1]) and torch.equal(outs[2], ref[2]), ti
+        assert crcs[ti] == int(ref[3].cpu().numpy().view(np.uint32)[0]), ti
+
+
 @pytest.mark.parametrize("resident", [False, True])
 def test_harq_buffer_timeout_eviction(dec, templates, resident):
     """A HARQ buffer left unused for harq_timeout_us is freed; a later
