@@ -71,8 +71,8 @@ struct timer {
     }
     ~timer()
     {
-        hipEventDestroy(a);
-        hipEventDestroy(b);
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
     }
 };
 
@@ -116,8 +116,8 @@ void run_tb(cuphyLDPCDecoder_t dec, hipStream_t strm, int reps)
     CHECK_HIP(hipEventElapsedTime(&ms, t.a, t.b));
     std::printf("tb: BG1 Z=384 mb=46 8 CBs fp32 10 it: %.2f us per TB launch (%d launches)\n", 1e3 * ms / reps,
                 reps);
-    hipFree(d_llr);
-    hipFree(d_out);
+    (void)hipFree(d_llr);
+    (void)hipFree(d_out);
 }
 
 void run_batch(cuphyLDPCDecoder_t dec, hipStream_t strm, int reps, int ncb, int mb, const char* name)
@@ -170,10 +170,10 @@ void run_batch(cuphyLDPCDecoder_t dec, hipStream_t strm, int reps, int ncb, int 
     const double gbps = (double)ncb * 22 * Z / (us * 1e-6) / 1e9;
     std::printf("%s: %d CBs BG1 Z=384 mb=%d fp16 10 it: %.2f us per launch, %.3f Gbps (%d launches)\n", name, ncb,
                 mb, us, gbps, reps);
-    hipFree(d_items);
-    hipFree(d_ws);
-    hipFree(d_llr);
-    hipFree(d_out);
+    (void)hipFree(d_items);
+    (void)hipFree(d_ws);
+    (void)hipFree(d_llr);
+    (void)hipFree(d_out);
 }
 
 }  // namespace
@@ -196,7 +196,7 @@ int main(int argc, char** argv)
     if (what == "batch" || what == "all") run_batch(dec, strm, reps, 2800, 4, "batch");
     if (what == "wide" || what == "all") run_batch(dec, strm, reps, 256, 46, "wide");
     CHECK_HIP(hipStreamDestroy(strm));
-    cuphyDestroyLDPCDecoder(dec);
-    cuphyDestroyContext(ctx);
+    (void)cuphyDestroyLDPCDecoder(dec);
+    (void)cuphyDestroyContext(ctx);
     return 0;
 }
