@@ -121,7 +121,7 @@ def fused_slot_section(dec, steps):
 
 
 def serving_section(dec, serve_ms, config3_gbps=None, inflight=3, loads=(0.6, 0.9, 1.2), resident=True,
-                    templates=None):
+                    templates=None, config3_cbs_per_s=None):
     """Serving loop: Poisson TB arrivals from 32 cells (TB shapes as in the
     heterogeneous slot, binary16 BPSK-AWGN LLRs), HARQ with rv 0, 2, 3, 1
     and up to 4 transmissions, retransmissions released one HARQ round trip
@@ -139,6 +139,7 @@ def serving_section(dec, serve_ms, config3_gbps=None, inflight=3, loads=(0.6, 0.
     srv = HarqServer(dec, pool_bytes=8 << 30, max_inflight=inflight, max_tbs=2048, max_cbs=16384, resident=resident)
     rtt = 2000.0
     mean_bits = float(np.mean([tb.info_bits for tb in templates.tbs]))
+    mean_cbs = float(np.mean([tb.C for tb in templates.tbs]))
     n_sat = 1500
     sat = [(0.0, i % 32, i % n_templates) for i in range(n_sat)]
     entries, outs, meta = build_trace(srv, templates, sat)
@@ -147,7 +148,8 @@ def serving_section(dec, serve_ms, config3_gbps=None, inflight=3, loads=(0.6, 0.
     first_tx_span_us = max(done[4 * a].t_done_us for a in range(n_sat))
     cap_tbs = n_sat / (first_tx_span_us * 1e-6)
     out = {"saturation": rep, "capacity_first_tx_tbs_per_s": round(cap_tbs, 1),
-           "capacity_first_tx_gbps": round(cap_tbs * mean_bits / 1e9, 3), "loads": []}
+           "capacity_first_tx_gbps": round(cap_tbs * mean_bits / 1e9, 3),
+           "capacity_first_tx_cbs_per_s": round(cap_tbs * mean_cbs, 1), "loads": []}
     for x in loads:
         arr = poisson_arrivals(32, x * cap_tbs, serve_ms * 1e3, n_templates, seed=int(100 * x))
         entries, outs, meta = build_trace(srv, templates, arr)
@@ -157,6 +159,8 @@ def serving_section(dec, serve_ms, config3_gbps=None, inflight=3, loads=(0.6, 0.
         r["offered_gbps"] = round(x * cap_tbs * mean_bits / 1e9, 3)
         if config3_gbps:
             r["offered_vs_config3"] = round(x * cap_tbs * mean_bits / 1e9 / config3_gbps, 4)
+        if config3_cbs_per_s:
+            r["offered_cbs_vs_config3"] = round(x * cap_tbs * mean_cbs / config3_cbs_per_s, 4)
         r["engine"] = srv.stats()
         out["loads"].append(r)
         print("serving (%s) load %.1f: %s" % ("resident" if resident else "batched", x, json.dumps(r)),
@@ -360,10 +364,11 @@ def main():
     serving = None
     if "serving" in sections:
         c3g = config3["batched_gbps"] if config3 else None
+        c3c = config3["code_blocks"] / (config3["batched_ms"] * 1e-3) if config3 else None
         serving = serving_section(dec, a.serve_ms, c3g, loads=[float(x) for x in a.serve_loads.split(",")],
-                                  resident=True)
+                                  resident=True, config3_cbs_per_s=c3c)
         batched = serving_section(dec, a.serve_ms, c3g, inflight=a.serve_inflight, loads=(0.6,), resident=False,
-                                  templates=serving.pop("templates"))
+                                  templates=serving.pop("templates"), config3_cbs_per_s=c3c)
         batched.pop("templates")
         serving["batched_launches_for_comparison"] = batched
 
