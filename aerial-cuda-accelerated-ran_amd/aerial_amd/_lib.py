@@ -41,6 +41,7 @@ MAX_WORDS_PER_CODE_BLOCK = 264
 SLOT_CB_ERROR = 0xFFFFFFFF  # CRC value of a CB/TB the fused slot kernel rejected
 CUPHY_HIP_HARQ_LAST_TX = 0x100
 CUPHY_HIP_SLOT_TB_NOTIFY = 0x200
+CUPHY_HIP_SLOT_TB_DESCRAMBLE = 0x400
 CUPHY_HIP_HARQ_SERVER_RESIDENT = 0x1
 
 

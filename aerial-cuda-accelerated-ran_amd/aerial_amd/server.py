@@ -65,11 +65,11 @@ class HarqServer:
             pass
 
     def request(self, harq_id, params, llr, outputs, mb, max_iterations=10, early_term=True, llr_scale=1.0,
-                clamp_value=32.0, last=False, tag=0):
+                clamp_value=32.0, last=False, tag=0, descramble=False):
         """A cuphyHipHarqRequest_t; outputs = (cb_bits, cb_crc, tb_bytes,
         tb_crc, cb_iterations) device tensors as SlotTb.alloc_outputs makes
         them (the engine reports the TB CRC in the completion; tb_crc and
-        cb_iterations may be None)."""
+        cb_iterations may be None); descramble: with params.cinit."""
         r = L.HarqRequest()
         r.harq_id = harq_id
         r.tag = tag
@@ -83,7 +83,8 @@ class HarqServer:
         r.cb_iterations = cb_iters.data_ptr() if cb_iters is not None else None
         r.num_parity_nodes = mb
         r.max_iterations = max_iterations
-        r.flags = (L.CUPHY_LDPC_DECODE_EARLY_TERM if early_term else 0) | (L.CUPHY_HIP_HARQ_LAST_TX if last else 0)
+        r.flags = ((L.CUPHY_LDPC_DECODE_EARLY_TERM if early_term else 0) | (L.CUPHY_HIP_HARQ_LAST_TX if last else 0) |
+                   (L.CUPHY_HIP_SLOT_TB_DESCRAMBLE if descramble else 0))
         r.llr_scale = llr_scale
         key = (params.bg, params.Zc, mb)
         if key not in self._norm:

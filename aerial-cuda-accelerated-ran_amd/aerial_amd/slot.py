@@ -28,10 +28,19 @@ def half_bits(x):
     return int(torch.tensor([x], dtype=torch.float16).view(torch.int16).item()) & 0xFFFF
 
 
-def lds_values(p, mb):
+def lds_values(p, mb, descramble=False):
     """LDS values a CB of TB parameters p needs: the HARQ circular buffer
-    and the decoder's APP array share one binary16 array."""
-    return max(2 * p.Zc + p.Ncb, (nsys(p.bg) + mb) * p.Zc)
+    and the decoder's APP array share one binary16 array; a descrambled CB
+    adds its Gold-sequence words after it (two values per word, for the
+    largest per-CB E starting at bit 31 of a word, plus one word)."""
+    arr = max(2 * p.Zc + p.Ncb, (nsys(p.bg) + mb) * p.Zc)
+    if not descramble or p.uciOnPuschFlag:
+        return arr
+    nq, C = p.Nl * p.Qm, p.num_CBs
+    q = p.encodedSize // nq // C
+    emax = nq * q + (nq if q * nq * C < p.encodedSize else 0)
+    words = (31 + emax - 1) // 32 + 1
+    return ((arr + 1) & ~1) + 2 * (words + 1)
 
 
 class SlotTb:
@@ -40,16 +49,18 @@ class SlotTb:
     params: PerTbParams (ndi, rv/k0, G, ...); llr: this transmission's LLRs in
     the rate-match input layout (int8 / float16 / float32 device tensor);
     harq: float16 device tensor of >= (firstCodeBlockIndex + C) * Ncb_padded
-    values; mb: decoder parity rows (cover every position received so far)."""
+    values; mb: decoder parity rows (cover every position received so far);
+    descramble: negate LLRs where the Gold sequence of params.cinit is 1."""
 
     def __init__(self, params, llr, harq, mb, max_iterations=10, early_term=False, llr_scale=1.0, alpha=None,
-                 clamp_value=32.0, outputs=None):
+                 clamp_value=32.0, outputs=None, descramble=False):
         self.params = params
         self.llr = llr
         self.harq = harq
         self.mb = int(mb)
         self.max_iterations = max_iterations
         self.early_term = early_term
+        self.descramble = descramble
         self.llr_scale = llr_scale
         self.alpha = alpha
         self.clamp_value = clamp_value
@@ -150,7 +161,8 @@ class SlotDecoder:
             j.cb_iterations = tb.cb_iterations.data_ptr() if tb.cb_iterations is not None else None
             j.num_parity_nodes = tb.mb
             j.max_iterations = tb.max_iterations
-            j.flags = L.CUPHY_LDPC_DECODE_EARLY_TERM if tb.early_term else 0
+            j.flags = ((L.CUPHY_LDPC_DECODE_EARLY_TERM if tb.early_term else 0) |
+                       (L.CUPHY_HIP_SLOT_TB_DESCRAMBLE if tb.descramble else 0))
             j.llr_scale = tb.llr_scale
             h = self._alpha_bits(tb)
             j.norm.f16x2 = h | (h << 16)
@@ -176,7 +188,7 @@ class SlotDecoder:
             ln.items = torch.from_numpy(ids).cuda()
             ln.n_items = len(rows)
             for _, t, _ in rows:
-                ln.max_values = max(ln.max_values, lds_values(tbs[t].params, tbs[t].mb))
+                ln.max_values = max(ln.max_values, lds_values(tbs[t].params, tbs[t].mb, tbs[t].descramble))
                 ln.max_mb = max(ln.max_mb, tbs[t].mb)
             ln.workspace = torch.zeros(sz.value // 4, dtype=torch.int32, device="cuda")
             self.launches.append(ln)

@@ -420,7 +420,8 @@ cuphyStatus_t cuphyHipHarqServer::admit_resident()
         j.cb_iterations = r.cb_iterations;
         j.num_parity_nodes = r.num_parity_nodes;
         j.max_iterations = r.max_iterations;
-        j.flags = (uint16_t)((r.flags & CUPHY_LDPC_DECODE_EARLY_TERM) | CUPHY_HIP_SLOT_TB_NOTIFY);
+        j.flags = (uint16_t)((r.flags & (CUPHY_LDPC_DECODE_EARLY_TERM | CUPHY_HIP_SLOT_TB_DESCRAMBLE)) |
+                             CUPHY_HIP_SLOT_TB_NOTIFY);
         j.reserved = 0;
         j.llr_scale = r.llr_scale;
         j.norm = r.norm;
@@ -585,7 +586,8 @@ cuphyStatus_t cuphyHipHarqServer::launch(launch_slot& s)
         j.cb_iterations = r.cb_iterations;
         j.num_parity_nodes = r.num_parity_nodes;
         j.max_iterations = r.max_iterations;
-        j.flags = (uint16_t)((r.flags & CUPHY_LDPC_DECODE_EARLY_TERM) | CUPHY_HIP_SLOT_TB_NOTIFY);
+        j.flags = (uint16_t)((r.flags & (CUPHY_LDPC_DECODE_EARLY_TERM | CUPHY_HIP_SLOT_TB_DESCRAMBLE)) |
+                             CUPHY_HIP_SLOT_TB_NOTIFY);
         j.reserved = 0;
         j.llr_scale = r.llr_scale;
         j.norm = r.norm;
@@ -595,7 +597,7 @@ cuphyStatus_t cuphyHipHarqServer::launch(launch_slot& s)
         const PerTbParams& p = r.prm;
         const int mb = r.num_parity_nodes;
         const uint64_t cost = (uint64_t)p.Zc * (uint64_t)num_edges((int)p.bg, mb) * (uint64_t)std::max(1, (int)r.max_iterations);
-        const int lds = (int)std::max<uint32_t>(2 * p.Zc + p.Ncb, (uint32_t)(nsys_of((int)p.bg) + mb) * p.Zc);
+        const int lds = nrldpc::slot_lds_values(p, mb, (r.flags & CUPHY_HIP_SLOT_TB_DESCRAMBLE) != 0);
         auto& dst = (mb <= SMALL_MB) ? small : big;
         for (uint32_t c = 0; c < p.num_CBs; ++c) dst.push_back({cost, ((uint32_t)t << 16) | c});
         if (mb <= SMALL_MB) {
@@ -749,6 +751,10 @@ cuphyStatus_t CUPHYWINAPI cuphyHipHarqServerSubmit(cuphyHipHarqServer_t server, 
     if (!req->llr || !req->cb_bits || !req->cb_crc || !req->tb_bytes) return CUPHY_STATUS_INVALID_ARGUMENT;
     const int mb = req->num_parity_nodes;
     if (mb < 4 || mb > (bg == 1 ? 46 : 42) || req->max_iterations < 0) return CUPHY_STATUS_INVALID_ARGUMENT;
+    // descrambled CBs: sequence words within the launch's LDS
+    const int lds = nrldpc::slot_lds_values(p, mb, (req->flags & CUPHY_HIP_SLOT_TB_DESCRAMBLE) != 0);
+    if (lds > (server->resident ? nrldpc::SERVE_LDS_VALUES : nrldpc::SLOT_MAX_LDS_VALUES))
+        return CUPHY_STATUS_INVALID_ARGUMENT;
     if ((int)p.num_CBs > server->cfg.max_cbs_per_launch) return CUPHY_STATUS_INVALID_ARGUMENT;
     if (server->harq_bytes(p) > server->cfg.harq_pool_bytes) return CUPHY_STATUS_INVALID_ARGUMENT;
     queued q;

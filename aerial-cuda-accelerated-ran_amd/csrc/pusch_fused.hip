@@ -35,6 +35,7 @@
 // leave, so the launch replays from a graph.
 #include "crc_core.hpp"
 #include "derate_core.hpp"
+#include "gold_core.hpp"
 #include "ldpc_core.hpp"
 #include "pusch_fused.hpp"
 
@@ -114,10 +115,14 @@ __device__ __forceinline__ void cb_and_share_crc(const fused_lds& sh, const uint
 
 // Combine this transmission's E LLRs of the CB into the LDS circular buffer
 // `cbuf` (passes of distinct positions, barrier after each).  Indices are
-// advanced incrementally: one division per thread and pass.
-template <typename TI>
+// advanced incrementally: one division per thread and pass.  DESCR: LLR i
+// (= k * EQ + j) is bit n = start + j * Qm + k of the TB's rate-matched
+// sequence and is negated where c(n) = 1; seq holds the words of c from
+// word start / 32 on, soff = start % 32.
+template <typename TI, bool DESCR>
 __device__ __forceinline__ void combine_tx(_Float16* __restrict__ cbuf, const TI* __restrict__ in,
-                                           const rm_geometry& g, const PerTbParams& p, float scale)
+                                           const rm_geometry& g, const PerTbParams& p, float scale,
+                                           const uint32_t* __restrict__ seq, int soff)
 {
     const int nthr = blockDim.x;
     const int npass = rm_num_passes(g);
@@ -142,7 +147,11 @@ __device__ __forceinline__ void combine_tx(_Float16* __restrict__ cbuf, const TI
                 }
                 const int o = (c < g.Kd) ? c : c + g.F;
                 const float x = llr_as_float(in[idx]) * scale;
-                const _Float16 v = rm_num<_Float16>::clamp((_Float16)x);
+                _Float16 v = rm_num<_Float16>::clamp((_Float16)x);
+                if (DESCR) {
+                    const int n = soff + j * g.Qm + k;
+                    if ((seq[n >> 5] >> (n & 31)) & 1u) v = -v;
+                }
                 cbuf[o] = rm_num<_Float16>::clamp(cbuf[o] + v);
                 c += nthr;
                 while (c >= g.L) c -= g.L;
@@ -173,10 +182,13 @@ template <> struct slot_vec<float> {
 // layers and the LLRs are not UCI-multiplexed: thread t takes whole symbols
 // j = t, t + nthr, ... (one vector load of the symbol's QAM_STRIDE slots,
 // all loads of a thread issued before its LDS updates); bit k of symbol j
-// is de-interleaved bit i = k * E/Qm + j, at circle position c0 + i.
-template <typename TI>
+// is de-interleaved bit i = k * E/Qm + j, at circle position c0 + i, and
+// bit n = start + j * Qm + k of the TB's rate-matched sequence (DESCR: the
+// symbol's Qm sequence bits are taken from one 64-bit window of seq).
+template <typename TI, bool DESCR>
 __device__ __forceinline__ void combine_symbols(_Float16* __restrict__ cbuf, const TI* __restrict__ in,
-                                                const rm_geometry& g, float scale)
+                                                const rm_geometry& g, float scale, const uint32_t* __restrict__ seq,
+                                                int soff)
 {
     typedef typename slot_vec<TI>::type vec_t;
     constexpr int NB = 4;  // symbols per thread in flight
@@ -193,11 +205,18 @@ __device__ __forceinline__ void combine_symbols(_Float16* __restrict__ cbuf, con
         for (int u = 0; u < NB; ++u) {
             const int j = j0 + u * nthr;
             if (j >= g.EQ) break;
+            uint32_t bits = 0;
+            if (DESCR) {  // bits n0 .. n0 + Qm - 1, n0 = soff + j * Qm (word n0/32 + 1 exists)
+                const int n0 = soff + j * g.Qm;
+                const uint64_t w2 = ((uint64_t)seq[(n0 >> 5) + 1] << 32) | seq[n0 >> 5];
+                bits = (uint32_t)(w2 >> (n0 & 31));
+            }
             for (int k = 0; k < g.Qm; ++k) {
                 int c = g.c0 + k * g.EQ + j;
                 if (c >= g.L) c -= g.L;
                 const int o = (c < g.Kd) ? c : c + g.F;
-                const _Float16 x = rm_num<_Float16>::clamp((_Float16)(llr_as_float(v[u][k]) * scale));
+                _Float16 x = rm_num<_Float16>::clamp((_Float16)(llr_as_float(v[u][k]) * scale));
+                if (DESCR && ((bits >> k) & 1u)) x = -x;
                 cbuf[o] = rm_num<_Float16>::clamp(cbuf[o] + x);
             }
         }
@@ -227,9 +246,16 @@ __device__ __forceinline__ uint32_t fused_cb(_Float16* __restrict__ hb, fused_ld
     const int Kb = __builtin_amdgcn_readfirstlane((int)p.K) / Zc;
     const int C = __builtin_amdgcn_readfirstlane((int)p.num_CBs);
     const size_t ncb_padded = (size_t)__builtin_amdgcn_readfirstlane((int)p.Ncb_padded);
+    // descrambling: words w_base .. w_base + nw - 1 of c (covering the CB's
+    // bits [start, start + E)) and one more word, after the HARQ/APP values
+    const bool descr = (job.flags & CUPHY_HIP_SLOT_TB_DESCRAMBLE) != 0 && !g.uci;
+    const int seq_at = (max(nh, nval) + 1) & ~1;
+    const uint32_t w_base = (uint32_t)g.start >> 5;
+    const int nw = descr && g.E > 0 ? (int)(((uint32_t)(g.start + g.E - 1) >> 5) - w_base + 1u) : 0;
     const bool ok = (bg == 1 || bg == 2) && z_index(Zc) >= 0 && mb >= 4 &&
                     mb <= (bg == 1 ? 46 : 42) && mb <= MAXR && nh <= lds_values && nval <= lds_values &&
-                    Kb * Zc <= 32 * (int)SLOT_CB_WORDS && g.L > 0 && g.E > 0;
+                    Kb * Zc <= 32 * (int)SLOT_CB_WORDS && g.L > 0 && g.E > 0 &&
+                    (!descr || seq_at + 2 * (nw + 1) <= lds_values);
     if (!ok) {
         if (threadIdx.x == 0) job.cb_crc[r] = FUSED_CB_ERROR;
         return TB_ACC_ERROR;
@@ -259,6 +285,14 @@ __device__ __forceinline__ uint32_t fused_cb(_Float16* __restrict__ hb, fused_ld
         }
         for (int n = nv * 8 + tid; n < nh; n += nthr) hb[n] = harq[n];
     }
+    uint32_t* seq = reinterpret_cast<uint32_t*>(hb + seq_at);
+    if (descr) {  // each wave generates its share of the words while the loads land
+        const int nwave = nthr >> 6, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+        const int per = (nw + nwave - 1) / nwave;
+        const int a = min(nw, wave * per), b = min(nw, a + per);
+        gold_words_wave(p.cinit, w_base + (uint32_t)a, w_base + (uint32_t)b, seq + a);
+        if (tid == 0) seq[nw] = 0u;
+    }
     __syncthreads();
     if (g.ndi)
         for (int n = tid; n < g.F; n += nthr) cbuf[g.Kd + n] = (_Float16)LLR_MAX_ABS;
@@ -266,10 +300,18 @@ __device__ __forceinline__ uint32_t fused_cb(_Float16* __restrict__ hb, fused_ld
     // ---- 2. combine this transmission -------------------------------------
     const bool sym_path = !g.uci && g.Nl == g.nbbu && g.E <= g.L &&
                           ((uintptr_t)(in + g.adj) % sizeof(typename slot_vec<TI>::type)) == 0;
-    if (sym_path)
-        combine_symbols<TI>(cbuf, in, g, job.llr_scale);
-    else
-        combine_tx<TI>(cbuf, in, g, p, job.llr_scale);
+    const int soff = g.start & 31;
+    if (descr) {
+        if (sym_path)
+            combine_symbols<TI, true>(cbuf, in, g, job.llr_scale, seq, soff);
+        else
+            combine_tx<TI, true>(cbuf, in, g, p, job.llr_scale, seq, soff);
+    } else {
+        if (sym_path)
+            combine_symbols<TI, false>(cbuf, in, g, job.llr_scale, seq, soff);
+        else
+            combine_tx<TI, false>(cbuf, in, g, p, job.llr_scale, seq, soff);
+    }
 
     // ---- 3. write back the HARQ buffer; APP = clamp(HARQ), +0 for -0 ----
     {
@@ -561,7 +603,7 @@ cuphyStatus_t ensure_lds_attr(void* func, size_t bytes);  // ldpc_decode.hip
 int serve_resident_grid(int cu_count)
 {
     void* func = (void*)&pusch_serve_resident<_Float16>;
-    const size_t lds = ((size_t)68 * 384 * 2 + 15) / 16 * 16 + 64u;
+    const size_t lds = ((size_t)SERVE_LDS_VALUES * 2 + 15) / 16 * 16 + 64u;
     if (ensure_lds_attr(func, lds) != CUPHY_STATUS_SUCCESS) return 0;
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, func, 384, lds) != hipSuccess || per_cu < 1) per_cu = 1;
@@ -580,7 +622,7 @@ cuphyStatus_t serve_resident_launch(const serve_ring_hdr* ring, const uint32_t* 
     case CUPHY_R_32F: func = (void*)&pusch_serve_resident<float>; break;
     default: return CUPHY_STATUS_UNSUPPORTED_TYPE;
     }
-    int lds_values = 68 * 384;  // any TB
+    int lds_values = SERVE_LDS_VALUES;  // any TB (descrambled: E within the word budget)
     const size_t lds = ((size_t)lds_values * 2 + 15) / 16 * 16 + 64u;
     cuphyStatus_t s = ensure_lds_attr(func, lds);
     if (s != CUPHY_STATUS_SUCCESS) return s;
@@ -598,7 +640,7 @@ cuphyStatus_t slot_decode(const cuphyHipPuschSlotTb_t* d_tbs, int n_tbs, const u
 {
     if (n_items < 0 || n_tbs < 0 || n_tbs > 65536 || !d_ws) return CUPHY_STATUS_INVALID_ARGUMENT;
     if (n_items > 0 && (!d_tbs || !d_items || n_tbs == 0)) return CUPHY_STATUS_INVALID_ARGUMENT;
-    if (max_lds_values <= 0 || max_lds_values > 68 * 384) return CUPHY_STATUS_INVALID_ARGUMENT;
+    if (max_lds_values <= 0 || max_lds_values > SLOT_MAX_LDS_VALUES) return CUPHY_STATUS_INVALID_ARGUMENT;
     if (max_parity_nodes < 4 || max_parity_nodes > 46) return CUPHY_STATUS_INVALID_ARGUMENT;
     void* func = fused_kernel(llr_type, max_parity_nodes);
     if (!func) return CUPHY_STATUS_UNSUPPORTED_TYPE;

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
+
 #include "cuphy_hip.h"
 
 namespace nrldpc {
@@ -11,6 +13,33 @@ namespace nrldpc {
 // are used) followed by two words per CB slot, of which each TB uses the
 // pair at its first slot: TB-CRC accumulator and count of finished CBs.
 constexpr int FUSED_WS_QUEUE_WORDS = 64;
+
+// LDS values (binary16) of the fused kernels: a slot launch takes up to
+// SLOT_MAX_LDS_VALUES; a resident launch always has SERVE_LDS_VALUES (any
+// HARQ/APP array, 68 * 384, plus 2048 descrambling words: E up to 65,473
+// bits per descrambled CB of the largest array, two workgroups per CU).
+constexpr int SLOT_MAX_LDS_VALUES = 68 * 384 + 2 * 12288;
+constexpr int SERVE_LDS_VALUES = 68 * 384 + 2 * 2048;
+
+// LDS values a CB of TB parameters p needs in the fused kernels: the HARQ
+// circular buffer [2Zc | Ncb] and the decoder's APP array ((nsys + mb) * Zc)
+// share one array; descrambled CBs add their sequence words after it (two
+// values per word; words = (start + E - 1) / 32 - start / 32 + 1 of the
+// largest E, start % 32 at most 31, plus one).  The same per-CB split of G
+// into E as the de-rate-match kernel (TS 38.212 5.4.2.1).
+inline int slot_lds_values(const PerTbParams& p, int mb, bool descramble)
+{
+    const int nsys = (p.bg == 1) ? 22 : 10;
+    const int arr = std::max<int>((int)(2 * p.Zc + p.Ncb), (nsys + mb) * (int)p.Zc);
+    if (!descramble || p.uciOnPuschFlag) return arr;
+    const long long nq = (long long)p.Nl * p.Qm, C = p.num_CBs;
+    const long long total = p.encodedSize;
+    if (nq <= 0 || C <= 0) return arr;
+    const long long q = total / nq / C;
+    const long long emax = nq * q + (q * nq * C < total ? nq : 0);
+    const long long words = (31 + emax - 1) / 32 + 1;
+    return (int)std::min<long long>(((arr + 1) & ~1) + 2 * (words + 1), 1LL << 30);
+}
 
 // Resident serving launch (pusch_serve_resident): the host-written ring
 // header (host-mapped pinned memory; items follow in their own array) and

@@ -418,7 +418,8 @@ typedef struct {
     uint8_t* cb_iterations;          /* optional: iterations run per CB r, NULL for none */
     int16_t num_parity_nodes;        /* mb of the decoder, 4..46 (BG2: ..42) */
     int16_t max_iterations;
-    uint16_t flags;                  /* CUPHY_LDPC_DECODE_EARLY_TERM | CUPHY_HIP_SLOT_TB_NOTIFY */
+    uint16_t flags;                  /* CUPHY_LDPC_DECODE_EARLY_TERM | CUPHY_HIP_SLOT_TB_NOTIFY |
+                                        CUPHY_HIP_SLOT_TB_DESCRAMBLE */
     uint16_t reserved;
     float llr_scale;                 /* combined LLR = stored value * llr_scale, rounded to binary16
                                         (int8 LLRs with 4 steps per unit LLR: 0.25) */
@@ -430,6 +431,16 @@ typedef struct {
 /* cuphyHipPuschSlotTb_t.flags: publish the TB CRC as soon as the TB is decoded
  * (see cuphyHipPuschSlotDecode) */
 #define CUPHY_HIP_SLOT_TB_NOTIFY 0x200u
+/* cuphyHipPuschSlotTb_t.flags: descramble the LLRs with the TS 38.211 6.3.1.1
+ * Gold sequence of c_init = PerTbParams.cinit before combining (the
+ * reference's descramblingOn of cuphySetupPuschRxRateMatch,
+ * rate_matching.cu:276, 378-387; not for UCI-multiplexed TBs).  The CB's
+ * sequence words are held in LDS after its HARQ/APP values: max_lds_values
+ * must cover, per descrambled CB, align2(max(2Zc + Ncb, (nsys + mb) * Zc)) +
+ * 2 * (words + 1) with words = (start + E - 1) / 32 - start / 32 + 1 (start:
+ * index of the CB's first rate-matched bit in the TB); a CB that does not
+ * fit is rejected. */
+#define CUPHY_HIP_SLOT_TB_DESCRAMBLE 0x400u
 
 /* Workspace of cuphyHipPuschSlotDecode: queue state and num_cbs CB slots of
  * two words (TB t owns slots share_offset .. share_offset + num_CBs - 1 and
@@ -451,7 +462,9 @@ cuphyStatus_t CUPHYWINAPI cuphyHipPuschSlotDecodeGetWorkspaceSize(int num_cbs, s
  * the launch ends (the TB's HARQ buffer must still not be reused by another
  * launch before this one ends).  llr_type: CUPHY_R_8I,
  * CUPHY_R_16F or CUPHY_R_32F.  max_lds_values: max over TBs of
- * max(2Zc + Ncb, (nsys + mb) * Zc) (sizes LDS); max_parity_nodes: max mb
+ * max(2Zc + Ncb, (nsys + mb) * Zc), plus the sequence words of descrambled
+ * TBs (see CUPHY_HIP_SLOT_TB_DESCRAMBLE), at most 68 * 384 + 2 * 12288
+ * (sizes LDS); max_parity_nodes: max mb
  * (<= 8 selects the register-lean high-code-rate variant; a CB with a
  * larger mb is rejected). */
 cuphyStatus_t CUPHYWINAPI cuphyHipPuschSlotDecode(cuphyLDPCDecoder_t decoder, const cuphyHipPuschSlotTb_t* d_tbs,
@@ -500,7 +513,10 @@ typedef struct {
     uint8_t* cb_iterations;          /* optional, NULL for none */
     int16_t num_parity_nodes;        /* decoder mb: cover every position received so far */
     int16_t max_iterations;
-    uint16_t flags;                  /* CUPHY_LDPC_DECODE_EARLY_TERM | CUPHY_HIP_HARQ_LAST_TX */
+    uint16_t flags;                  /* CUPHY_LDPC_DECODE_EARLY_TERM | CUPHY_HIP_HARQ_LAST_TX |
+                                        CUPHY_HIP_SLOT_TB_DESCRAMBLE (c_init = prm.cinit; rejected when
+                                        the CB's sequence words exceed the launch's LDS: resident mode
+                                        E up to 65,473 bits per CB) */
     uint16_t reserved;
     float llr_scale;
     cuphyLDPCNormalization_t norm;   /* f16x2 */

@@ -260,6 +260,65 @@ def test_harq_buffer_timeout_eviction(dec, templates, resident):
     srv.close()
 
 
+@pytest.mark.parametrize("resident", [False, True])
+def test_descrambled_requests(dec, resident):
+    """Requests with CUPHY_HIP_SLOT_TB_DESCRAMBLE: the channel carries
+    scrambled bits (per-TB c_init); every TB passes its CRC and its outputs
+    equal the fused slot decoder's alone.  In resident mode a CB whose
+    sequence words exceed the launch's LDS budget is rejected at submit."""
+    import oracle as O
+    from aerial_amd import _lib as L
+    from aerial_amd.server import HarqServer, TraceTemplates
+    from aerial_amd.slot import SlotDecoder, SlotTb
+    print("[case] server: descrambled requests, resident=%s" % resident, flush=True)
+    specs = [(1, 384, 4, 8448, 8, 1, 9088, 12.0), (2, 104, 1, 1040, 4, 1, 2080, 8.0),
+             (1, 208, 3, 4576, 2, 1, 9152, 8.0)]
+    tt = TraceTemplates(specs, seed=11)
+    srv = HarqServer(dec, pool_bytes=1 << 30, max_inflight=2, max_tbs=16, max_cbs=256, resident=resident)
+    reqs = []
+    for i, tb in enumerate(tt.tbs):
+        p, _ = tb.params(0, 1)
+        p.cinit = (0x5A5A5A5 + 7919 * i) & 0x7FFFFFFF
+        c = O.gold(p.cinit, 0, tb.G).reshape(-1, tb.Qm)
+        sign = torch.ones((tb.G // tb.Qm, 8), dtype=torch.float16, device="cuda")
+        sign[:, :tb.Qm] = torch.from_numpy(1.0 - 2.0 * c.astype(np.float32)).to(sign)
+        llr = (tb.llrs[0].view(-1, 8) * sign).reshape(-1).contiguous()
+        reqs.append((p, llr))
+        o = SlotTb.alloc_outputs(p)
+        srv.submit(srv.request(300 + i, p, llr, o, tt.mbs[i][0], max_iterations=10, early_term=True, last=True,
+                               tag=i, descramble=True))
+        reqs[-1] += (o,)
+    if resident:
+        # a server that is never polled (nothing it accepted would launch)
+        from aerial_amd.slot import lds_values
+        chk = HarqServer(dec, pool_bytes=1 << 26, max_inflight=1, max_tbs=4, max_cbs=16, resident=True)
+        p, llr, o = reqs[1]
+        r = chk.request(399, p, llr, o, tt.mbs[1][0], descramble=True)
+        r.prm.encodedSize = r.prm.G = 500000  # 15,627 sequence words for its one CB
+        assert lds_values(r.prm, tt.mbs[1][0], True) > 68 * 384 + 2 * 2048
+        rc = L.lib().cuphyHipHarqServerSubmit(chk.h, r)
+        assert chk.stats()["submitted"] == 0
+        chk.close()
+        assert rc == L.CUPHY_STATUS_INVALID_ARGUMENT
+    done = {}
+    while len(done) < len(reqs):
+        for c in srv.poll():
+            done[c.tag] = c
+    srv.close()
+    for i, (p, llr, o) in enumerate(reqs):
+        assert done[i].status == 0 and done[i].tb_crc == 0, (i, done[i].status, done[i].tb_crc)
+        harq = torch.zeros(tt.tbs[i].C * p.Ncb_padded, dtype=torch.float16, device="cuda")
+        ref = SlotTb.alloc_outputs(p)
+        SlotDecoder(dec).build([SlotTb(p, llr, harq, tt.mbs[i][0], max_iterations=10, early_term=True,
+                                       outputs=ref, descramble=True)]).launch()
+        torch.cuda.synchronize()
+        nw = (tt.tbs[i].Kp + 31) // 32
+        assert torch.equal(o[0][:, :nw], ref[0][:, :nw]), i
+        for j in (1, 2, 4):
+            assert torch.equal(o[j], ref[j]), (i, j)
+        assert int(ref[3].item()) == 0, i
+
+
 def test_invalid_requests_rejected(dec, templates):
     from aerial_amd import _lib as L
     from aerial_amd.server import HarqServer

@@ -2,8 +2,8 @@
 NMS decoding + CB/TB CRC in one persistent launch, cuphyHipPuschSlotDecode):
 bit-exact against the oracle's three stages (HARQ buffers, decoded bits,
 iterations run, CB CRCs, TB bytes, TB CRCs) over two HARQ transmissions,
-for int8 / binary16 / binary32 input LLRs; equal to the unfused GPU
-pipeline; replayable from a graph."""
+for int8 / binary16 / binary32 input LLRs, with and without descrambling;
+equal to the unfused GPU pipeline; replayable from a graph."""
 import numpy as np
 import pytest
 
@@ -43,10 +43,12 @@ def half_to_float(h):
 class Case:
     """Device state and oracle state of one TB across transmissions."""
 
-    def __init__(self, k, spec, dtype):
+    def __init__(self, k, spec, dtype, descramble=False):
         from aerial_amd.pusch import make_tb_params
         bg, Zc, C, Kp, Qm, Nl, nbbu, E, snr = spec
         self.spec, self.dtype, self.k = spec, dtype, k
+        self.descramble = descramble
+        self.cinit = (0x2468ACE * (k + 1) + 0x1357) & 0x7FFFFFFF if descramble else 0
         self.tb = P.TbVector(bg, Zc, C, Kp, seed=700 + k)
         self.G = C * E
         assert E % (Qm * Nl) == 0
@@ -63,6 +65,8 @@ class Case:
         p, mb = make_tb_params(bg, Zc, C, Kp, Qm, Nl, self.G, rv, ndi=ndi, n_bbu_layers=nbbu, layer_map=self.lmap)
         self.mb = mb if ndi else max(self.mb, mb)
         llr = P.bpsk_llrs(self.tb.tx_bits(self.G, Qm, Nl, rv), snr, seed=9000 + 10 * self.k + tx)
+        if self.descramble:  # the channel carries the scrambled bits b(n) ^ c(n)
+            llr = np.where(O.gold(self.cinit, 0, self.G) != 0, -llr, llr).astype(np.float32)
         if self.dtype == "int8":
             q = P.quantize_int8(llr)
             seen = q.astype(np.float32) * 0.25
@@ -78,8 +82,10 @@ class Case:
                 seen = llr.astype(np.float16).astype(np.float32)
             scale = 1.0
         ocfg = O.rm_cfg(bg=bg, Zc=Zc, C=C, Qm=Qm, Nl=Nl, G=self.G, K=p.K, F=p.F, Ncb=p.Ncb,
-                        Ncb_padded=p.Ncb_padded, k0=p.k0, ndi=ndi)
+                        Ncb_padded=p.Ncb_padded, k0=p.k0, ndi=ndi, descramble=int(self.descramble),
+                        cinit=self.cinit)
         O.derate_match(ocfg, seen, self.ref_harq, fp16=True)
+        p.cinit = self.cinit
         self.p = p
         return p, dev, scale
 
@@ -97,18 +103,20 @@ class Case:
         return words, iters, cb_crc, tb_bytes, tb_crc
 
 
-def run_slot(dec, dtype, early_term, specs=SLOT):
+def run_slot(dec, dtype, early_term, specs=SLOT, descramble=False):
     from aerial_amd.slot import SlotDecoder, SlotTb
-    cases = [Case(k, s, dtype) for k, s in enumerate(specs)]
+    cases = [Case(k, s, dtype, descramble) for k, s in enumerate(specs)]
     # int8: one launch of the full variant; otherwise the high-rate CBs run
     # on the register-lean variant in a second launch
     sd = SlotDecoder(dec, split_min=256 if dtype == "int8" else 1)
     for tx, (ndi, rv) in enumerate(TX):
-        print("[case] slot %s tx %d ndi %d rv %d ET %d" % (dtype, tx, ndi, rv, early_term), flush=True)
+        print("[case] slot %s tx %d ndi %d rv %d ET %d descr %d" % (dtype, tx, ndi, rv, early_term, descramble),
+              flush=True)
         tbs = []
         for c in cases:
             p, dev, scale = c.transmission(ndi, rv, tx)
-            tbs.append(SlotTb(p, dev, c.harq, c.mb, max_iterations=ITERS, early_term=early_term, llr_scale=scale))
+            tbs.append(SlotTb(p, dev, c.harq, c.mb, max_iterations=ITERS, early_term=early_term, llr_scale=scale,
+                              descramble=descramble))
         sd.build(tbs).launch()
         torch.cuda.synchronize()
         assert all(int(w[:2].abs().sum().item()) == 0 for w in sd.workspaces), "queue not returned to zero"
@@ -140,16 +148,26 @@ def test_slot_early_termination_matches_oracle(dec):
     run_slot(dec, "fp16", early_term=True)
 
 
-def test_slot_decodes_payload_and_passes_crc(dec):
-    """At high SNR every TB passes its CRC and the TB bytes are the payload."""
+@pytest.mark.parametrize("dtype", ["fp16", "int8", "fp32"])
+def test_slot_descrambled_matches_oracle(dec, dtype):
+    """Descrambling (per-TB c_init) before combining, both combining paths
+    (whole symbols; repetition and layer-mapped LLRs), two transmissions."""
+    run_slot(dec, dtype, early_term=False, descramble=True)
+
+
+@pytest.mark.parametrize("descramble", [False, True])
+def test_slot_decodes_payload_and_passes_crc(dec, descramble):
+    """At high SNR every TB passes its CRC and the TB bytes are the payload
+    (descramble: the channel carries scrambled bits)."""
     from aerial_amd.slot import SlotDecoder, SlotTb
     specs = [(b, z, c, k, q, l, n, e, 8.0) for (b, z, c, k, q, l, n, e, _) in SLOT]
-    print("[case] slot fp16 high SNR payload check", flush=True)
-    cases = [Case(k, s, "fp16") for k, s in enumerate(specs)]
+    print("[case] slot fp16 high SNR payload check descr %d" % descramble, flush=True)
+    cases = [Case(k, s, "fp16", descramble) for k, s in enumerate(specs)]
     tbs = []
     for c in cases:
         p, dev, scale = c.transmission(1, 0, 0)
-        tbs.append(SlotTb(p, dev, c.harq, c.mb, max_iterations=ITERS, early_term=True, llr_scale=scale))
+        tbs.append(SlotTb(p, dev, c.harq, c.mb, max_iterations=ITERS, early_term=True, llr_scale=scale,
+                          descramble=descramble))
     SlotDecoder(dec).build(tbs).launch()
     torch.cuda.synchronize()
     for c, tb in zip(cases, tbs):
@@ -158,16 +176,17 @@ def test_slot_decodes_payload_and_passes_crc(dec):
         assert np.array_equal(payload, c.tb.a), c.spec
 
 
-def test_slot_equals_unfused_pipeline(dec):
+@pytest.mark.parametrize("descramble", [False, True])
+def test_slot_equals_unfused_pipeline(dec, descramble):
     """Fused result == de-rate-match kernel + TB decode + CRC kernels."""
     from aerial_amd import _lib as L
     from aerial_amd.decoder import LdpcDecodeConfig
     from aerial_amd.pusch import CrcDecode, DeRateMatch
     from aerial_amd.slot import SlotDecoder, SlotTb
-    print("[case] slot fused vs unfused fp16", flush=True)
-    cases = [Case(k, s, "fp16") for k, s in enumerate(SLOT)]
+    print("[case] slot fused vs unfused fp16 descr %d" % descramble, flush=True)
+    cases = [Case(k, s, "fp16", descramble) for k, s in enumerate(SLOT)]
     harq2 = [c.harq.clone() for c in cases]
-    rm = DeRateMatch(fp_config=3)
+    rm = DeRateMatch(fp_config=3, descrambling=descramble)
     crc = CrcDecode(reverse_bytes=True)
     sd = SlotDecoder(dec)
     for tx, (ndi, rv) in enumerate(TX):
@@ -175,7 +194,7 @@ def test_slot_equals_unfused_pipeline(dec):
         for c in cases:
             p, dev, scale = c.transmission(ndi, rv, tx)
             p.userGroupIndex = len(params)  # the RM API takes TB t's LLRs from input tensor t
-            tbs.append(SlotTb(p, dev, c.harq, c.mb, max_iterations=ITERS, llr_scale=scale))
+            tbs.append(SlotTb(p, dev, c.harq, c.mb, max_iterations=ITERS, llr_scale=scale, descramble=descramble))
             params.append(p)
             inputs.append(dev)
         sd.build(tbs).launch()
