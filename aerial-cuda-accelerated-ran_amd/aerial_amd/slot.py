@@ -28,12 +28,19 @@ def half_bits(x):
     return int(torch.tensor([x], dtype=torch.float16).view(torch.int16).item()) & 0xFFFF
 
 
-def lds_values(p, mb, descramble=False):
-    """LDS values a CB of TB parameters p needs: the HARQ circular buffer
-    and the decoder's APP array share one binary16 array; a descrambled CB
-    adds its Gold-sequence words after it (two values per word, for the
-    largest per-CB E starting at bit 31 of a word, plus one word)."""
-    arr = max(2 * p.Zc + p.Ncb, (nsys(p.bg) + mb) * p.Zc)
+FUSED_BLOCK = 384                    # threads of a fused-kernel workgroup
+FUSED_MAX_GROUP = 16                 # CBs of one work item at most
+SLOT_MAX_LDS_VALUES = 68 * 384 + 2 * 12288
+
+
+def lds_values(p, mb, descramble=False, count=1):
+    """LDS values a work item of `count` CBs of TB parameters p needs: per
+    CB, the HARQ circular buffer and the decoder's APP array share one
+    binary16 array (all but the last rounded up to whole 16-byte units); a
+    descrambled TB adds Gold-sequence words after them (two values per word,
+    for the largest per-CB E starting at bit 31 of a word, plus one word)."""
+    arr1 = max(2 * p.Zc + p.Ncb, (nsys(p.bg) + mb) * p.Zc)
+    arr = (count - 1) * ((arr1 + 7) & ~7) + arr1
     if not descramble or p.uciOnPuschFlag:
         return arr
     nq, C = p.Nl * p.Qm, p.num_CBs
@@ -41,6 +48,20 @@ def lds_values(p, mb, descramble=False):
     emax = nq * q + (nq if q * nq * C < p.encodedSize else 0)
     words = (31 + emax - 1) // 32 + 1
     return ((arr + 1) & ~1) + 2 * (words + 1)
+
+
+def group_size(p, mb, descramble=False, lds_limit=SLOT_MAX_LDS_VALUES):
+    """CBs per work item: consecutive CBs of the TB decoded side by side in
+    one workgroup (count * Zc <= FUSED_BLOCK), within lds_limit."""
+    n = min(FUSED_MAX_GROUP, FUSED_BLOCK // p.Zc, p.num_CBs)
+    while n > 1 and lds_values(p, mb, descramble, n) > lds_limit:
+        n -= 1
+    return max(n, 1)
+
+
+def make_item(t, cb, count=1):
+    """Work item: (TB << 16) | ((count - 1) << 12) | first CB."""
+    return (t << 16) | ((count - 1) << 12) | cb
 
 
 class SlotTb:
@@ -95,6 +116,7 @@ class _Launch:
         self.n_items = 0
         self.max_values = 0
         self.max_mb = 4
+        self.max_group = 1  # CBs of the largest item (1: the one-CB-per-workgroup kernel)
         self.workspace = None
 
 
@@ -103,10 +125,11 @@ class SlotDecoder:
     two streams: the mb <= 8 register-lean variant when at least split_min
     CBs (enough to fill the GPU) have mb <= 8, and the full variant."""
 
-    def __init__(self, decoder, split_min=256, concurrent=True):
+    def __init__(self, decoder, split_min=256, concurrent=True, pack=True):
         self.dec = decoder
         self.split_min = split_min
         self.concurrent = concurrent
+        self.pack = pack  # consecutive CBs of a TB side by side in one workgroup (small Zc)
         self._lib = L.lib()
         self.launches = []
         self.graph = None
@@ -171,8 +194,10 @@ class SlotDecoder:
             share_offset += p.num_CBs
             cost = p.Zc * num_edges(p.bg, tb.mb) * max(1, tb.max_iterations)
             key = (tb.mb <= SMALL_MAXR) if split else False
-            for c in range(p.num_CBs):
-                classes.setdefault(key, []).append((cost, t, c))
+            g = group_size(p, tb.mb, tb.descramble) if self.pack else 1
+            for c in range(0, p.num_CBs, g):
+                n_cb = min(g, p.num_CBs - c)
+                classes.setdefault(key, []).append((cost * n_cb, t, c, n_cb))
         self._jobs = torch.from_numpy(np.frombuffer(bytes(jobs), dtype=np.uint8).copy()).cuda()
         self.n_tbs = n
         sz = C.c_size_t()
@@ -184,11 +209,12 @@ class SlotDecoder:
             if order == "lpt":
                 rows.sort(key=lambda r: -r[0])
             ln = _Launch()
-            ids = np.array([(t << 16) | c for _, t, c in rows], dtype=np.uint32).view(np.int32)
+            ids = np.array([make_item(t, c, n_cb) for _, t, c, n_cb in rows], dtype=np.uint32).view(np.int32)
             ln.items = torch.from_numpy(ids).cuda()
             ln.n_items = len(rows)
-            for _, t, _ in rows:
-                ln.max_values = max(ln.max_values, lds_values(tbs[t].params, tbs[t].mb, tbs[t].descramble))
+            for _, t, _, n_cb in rows:
+                ln.max_values = max(ln.max_values, lds_values(tbs[t].params, tbs[t].mb, tbs[t].descramble, n_cb))
+                ln.max_group = max(ln.max_group, n_cb)
                 ln.max_mb = max(ln.max_mb, tbs[t].mb)
             ln.workspace = torch.zeros(sz.value // 4, dtype=torch.int32, device="cuda")
             self.launches.append(ln)
@@ -202,7 +228,8 @@ class SlotDecoder:
         n_items = ln.n_items if n_items is None else n_items
         st = self._lib.cuphyHipPuschSlotDecode(self.dec.dec, C.c_void_p(self._jobs.data_ptr()), self.n_tbs,
                                                C.c_void_p(ln.items.data_ptr()), n_items, self.llr_type,
-                                               ln.max_values, ln.max_mb, C.c_void_p(ln.workspace.data_ptr()),
+                                               ln.max_values, ln.max_mb, ln.max_group,
+                                               C.c_void_p(ln.workspace.data_ptr()),
                                                _stream_handle(stream))
         return L.check("cuphyHipPuschSlotDecode", st)
 

@@ -617,11 +617,11 @@ cuphyStatus_t CUPHYWINAPI cuphyHipPuschSlotDecodeGetWorkspaceSize(int num_cbs, s
 cuphyStatus_t CUPHYWINAPI cuphyHipPuschSlotDecode(cuphyLDPCDecoder_t decoder, const cuphyHipPuschSlotTb_t* d_tbs,
                                                   int num_tbs, const uint32_t* d_cb_items, int num_cb_items,
                                                   cuphyDataType_t llr_type, int max_lds_values, int max_parity_nodes,
-                                                  void* d_workspace, hipStream_t strm)
+                                                  int max_cbs_per_item, void* d_workspace, hipStream_t strm)
 {
     if (!decoder) return CUPHY_STATUS_INVALID_ARGUMENT;
     return nrldpc::slot_decode(d_tbs, num_tbs, d_cb_items, num_cb_items, llr_type, max_lds_values, max_parity_nodes,
-                               d_workspace, decoder->cu_count, strm);
+                               max_cbs_per_item, d_workspace, decoder->cu_count, strm);
 }
 
 }  // extern "C"

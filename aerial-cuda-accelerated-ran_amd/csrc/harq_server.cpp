@@ -428,8 +428,13 @@ cuphyStatus_t cuphyHipHarqServer::admit_resident()
         j.clamp_value = r.clamp_value;
         j.share_offset = (uint32_t)k;  // accumulator pair of slot k
         h_tbcrc[k] = CRC_PENDING;
-        for (uint32_t c = 0; c < C; ++c) h_items[(published + c) & (ring_size - 1)] = ((uint32_t)k << 16) | c;
-        published += C;
+        // groups of consecutive CBs decoded side by side in one workgroup
+        const uint32_t P = (uint32_t)nrldpc::fused_group_size(
+            r.prm, r.num_parity_nodes, (r.flags & CUPHY_HIP_SLOT_TB_DESCRAMBLE) != 0, nrldpc::SERVE_LDS_VALUES);
+        uint32_t n_items = 0;
+        for (uint32_t c = 0; c < C; c += P)
+            h_items[(published + n_items++) & (ring_size - 1)] = nrldpc::make_item(k, (int)c, (int)std::min(P, C - c));
+        published += n_items;
         items_open += C;
         slot_req[k] = q;
         slot_open[k] = 1;
@@ -569,7 +574,7 @@ cuphyStatus_t cuphyHipHarqServer::launch(launch_slot& s)
         uint32_t id;
     };
     std::vector<cost_item> small, big;
-    int lds_small = 1, lds_big = 1, mb_small = 4, mb_big = 4;
+    int lds_small = 1, lds_big = 1, mb_small = 4, mb_big = 4, g_small = 1, g_big = 1;
     uint32_t share_offset = 0;
     for (int t = 0; t < n; ++t) {
         const cuphyHipHarqRequest_t& r = s.reqs[t].req;
@@ -597,15 +602,23 @@ cuphyStatus_t cuphyHipHarqServer::launch(launch_slot& s)
         const PerTbParams& p = r.prm;
         const int mb = r.num_parity_nodes;
         const uint64_t cost = (uint64_t)p.Zc * (uint64_t)num_edges((int)p.bg, mb) * (uint64_t)std::max(1, (int)r.max_iterations);
-        const int lds = nrldpc::slot_lds_values(p, mb, (r.flags & CUPHY_HIP_SLOT_TB_DESCRAMBLE) != 0);
+        const bool descr = (r.flags & CUPHY_HIP_SLOT_TB_DESCRAMBLE) != 0;
+        const int P = nrldpc::fused_group_size(p, mb, descr, nrldpc::SLOT_MAX_LDS_VALUES);
+        const int lds = nrldpc::slot_lds_values(p, mb, descr, std::min<int>(P, (int)p.num_CBs));
         auto& dst = (mb <= SMALL_MB) ? small : big;
-        for (uint32_t c = 0; c < p.num_CBs; ++c) dst.push_back({cost, ((uint32_t)t << 16) | c});
+        for (int c = 0; c < (int)p.num_CBs; c += P) {
+            const int n = std::min<int>(P, (int)p.num_CBs - c);
+            dst.push_back({cost * (uint64_t)n, nrldpc::make_item(t, c, n)});
+        }
+        const int g = std::min<int>(P, (int)p.num_CBs);
         if (mb <= SMALL_MB) {
             lds_small = std::max(lds_small, lds);
             mb_small = std::max(mb_small, mb);
+            g_small = std::max(g_small, g);
         } else {
             lds_big = std::max(lds_big, lds);
             mb_big = std::max(mb_big, mb);
+            g_big = std::max(g_big, g);
         }
     }
     // high-rate CBs run on the register-lean variant only when they fill the GPU
@@ -614,6 +627,7 @@ cuphyStatus_t cuphyHipHarqServer::launch(launch_slot& s)
         small.clear();
         lds_big = std::max(lds_big, lds_small);
         mb_big = std::max(mb_big, mb_small);
+        g_big = std::max(g_big, g_small);
     }
     auto by_cost = [](const cost_item& a, const cost_item& b) { return a.cost > b.cost; };
     std::stable_sort(small.begin(), small.end(), by_cost);
@@ -629,10 +643,10 @@ cuphyStatus_t cuphyHipHarqServer::launch(launch_slot& s)
     const uint32_t* d_items = reinterpret_cast<const uint32_t*>(s.d_stage + items_off());
     cuphyStatus_t st = CUPHY_STATUS_SUCCESS;
     if (!small.empty())
-        st = nrldpc::slot_decode(d_jobs, n, d_items, (int)small.size(), cfg.llr_type, lds_small, mb_small, s.d_ws[0],
-                                 cu_count, s.stream[0]);
+        st = nrldpc::slot_decode(d_jobs, n, d_items, (int)small.size(), cfg.llr_type, lds_small, mb_small, g_small,
+                                 s.d_ws[0], cu_count, s.stream[0]);
     if (st == CUPHY_STATUS_SUCCESS && !big.empty())
-        st = nrldpc::slot_decode(d_jobs, n, d_items + small.size(), (int)big.size(), cfg.llr_type, lds_big, mb_big,
+        st = nrldpc::slot_decode(d_jobs, n, d_items + small.size(), (int)big.size(), cfg.llr_type, lds_big, mb_big, g_big,
                                  s.d_ws[1], cu_count, s.stream[1]);
     if (st != CUPHY_STATUS_SUCCESS) return st;
     for (int q = 0; q < 2; ++q)
@@ -744,7 +758,7 @@ cuphyStatus_t CUPHYWINAPI cuphyHipHarqServerSubmit(cuphyHipHarqServer_t server, 
     if (!server || !req) return CUPHY_STATUS_INVALID_ARGUMENT;
     const PerTbParams& p = req->prm;
     const int bg = (int)p.bg;
-    if ((bg != 1 && bg != 2) || !is_lifting((int)p.Zc) || p.num_CBs < 1 || p.num_CBs > 65535 || p.Qm < 1 ||
+    if ((bg != 1 && bg != 2) || !is_lifting((int)p.Zc) || p.num_CBs < 1 || p.num_CBs > 4096 || p.Qm < 1 ||
         p.Qm > 8 || p.Nl < 1 || p.Ncb < 1 || p.Ncb > (uint32_t)(bg == 1 ? 66 : 50) * p.Zc ||
         p.Ncb_padded < 2 * p.Zc + p.Ncb || p.K != (uint32_t)nsys_of(bg) * p.Zc || p.F >= p.K)
         return CUPHY_STATUS_INVALID_ARGUMENT;

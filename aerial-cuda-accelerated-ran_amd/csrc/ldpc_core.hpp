@@ -279,9 +279,10 @@ template <> __device__ __forceinline__ _Float16 job_alpha<_Float16>(const cb_job
 // app + c*stride.  With early termination every code block stops on its own
 // syndrome (its lanes freeze, exactly as if decoded alone); `cb_flags` is
 // LDS scratch of `count` words for that (unused when count == 1).
+// cb_iters (optional, LDS, `count` words): iterations each code block ran.
 template <int BG, typename T, int MAXR>
 __device__ __forceinline__ int run_iterations(T* __restrict__ app, const cb_job& job, int count = 1, int stride = 0,
-                                              uint32_t* cb_flags = nullptr)
+                                              uint32_t* cb_flags = nullptr, uint32_t* cb_iters = nullptr)
 {
     using BGT = bg_traits<BG>;
     constexpr int NR = (MAXR < BGT::NROWS) ? MAXR : BGT::NROWS;
@@ -318,6 +319,7 @@ __device__ __forceinline__ int run_iterations(T* __restrict__ app, const cb_job&
     int it = 0;
     while (it < job.max_iter) {
         ++it;
+        if (cb_iters && active && zlane == 0u) cb_iters[cb] = (uint32_t)it;
         // Opaque copies of the lane offset and of the shift pointer, renewed
         // per row: the 316 (BG1) per-edge shifts and addresses are reloaded
         // and recomputed where used instead of being hoisted out of the

@@ -57,6 +57,8 @@ struct fused_lds {
     uint32_t crc_t[3][256];              // byte tables of the three CRCs
     uint32_t shift[3][CRC_SHIFT_LO];     // x^(8m) mod g of the three CRCs
     uint32_t red[2][8];
+    uint32_t cb_flags[FUSED_MAX_GROUP];  // per-CB early-termination flags of a group
+    uint32_t cb_iters[FUSED_MAX_GROUP];  // iterations each CB of a group ran
     int item, next;
 };
 
@@ -224,42 +226,24 @@ __device__ __forceinline__ void combine_symbols(_Float16* __restrict__ cbuf, con
     __syncthreads();
 }
 
-// One CB.  Returns, in every thread, this CB's share of the TB CRC24A (the
-// remainder of its payload bytes shifted to their place in the TB; for a
-// one-CB TB its CRC, which is the TB CRC), or TB_ACC_ERROR when the CB's
-// parameters do not fit the launch (then only the error marker is written,
-// into its CRC).  The TB CRC itself is written by complete_cb.
-template <typename TI, int MAXR>
-__device__ __forceinline__ uint32_t fused_cb(_Float16* __restrict__ hb, fused_lds& sh,
-                                             const cuphyHipPuschSlotTb_t& job, const PerTbParams& p, int cb_in_tb,
-                                             int lds_values)
+// Words of the descrambling sequence covering a CB's rate-matched bits
+// [start, start + E): from word start / 32 to word (start + E - 1) / 32.
+__device__ __forceinline__ int rm_seq_words(const rm_geometry& g)
 {
-    rm_geometry g = rm_geometry_of(p, cb_in_tb);
-    rm_make_uniform(g);
+    return (int)(((uint32_t)(g.start + g.E - 1) >> 5) - ((uint32_t)g.start >> 5) + 1u);
+}
+
+// Steps 1-3 of one CB: its HARQ circular buffer into LDS `hb`, this
+// transmission combined (descrambled with `seq` when descr), the buffer
+// written back to HBM and converted in place to the decoder's APP array of
+// nval values.  Ends with a barrier.
+template <typename TI>
+__device__ __forceinline__ void fused_combine(_Float16* __restrict__ hb, uint32_t* __restrict__ seq,
+                                              const cuphyHipPuschSlotTb_t& job, const PerTbParams& p,
+                                              const rm_geometry& g, int r, int nh, int nval, bool descr)
+{
     const int Zc = g.Zc;
-    const int bg = __builtin_amdgcn_readfirstlane((int)p.bg);
-    const int nsys = bg == 1 ? 22 : 10;
-    const int mb = job.num_parity_nodes;
-    const int nh = 2 * Zc + g.Ncb;
-    const int nval = (nsys + mb) * Zc;
-    const int r = cb_in_tb + __builtin_amdgcn_readfirstlane((int)p.firstCodeBlockIndex);
-    const int Kb = __builtin_amdgcn_readfirstlane((int)p.K) / Zc;
-    const int C = __builtin_amdgcn_readfirstlane((int)p.num_CBs);
     const size_t ncb_padded = (size_t)__builtin_amdgcn_readfirstlane((int)p.Ncb_padded);
-    // descrambling: words w_base .. w_base + nw - 1 of c (covering the CB's
-    // bits [start, start + E)) and one more word, after the HARQ/APP values
-    const bool descr = (job.flags & CUPHY_HIP_SLOT_TB_DESCRAMBLE) != 0 && !g.uci;
-    const int seq_at = (max(nh, nval) + 1) & ~1;
-    const uint32_t w_base = (uint32_t)g.start >> 5;
-    const int nw = descr && g.E > 0 ? (int)(((uint32_t)(g.start + g.E - 1) >> 5) - w_base + 1u) : 0;
-    const bool ok = (bg == 1 || bg == 2) && z_index(Zc) >= 0 && mb >= 4 &&
-                    mb <= (bg == 1 ? 46 : 42) && mb <= MAXR && nh <= lds_values && nval <= lds_values &&
-                    Kb * Zc <= 32 * (int)SLOT_CB_WORDS && g.L > 0 && g.E > 0 &&
-                    (!descr || seq_at + 2 * (nw + 1) <= lds_values);
-    if (!ok) {
-        if (threadIdx.x == 0) job.cb_crc[r] = FUSED_CB_ERROR;
-        return TB_ACC_ERROR;
-    }
     _Float16* harq = static_cast<_Float16*>(job.harq) + (size_t)r * ncb_padded;
     const TI* in = static_cast<const TI*>(job.llr);
     _Float16* cbuf = hb + 2 * Zc;
@@ -285,8 +269,9 @@ __device__ __forceinline__ uint32_t fused_cb(_Float16* __restrict__ hb, fused_ld
         }
         for (int n = nv * 8 + tid; n < nh; n += nthr) hb[n] = harq[n];
     }
-    uint32_t* seq = reinterpret_cast<uint32_t*>(hb + seq_at);
+    const uint32_t w_base = (uint32_t)g.start >> 5;
     if (descr) {  // each wave generates its share of the words while the loads land
+        const int nw = rm_seq_words(g);
         const int nwave = nthr >> 6, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
         const int per = (nw + nwave - 1) / nwave;
         const int a = min(nw, wave * per), b = min(nw, a + per);
@@ -314,39 +299,103 @@ __device__ __forceinline__ uint32_t fused_cb(_Float16* __restrict__ hb, fused_ld
     }
 
     // ---- 3. write back the HARQ buffer; APP = clamp(HARQ), +0 for -0 ----
-    {
-        const _Float16 cl = (_Float16)job.clamp_value;
-        const bool do_clamp = job.clamp_value > 0.0f;
-        for (int n = tid; n < nv; n += nthr) {
-            half8_t v = reinterpret_cast<const half8_t*>(hb)[n];
-            reinterpret_cast<half8_t*>(harq)[n] = v;
-            if (n * 8 < nval) {
+    const _Float16 cl = (_Float16)job.clamp_value;
+    const bool do_clamp = job.clamp_value > 0.0f;
+    for (int n = tid; n < nv; n += nthr) {
+        half8_t v = reinterpret_cast<const half8_t*>(hb)[n];
+        reinterpret_cast<half8_t*>(harq)[n] = v;
+        if (n * 8 < nval) {
 #pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    _Float16 a = v[e];
-                    if (do_clamp) a = __builtin_fmaxf16(__builtin_fminf16(a, cl), -cl);
-                    v[e] = (a == (_Float16)0) ? (_Float16)0 : a;
-                }
-                reinterpret_cast<half8_t*>(hb)[n] = v;  // values past nval are not read by the decoder
-            }
-        }
-        for (int n = nv * 8 + tid; n < nh; n += nthr) {
-            const _Float16 v = hb[n];
-            harq[n] = v;
-            if (n < nval) {
-                _Float16 a = v;
+            for (int e = 0; e < 8; ++e) {
+                _Float16 a = v[e];
                 if (do_clamp) a = __builtin_fmaxf16(__builtin_fminf16(a, cl), -cl);
-                hb[n] = (a == (_Float16)0) ? (_Float16)0 : a;
+                v[e] = (a == (_Float16)0) ? (_Float16)0 : a;
             }
+            reinterpret_cast<half8_t*>(hb)[n] = v;  // values past nval are not read by the decoder
         }
-        for (int n = nh + tid; n < nval; n += nthr) hb[n] = (_Float16)0.0f;
     }
+    for (int n = nv * 8 + tid; n < nh; n += nthr) {
+        const _Float16 v = hb[n];
+        harq[n] = v;
+        if (n < nval) {
+            _Float16 a = v;
+            if (do_clamp) a = __builtin_fmaxf16(__builtin_fminf16(a, cl), -cl);
+            hb[n] = (a == (_Float16)0) ? (_Float16)0 : a;
+        }
+    }
+    for (int n = nh + tid; n < nval; n += nthr) hb[n] = (_Float16)0.0f;
     __syncthreads();
+}
+
+// A group of `count` consecutive CBs cb0 .. cb0 + count - 1 of one TB
+// (count * Zc <= blockDim.x).  CB c's HARQ/APP array is at hb + c * S
+// (S = max(2Zc + Ncb, (nsys + mb) * Zc), rounded up to whole 16-byte units
+// when count > 1); descrambling words follow the last array.  Steps 1-3
+// run CB by CB with the whole workgroup; the decode runs the group side by
+// side (lanes [c * Zc, (c + 1) * Zc) on CB c, each CB stopping on its own
+// syndrome: results as decoded alone); hard decisions and CRCs again CB by
+// CB.  Returns, in every thread, the XOR of the CBs' shares of the TB
+// CRC24A (each the remainder of the CB's payload bytes shifted to their
+// place in the TB; for a one-CB TB its CRC, which is the TB CRC), or
+// TB_ACC_ERROR when the group's parameters do not fit the launch (then only
+// the error marker is written, into the CBs' CRCs).  The TB CRC itself is
+// written by complete_cb.  GROUPED = false: the one-CB form (count = 1 at
+// compile time, no group state; an item of several CBs is rejected).
+template <typename TI, int MAXR, bool GROUPED>
+__device__ __forceinline__ uint32_t fused_group(_Float16* __restrict__ hb, fused_lds& sh,
+                                                const cuphyHipPuschSlotTb_t& job, const PerTbParams& p, int cb0,
+                                                int count_in, int lds_values)
+{
+    const int count = GROUPED ? count_in : 1;
+    rm_geometry g0 = rm_geometry_of(p, cb0);
+    rm_make_uniform(g0);
+    const int Zc = g0.Zc;
+    const int bg = __builtin_amdgcn_readfirstlane((int)p.bg);
+    const int nsys = bg == 1 ? 22 : 10;
+    const int mb = job.num_parity_nodes;
+    const int nh = 2 * Zc + g0.Ncb;
+    const int nval = (nsys + mb) * Zc;
+    const int arr = max(nh, nval);
+    const int S = (count == 1) ? arr : group_stride(arr);
+    const int r0 = cb0 + __builtin_amdgcn_readfirstlane((int)p.firstCodeBlockIndex);
+    const int Kb = __builtin_amdgcn_readfirstlane((int)p.K) / Zc;
+    const int C = __builtin_amdgcn_readfirstlane((int)p.num_CBs);
+    // descrambling: the CB's words of c (and one more word) after the arrays
+    const bool descr = (job.flags & CUPHY_HIP_SLOT_TB_DESCRAMBLE) != 0 && !g0.uci;
+    const int seq_at = ((count - 1) * S + arr + 1) & ~1;
+    // every CB of the group: E > 0 and (descrambled) its words within LDS
+    int nw_max = 0;
+    bool e_ok = g0.L > 0;
+    for (int c = 0; c < count; ++c) {
+        const rm_geometry g = rm_geometry_of(p, cb0 + c);
+        e_ok = e_ok && g.E > 0;
+        if (descr && g.E > 0) nw_max = max(nw_max, rm_seq_words(g));
+    }
+    const bool ok = (bg == 1 || bg == 2) && z_index(Zc) >= 0 && mb >= 4 && mb <= (bg == 1 ? 46 : 42) &&
+                    mb <= MAXR && count == count_in && count >= 1 && count <= FUSED_MAX_GROUP &&
+                    count * Zc <= (int)blockDim.x &&
+                    (count - 1) * S + arr <= lds_values && Kb * Zc <= 32 * (int)SLOT_CB_WORDS && e_ok &&
+                    (!descr || seq_at + 2 * (nw_max + 1) <= lds_values);
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    if (!ok) {
+        if (tid < count_in) job.cb_crc[r0 + tid] = FUSED_CB_ERROR;
+        return TB_ACC_ERROR;
+    }
+    uint32_t* seq = reinterpret_cast<uint32_t*>(hb + seq_at);
+
+    if (tid < count) sh.cb_iters[tid] = 0u;  // (read after the barriers below)
+
+    // ---- 1-3. per CB: HARQ buffer, combining, write-back, APP --------------
+    for (int c = 0; c < count; ++c) {
+        rm_geometry g = rm_geometry_of(p, cb0 + c);
+        rm_make_uniform(g);
+        fused_combine<TI>(hb + c * S, seq, job, p, g, r0 + c, nh, nval, descr);
+    }
 
     // ---- 4. decode -------------------------------------------------------
     cb_job dj;
     dj.llr = nullptr;
-    dj.out = job.cb_bits + (size_t)r * SLOT_CB_WORDS;
+    dj.out = nullptr;
     dj.soft = nullptr;
     dj.Z = Zc;
     dj.mb = mb;
@@ -356,51 +405,60 @@ __device__ __forceinline__ uint32_t fused_cb(_Float16* __restrict__ hb, fused_ld
     dj.clamp = job.clamp_value;
     dj.alpha_f32 = job.norm.f32;
     dj.alpha_f16 = (uint16_t)(job.norm.f16x2 & 0xFFFFu);
-    const int iters = (bg == 1) ? run_iterations<1, _Float16, MAXR>(hb, dj) : run_iterations<2, _Float16, MAXR>(hb, dj);
-    store_hard_words<_Float16>(hb, Kb * Zc, dj.out, sh.words);
-    if (job.cb_iterations && tid == 0) job.cb_iterations[r] = (uint8_t)iters;
+    uint32_t* const flags = (count > 1) ? sh.cb_flags : nullptr;
+    uint32_t* const cb_it = (count > 1) ? sh.cb_iters : nullptr;
+    const int iters = (bg == 1) ? run_iterations<1, _Float16, MAXR>(hb, dj, count, S, flags, cb_it)
+                                : run_iterations<2, _Float16, MAXR>(hb, dj, count, S, flags, cb_it);
 
-    // ---- 5. CB CRC over ceil(K'/32) words and the TB-CRC share ------------
+    // ---- 5-6. per CB: hard decisions, CB CRC, TB bytes, TB-CRC share --------
     const int Kp = (int)(p.K - p.F);
-    const uint32_t* words = sh.words;
-    auto byte_of = [&](int b) { return (rbit8x4(words[b >> 2]) >> (8 * (b & 3))) & 0xFFu; };
     const int kind = cb_crc_kind(p);
     const int data_bytes = cb_data_bytes(p);
-    uint32_t crc, part;
-    if (kind == K_CRC16)
-        cb_and_share_crc<16, CRC16_P.g>(sh, sh.crc_t[kind], sh.shift[kind], 4 * ((Kp + 31) / 32), data_bytes,
-                                        sh.red[0], sh.red[1], crc, part);
-    else if (kind == K_CRC24B)
-        cb_and_share_crc<24, CRC24B_P.g>(sh, sh.crc_t[kind], sh.shift[kind], 4 * ((Kp + 31) / 32), data_bytes,
-                                         sh.red[0], sh.red[1], crc, part);
-    else
-        cb_and_share_crc<24, CRC24A_P.g>(sh, sh.crc_t[kind], sh.shift[kind], 4 * ((Kp + 31) / 32), data_bytes,
-                                         sh.red[0], sh.red[1], crc, part);
-    for (int b = tid; b < data_bytes; b += nthr) job.tb_bytes[(size_t)r * data_bytes + b] = (uint8_t)byte_of(b);
-    if (tid == 0) {
-        job.cb_crc[r] = crc;
-        if (cb_in_tb == 0) {
-            const int bytes = C * data_bytes;
-            for (int b = bytes; b < (bytes + 3) / 4 * 4; ++b) job.tb_bytes[b] = 0;
+    const uint32_t* words = sh.words;
+    auto byte_of = [&](int b) { return (rbit8x4(words[b >> 2]) >> (8 * (b & 3))) & 0xFFu; };
+    uint32_t acc = 0;
+    for (int c = 0; c < count; ++c) {
+        const int r = r0 + c, cb_in_tb = cb0 + c;
+        store_hard_words<_Float16>(hb + c * S, Kb * Zc, job.cb_bits + (size_t)r * SLOT_CB_WORDS, sh.words);
+        if (job.cb_iterations && tid == 0) job.cb_iterations[r] = (uint8_t)(count == 1 ? iters : (int)sh.cb_iters[c]);
+        uint32_t crc, part;
+        if (kind == K_CRC16)
+            cb_and_share_crc<16, CRC16_P.g>(sh, sh.crc_t[kind], sh.shift[kind], 4 * ((Kp + 31) / 32), data_bytes,
+                                            sh.red[0], sh.red[1], crc, part);
+        else if (kind == K_CRC24B)
+            cb_and_share_crc<24, CRC24B_P.g>(sh, sh.crc_t[kind], sh.shift[kind], 4 * ((Kp + 31) / 32), data_bytes,
+                                             sh.red[0], sh.red[1], crc, part);
+        else
+            cb_and_share_crc<24, CRC24A_P.g>(sh, sh.crc_t[kind], sh.shift[kind], 4 * ((Kp + 31) / 32), data_bytes,
+                                             sh.red[0], sh.red[1], crc, part);
+        for (int b = tid; b < data_bytes; b += nthr) job.tb_bytes[(size_t)r * data_bytes + b] = (uint8_t)byte_of(b);
+        if (tid == 0) {
+            job.cb_crc[r] = crc;
+            if (cb_in_tb == 0) {
+                const int bytes = C * data_bytes;
+                for (int b = bytes; b < (bytes + 3) / 4 * 4; ++b) job.tb_bytes[b] = 0;
+            }
         }
+        if (C == 1) return crc;
+        // TB CRC = XOR over CBs of crc(payload bytes of CB c) * x^(8 * TB
+        // bytes after them), the TB zero-padded to whole words (CRC linearity)
+        const uint32_t total = (uint32_t)((C * data_bytes + 3) / 4 * 4);
+        const uint32_t after = total - (uint32_t)(cb_in_tb + 1) * (uint32_t)data_bytes;
+        if (part && after)
+            part = cmulmod<24, CRC24A_P.g>(part, cmulmod<24, CRC24A_P.g>(sh.shift[K_CRC24A][after & 1023u],
+                                                                         g_crc_shift.tb_hi[(after >> 10) & 255u]));
+        acc ^= part;
+        if (c + 1 < count) __syncthreads();  // the words in LDS are the next CB's
     }
-    if (C == 1) return crc;
-    // ---- 6. this CB's share of the TB CRC24A --------------------------------
-    // TB CRC = XOR over CBs of crc(payload bytes of CB c) * x^(8 * TB bytes
-    // after them), the TB zero-padded to whole words (CRC linearity)
-    const uint32_t total = (uint32_t)((C * data_bytes + 3) / 4 * 4);
-    const uint32_t after = total - (uint32_t)(cb_in_tb + 1) * (uint32_t)data_bytes;
-    if (part && after)
-        part = cmulmod<24, CRC24A_P.g>(part, cmulmod<24, CRC24A_P.g>(sh.shift[K_CRC24A][after & 1023u],
-                                                                     g_crc_shift.tb_hi[(after >> 10) & 255u]));
-    return part;
+    return acc;
 }
 
 // Workspace words of a TB (after the queue words): TB-CRC accumulator at
 // 2 * share_offset, count of finished CBs at 2 * share_offset + 1.  Called
-// by one thread once the whole workgroup has finished CB `cb` (all of its
-// outputs issued).  v: the CB's share (TB_ACC_ERROR: rejected CB).
-__device__ __forceinline__ void complete_cb(const cuphyHipPuschSlotTb_t& job, uint32_t C, uint32_t v,
+// by one thread once the whole workgroup has finished a group of n CBs of
+// the TB (all of their outputs issued).  v: the XOR of the group's shares
+// (TB_ACC_ERROR: rejected group).
+__device__ __forceinline__ void complete_cb(const cuphyHipPuschSlotTb_t& job, uint32_t C, uint32_t v, uint32_t n,
                                             uint32_t* __restrict__ tbw)
 {
     const bool notify = (job.flags & CUPHY_HIP_SLOT_TB_NOTIFY) != 0;
@@ -417,7 +475,7 @@ __device__ __forceinline__ void complete_cb(const cuphyHipPuschSlotTb_t& job, ui
         // the share is in the accumulator (its old value returned) before
         // the count is raised
         asm volatile("" : : "v"(prev) : "memory");
-        if (__hip_atomic_fetch_add(acc + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != C - 1u) return;
+        if (__hip_atomic_fetch_add(acc + 1, n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != C - n) return;
         const uint32_t a = __hip_atomic_exchange(acc, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(acc + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         tb_crc = (a & 0x80000000u) ? FUSED_CB_ERROR : a;
@@ -433,7 +491,7 @@ __device__ __forceinline__ void complete_cb(const cuphyHipPuschSlotTb_t& job, ui
 // each TB uses the pair at its first slot (TB-CRC accumulator and count of
 // finished CBs; see complete_cb).  The next item index is fetched while the current CB is
 // worked on.  The queue is returned to 0 by the last workgroup to leave.
-template <typename TI, int MAXR>
+template <typename TI, int MAXR, bool GROUPED>
 __global__ __launch_bounds__(384) void pusch_slot_fused(const cuphyHipPuschSlotTb_t* __restrict__ tbs, int n_tbs,
                                                         const uint32_t* __restrict__ items, int n_items,
                                                         int lds_values, uint32_t* __restrict__ ws)
@@ -451,16 +509,16 @@ __global__ __launch_bounds__(384) void pusch_slot_fused(const cuphyHipPuschSlotT
         const int i = sh.item;
         if (i >= n_items) break;
         if (threadIdx.x == 0) sh.next = (int)atomicAdd(&queue[0], 1u);  // consumed after this CB
-        const uint32_t it = items[i];
-        const int t = (int)(it >> 16), cb = (int)(it & 0xFFFFu);
+        const uint32_t it = (uint32_t)__builtin_amdgcn_readfirstlane((int)items[i]);
+        const int t = (int)(it >> 16), cb = item_cb(it), n = item_count(it);
         if (t < n_tbs) {
             const cuphyHipPuschSlotTb_t& job = tbs[t];
             const PerTbParams& p = *job.prm;
             const uint32_t C = (uint32_t)__builtin_amdgcn_readfirstlane((int)p.num_CBs);
-            if ((uint32_t)cb < C) {
-                const uint32_t part = fused_cb<TI, MAXR>(hb, sh, job, p, cb, lds_values);
-                __syncthreads();  // every output of the CB issued by every thread
-                if (threadIdx.x == 0) complete_cb(job, C, part, tbw);
+            if ((uint32_t)(cb + n) <= C) {
+                const uint32_t part = fused_group<TI, MAXR, GROUPED>(hb, sh, job, p, cb, n, lds_values);
+                __syncthreads();  // every output of the group issued by every thread
+                if (threadIdx.x == 0) complete_cb(job, C, part, (uint32_t)n, tbw);
             }
         }
         __syncthreads();
@@ -553,9 +611,9 @@ __global__ __launch_bounds__(384) void pusch_serve_resident(const serve_ring_hdr
         __syncthreads();
         const int i = sh.item;
         if (i < 0) break;
-        const uint32_t it = __hip_atomic_load(&ring_items[(uint32_t)i & ring_mask], __ATOMIC_RELAXED,
-                                              __HIP_MEMORY_SCOPE_SYSTEM);
-        const int t = (int)(it >> 16), cb = (int)(it & 0xFFFFu);
+        const uint32_t it = (uint32_t)__builtin_amdgcn_readfirstlane((int)__hip_atomic_load(
+            &ring_items[(uint32_t)i & ring_mask], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM));
+        const int t = (int)(it >> 16), cb = item_cb(it), n = item_count(it);
         if (t < n_slots) {
             // the TB's descriptor and parameters from host memory into LDS
             const uint32_t* js = reinterpret_cast<const uint32_t*>(&jobs[t]);
@@ -570,25 +628,37 @@ __global__ __launch_bounds__(384) void pusch_serve_resident(const serve_ring_hdr
             if (tid == 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             __syncthreads();
             const uint32_t C = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_prm.num_CBs);
-            if ((uint32_t)cb < C) {
-                const uint32_t part = fused_cb<TI, 46>(hb, sh, s_job, s_prm, cb, lds_values);
-                __syncthreads();  // every output of the CB issued by every thread
-                if (tid == 0) complete_cb(s_job, C, part, tbw);
+            if ((uint32_t)(cb + n) <= C) {
+                const uint32_t part = fused_group<TI, 46, true>(hb, sh, s_job, s_prm, cb, n, lds_values);
+                __syncthreads();  // every output of the group issued by every thread
+                if (tid == 0) complete_cb(s_job, C, part, (uint32_t)n, tbw);
             }
         }
         __syncthreads();
     }
 }
 
-void* fused_kernel(cuphyDataType_t llr_type, int max_parity_nodes)
+template <bool GROUPED>
+void* fused_kernel_of(cuphyDataType_t llr_type, bool small)
 {
-    const bool small = max_parity_nodes <= SMALL_MAXR;
     switch (llr_type) {
-    case CUPHY_R_8I: return small ? (void*)&pusch_slot_fused<int8_t, SMALL_MAXR> : (void*)&pusch_slot_fused<int8_t, 46>;
-    case CUPHY_R_16F: return small ? (void*)&pusch_slot_fused<_Float16, SMALL_MAXR> : (void*)&pusch_slot_fused<_Float16, 46>;
-    case CUPHY_R_32F: return small ? (void*)&pusch_slot_fused<float, SMALL_MAXR> : (void*)&pusch_slot_fused<float, 46>;
+    case CUPHY_R_8I:
+        return small ? (void*)&pusch_slot_fused<int8_t, SMALL_MAXR, GROUPED> : (void*)&pusch_slot_fused<int8_t, 46, GROUPED>;
+    case CUPHY_R_16F:
+        return small ? (void*)&pusch_slot_fused<_Float16, SMALL_MAXR, GROUPED>
+                     : (void*)&pusch_slot_fused<_Float16, 46, GROUPED>;
+    case CUPHY_R_32F:
+        return small ? (void*)&pusch_slot_fused<float, SMALL_MAXR, GROUPED> : (void*)&pusch_slot_fused<float, 46, GROUPED>;
     default: return nullptr;
     }
+}
+
+// Kernel of a launch: LLR type, mb class, and whether items hold groups
+// (the one-CB form keeps no group state: measurably faster per CB).
+void* fused_kernel(cuphyDataType_t llr_type, int max_parity_nodes, bool grouped)
+{
+    const bool small = max_parity_nodes <= SMALL_MAXR;
+    return grouped ? fused_kernel_of<true>(llr_type, small) : fused_kernel_of<false>(llr_type, small);
 }
 
 }  // namespace nrldpc
@@ -635,14 +705,15 @@ cuphyStatus_t serve_resident_launch(const serve_ring_hdr* ring, const uint32_t* 
 }
 
 cuphyStatus_t slot_decode(const cuphyHipPuschSlotTb_t* d_tbs, int n_tbs, const uint32_t* d_items, int n_items,
-                          cuphyDataType_t llr_type, int max_lds_values, int max_parity_nodes, void* d_ws,
-                          int cu_count, hipStream_t strm)
+                          cuphyDataType_t llr_type, int max_lds_values, int max_parity_nodes, int max_cbs_per_item,
+                          void* d_ws, int cu_count, hipStream_t strm)
 {
     if (n_items < 0 || n_tbs < 0 || n_tbs > 65536 || !d_ws) return CUPHY_STATUS_INVALID_ARGUMENT;
     if (n_items > 0 && (!d_tbs || !d_items || n_tbs == 0)) return CUPHY_STATUS_INVALID_ARGUMENT;
     if (max_lds_values <= 0 || max_lds_values > SLOT_MAX_LDS_VALUES) return CUPHY_STATUS_INVALID_ARGUMENT;
     if (max_parity_nodes < 4 || max_parity_nodes > 46) return CUPHY_STATUS_INVALID_ARGUMENT;
-    void* func = fused_kernel(llr_type, max_parity_nodes);
+    if (max_cbs_per_item < 1 || max_cbs_per_item > FUSED_MAX_GROUP) return CUPHY_STATUS_INVALID_ARGUMENT;
+    void* func = fused_kernel(llr_type, max_parity_nodes, max_cbs_per_item > 1);
     if (!func) return CUPHY_STATUS_UNSUPPORTED_TYPE;
     const size_t lds = ((size_t)max_lds_values * 2 + 15) / 16 * 16 + 64u;  // + whole 32-value groups of the hard decisions
     cuphyStatus_t s = ensure_lds_attr(func, lds);  // also for an empty launch: prepares graph capture

@@ -14,6 +14,18 @@ namespace nrldpc {
 // pair at its first slot: TB-CRC accumulator and count of finished CBs.
 constexpr int FUSED_WS_QUEUE_WORDS = 64;
 
+// Work items of the fused kernels: (TB << 16) | ((count - 1) << 12) | CB,
+// the group of `count` consecutive CBs CB .. CB + count - 1 of the TB,
+// decoded side by side in one workgroup (count * Zc <= 384).
+constexpr int FUSED_MAX_GROUP = 16;
+constexpr int FUSED_BLOCK = 384;
+__host__ __device__ inline int item_cb(uint32_t it) { return (int)(it & 0xFFFu); }
+__host__ __device__ inline int item_count(uint32_t it) { return (int)((it >> 12) & 0xFu) + 1; }
+__host__ __device__ inline uint32_t make_item(int tb, int cb, int count)
+{
+    return ((uint32_t)tb << 16) | ((uint32_t)(count - 1) << 12) | (uint32_t)cb;
+}
+
 // LDS values (binary16) of the fused kernels: a slot launch takes up to
 // SLOT_MAX_LDS_VALUES; a resident launch always has SERVE_LDS_VALUES (any
 // HARQ/APP array, 68 * 384, plus 2048 descrambling words: E up to 65,473
@@ -27,10 +39,13 @@ constexpr int SERVE_LDS_VALUES = 68 * 384 + 2 * 2048;
 // values per word; words = (start + E - 1) / 32 - start / 32 + 1 of the
 // largest E, start % 32 at most 31, plus one).  The same per-CB split of G
 // into E as the de-rate-match kernel (TS 38.212 5.4.2.1).
-inline int slot_lds_values(const PerTbParams& p, int mb, bool descramble)
+// A group of `count` CBs holds count arrays, each but the last rounded up to
+// whole 16-byte units.
+inline int slot_lds_values(const PerTbParams& p, int mb, bool descramble, int count = 1)
 {
     const int nsys = (p.bg == 1) ? 22 : 10;
-    const int arr = std::max<int>((int)(2 * p.Zc + p.Ncb), (nsys + mb) * (int)p.Zc);
+    const int arr1 = std::max<int>((int)(2 * p.Zc + p.Ncb), (nsys + mb) * (int)p.Zc);
+    const int arr = (count - 1) * ((arr1 + 7) & ~7) + arr1;
     if (!descramble || p.uciOnPuschFlag) return arr;
     const long long nq = (long long)p.Nl * p.Qm, C = p.num_CBs;
     const long long total = p.encodedSize;
@@ -39,6 +54,16 @@ inline int slot_lds_values(const PerTbParams& p, int mb, bool descramble)
     const long long emax = nq * q + (q * nq * C < total ? nq : 0);
     const long long words = (31 + emax - 1) / 32 + 1;
     return (int)std::min<long long>(((arr + 1) & ~1) + 2 * (words + 1), 1LL << 30);
+}
+
+// CBs per work item for a TB: as many as the workgroup's lanes hold (count *
+// Zc <= FUSED_BLOCK, at most FUSED_MAX_GROUP and num_CBs) whose LDS need
+// stays within lds_limit.
+inline int fused_group_size(const PerTbParams& p, int mb, bool descramble, int lds_limit)
+{
+    int n = std::min<int>({FUSED_MAX_GROUP, FUSED_BLOCK / std::max<int>(1, (int)p.Zc), (int)p.num_CBs});
+    while (n > 1 && slot_lds_values(p, mb, descramble, n) > lds_limit) --n;
+    return std::max(n, 1);
 }
 
 // Resident serving launch (pusch_serve_resident): the host-written ring
@@ -72,7 +97,7 @@ cuphyStatus_t serve_resident_launch(const serve_ring_hdr* ring, const uint32_t* 
 // Enqueue one persistent fused launch (de-rate-match + HARQ combine +
 // decode + CB/TB CRC) over the CB items of a slot.
 cuphyStatus_t slot_decode(const cuphyHipPuschSlotTb_t* d_tbs, int n_tbs, const uint32_t* d_items, int n_items,
-                          cuphyDataType_t llr_type, int max_lds_values, int max_parity_nodes, void* d_ws,
-                          int cu_count, hipStream_t strm);
+                          cuphyDataType_t llr_type, int max_lds_values, int max_parity_nodes, int max_cbs_per_item,
+                          void* d_ws, int cu_count, hipStream_t strm);
 
 }  // namespace nrldpc

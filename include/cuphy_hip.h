@@ -451,9 +451,13 @@ typedef struct {
 cuphyStatus_t CUPHYWINAPI cuphyHipPuschSlotDecodeGetWorkspaceSize(int num_cbs, size_t* sizeInBytes);
 
 /* Fused de-rate-match + HARQ combine + layered NMS decode (binary16) + CB
- * CRC + TB assembly + TB CRC for num_cb_items code blocks in one persistent
- * launch.  d_cb_items[i] = (TB index << 16) | CB index within the TB;
- * every CB of a TB must be in the launch exactly once.  A TB's CRC
+ * CRC + TB assembly + TB CRC for num_cb_items work items in one persistent
+ * launch.  d_cb_items[i] = (TB index << 16) | ((count - 1) << 12) | CB
+ * index within the TB: the `count` (1..16) consecutive CBs from that index
+ * on, decoded side by side in one 384-thread workgroup (count * Zc <= 384;
+ * each CB stops on its own syndrome, results as decoded alone); count = 1
+ * gives one CB per item, (TB index << 16) | CB index (CB index < 4096).
+ * Every CB of a TB must be in the launch exactly once.  A TB's CRC
  * (*tb_crc) is written as soon as its last CB is decoded.  With
  * CUPHY_HIP_SLOT_TB_NOTIFY in the TB's flags it is written only after every
  * other output of the TB has reached device memory: a caller that points
@@ -461,16 +465,19 @@ cuphyStatus_t CUPHYWINAPI cuphyHipPuschSlotDecodeGetWorkspaceSize(int num_cbs, s
  * (e.g. 0xFFFFFFFE), can take each TB as soon as that value changes, before
  * the launch ends (the TB's HARQ buffer must still not be reused by another
  * launch before this one ends).  llr_type: CUPHY_R_8I,
- * CUPHY_R_16F or CUPHY_R_32F.  max_lds_values: max over TBs of
- * max(2Zc + Ncb, (nsys + mb) * Zc), plus the sequence words of descrambled
- * TBs (see CUPHY_HIP_SLOT_TB_DESCRAMBLE), at most 68 * 384 + 2 * 12288
- * (sizes LDS); max_parity_nodes: max mb
+ * CUPHY_R_16F or CUPHY_R_32F.  max_lds_values: max over items of
+ * (count - 1) * roundup8(A) + A with A = max(2Zc + Ncb, (nsys + mb) * Zc),
+ * plus the sequence words of descrambled TBs (see
+ * CUPHY_HIP_SLOT_TB_DESCRAMBLE), at most 68 * 384 + 2 * 12288 (sizes LDS;
+ * an item that does not fit is rejected); max_cbs_per_item: the largest
+ * count of the items (1 runs the one-CB-per-workgroup kernel, which keeps
+ * no group state; an item of several CBs is then rejected); max_parity_nodes: max mb
  * (<= 8 selects the register-lean high-code-rate variant; a CB with a
  * larger mb is rejected). */
 cuphyStatus_t CUPHYWINAPI cuphyHipPuschSlotDecode(cuphyLDPCDecoder_t decoder, const cuphyHipPuschSlotTb_t* d_tbs,
                                                   int num_tbs, const uint32_t* d_cb_items, int num_cb_items,
                                                   cuphyDataType_t llr_type, int max_lds_values, int max_parity_nodes,
-                                                  void* d_workspace, hipStream_t strm);
+                                                  int max_cbs_per_item, void* d_workspace, hipStream_t strm);
 
 /* ------------------------------------------------------------------------
  * HARQ serving engine (no cuphy.h counterpart: cuPHY is driven slot by slot
@@ -544,7 +551,9 @@ typedef struct {
 cuphyStatus_t CUPHYWINAPI cuphyHipHarqServerCreate(cuphyLDPCDecoder_t decoder, const cuphyHipHarqServerConfig_t* cfg,
                                                    cuphyHipHarqServer_t* server);
 cuphyStatus_t CUPHYWINAPI cuphyHipHarqServerDestroy(cuphyHipHarqServer_t server);
-/* Queue one transmission (copied); returns at once. */
+/* Queue one transmission (copied); returns at once.  CUPHY_STATUS_INVALID_ARGUMENT
+ * for parameters the engine cannot run (num_CBs above 4096 or max_cbs_per_launch,
+ * a HARQ buffer larger than the pool, mb outside the base graph, ...). */
 cuphyStatus_t CUPHYWINAPI cuphyHipHarqServerSubmit(cuphyHipHarqServer_t server, const cuphyHipHarqRequest_t* req);
 /* Non-blocking progress: retire finished launches (their completions go to
  * out[0 .. *n_out-1]; the rest wait for the next call), admit pending

@@ -21,6 +21,7 @@ SLOT = [
     (1, 64, 1, 1296, 6, 3, 3, 4500, 1.0),      # C = 1: CRC16 TB; E > Ncb - F: repetition
     (2, 52, 2, 480, 2, 1, 1, 3000, -2.0),      # repetition, BG2 filler
     (2, 384, 2, 3840, 8, 4, 4, 4096, 3.0),     # high rate BG2
+    (2, 32, 12, 280, 2, 1, 1, 600, 0.5),       # 12 CBs of Zc=32: one work item of 12 side by side
 ]
 TX = [(1, 0), (0, 2)]  # (ndi, rv) of the two transmissions
 ITERS = 6
@@ -155,6 +156,35 @@ def test_slot_descrambled_matches_oracle(dec, dtype):
     run_slot(dec, dtype, early_term=False, descramble=True)
 
 
+def test_slot_packed_equals_one_cb_per_workgroup(dec):
+    """Consecutive CBs of a TB decoded side by side in one workgroup (each
+    stopping on its own syndrome) give the same outputs, iterations
+    included, as one CB per workgroup."""
+    from aerial_amd.slot import SlotDecoder, SlotTb
+    print("[case] slot packed vs unpacked, fp16, early termination", flush=True)
+    outs = []
+    for pack in (True, False):
+        cases = [Case(k, s, "fp16") for k, s in enumerate(SLOT)]
+        sd = SlotDecoder(dec, pack=pack)
+        res = []
+        for tx, (ndi, rv) in enumerate(TX):
+            tbs = []
+            for c in cases:
+                p, dev, scale = c.transmission(ndi, rv, tx)
+                tbs.append(SlotTb(p, dev, c.harq, c.mb, max_iterations=ITERS, early_term=True, llr_scale=scale))
+            sd.build(tbs).launch()
+            torch.cuda.synchronize()
+            if pack:
+                assert sum(ln.n_items for ln in sd.launches) < sum(c.spec[2] for c in cases)
+            res.append([(tb.cb_bits.clone(), tb.cb_crc.clone(), tb.tb_bytes.clone(), tb.tb_crc.clone(),
+                         tb.cb_iterations.clone(), c.harq.clone()) for c, tb in zip(cases, tbs)])
+        outs.append(res)
+    for tx in range(len(TX)):
+        for k, (a, b) in enumerate(zip(outs[0][tx], outs[1][tx])):
+            for j, (x, y) in enumerate(zip(a, b)):
+                assert torch.equal(x, y), (tx, SLOT[k], j)
+
+
 @pytest.mark.parametrize("descramble", [False, True])
 def test_slot_decodes_payload_and_passes_crc(dec, descramble):
     """At high SNR every TB passes its CRC and the TB bytes are the payload
@@ -269,7 +299,8 @@ def test_slot_graph_replay(dec):
 
 def test_slot_rejects_cb_beyond_variant(dec):
     """A CB whose mb exceeds the launched variant's row capacity is not
-    decoded: its CRC and its TB CRC carry the rejection marker."""
+    decoded: its CRC and its TB CRC carry the rejection marker; so are the
+    CBs of a grouped item launched on the one-CB kernel."""
     import ctypes as C
     from aerial_amd import _lib as L
     from aerial_amd.slot import SlotDecoder, SlotTb
@@ -281,10 +312,23 @@ def test_slot_rejects_cb_beyond_variant(dec):
     sd = SlotDecoder(dec).build([tb])
     ln = sd.launches[0]
     st = L.lib().cuphyHipPuschSlotDecode(dec.dec, C.c_void_p(sd._jobs.data_ptr()), 1, C.c_void_p(ln.items.data_ptr()),
-                                         ln.n_items, L.CUPHY_R_16F, ln.max_values, 8,
+                                         ln.n_items, L.CUPHY_R_16F, ln.max_values, 8, ln.max_group,
                                          C.c_void_p(ln.workspace.data_ptr()), None)
     L.check("cuphyHipPuschSlotDecode", st)
     torch.cuda.synchronize()
     assert (tb.cb_crc.cpu().numpy().view(np.uint32) == L.SLOT_CB_ERROR).all()
     assert int(tb.tb_crc.cpu().numpy().view(np.uint32)[0]) == L.SLOT_CB_ERROR
     assert int(ln.workspace[:2].abs().sum().item()) == 0
+    # an item of two CBs given to the one-CB-per-workgroup kernel
+    # (max_cbs_per_item = 1) is rejected as a whole, not half decoded
+    assert ln.n_items == 1 and ln.max_group == 2
+    tb.cb_crc.zero_()
+    tb.tb_crc.zero_()
+    st = L.lib().cuphyHipPuschSlotDecode(dec.dec, C.c_void_p(sd._jobs.data_ptr()), 1, C.c_void_p(ln.items.data_ptr()),
+                                         ln.n_items, L.CUPHY_R_16F, ln.max_values, 46, 1,
+                                         C.c_void_p(ln.workspace.data_ptr()), None)
+    L.check("cuphyHipPuschSlotDecode", st)
+    torch.cuda.synchronize()
+    assert (tb.cb_crc.cpu().numpy().view(np.uint32) == L.SLOT_CB_ERROR).all()
+    assert int(tb.tb_crc.cpu().numpy().view(np.uint32)[0]) == L.SLOT_CB_ERROR
+    assert int(ln.workspace.abs().sum().item()) == 0
