@@ -367,7 +367,7 @@ __device__ __forceinline__ uint32_t fused_group(_Float16* __restrict__ hb, fused
     int nw_max = 0;
     bool e_ok = g0.L > 0;
     for (int c = 0; c < count; ++c) {
-        const rm_geometry g = rm_geometry_of(p, cb0 + c);
+        const rm_geometry g = (c == 0) ? g0 : rm_geometry_of(p, cb0 + c);
         e_ok = e_ok && g.E > 0;
         if (descr && g.E > 0) nw_max = max(nw_max, rm_seq_words(g));
     }
@@ -387,8 +387,11 @@ __device__ __forceinline__ uint32_t fused_group(_Float16* __restrict__ hb, fused
 
     // ---- 1-3. per CB: HARQ buffer, combining, write-back, APP --------------
     for (int c = 0; c < count; ++c) {
-        rm_geometry g = rm_geometry_of(p, cb0 + c);
-        rm_make_uniform(g);
+        rm_geometry g = g0;  // (the one-CB form keeps the geometry computed above)
+        if (GROUPED) {
+            g = rm_geometry_of(p, cb0 + c);
+            rm_make_uniform(g);
+        }
         fused_combine<TI>(hb + c * S, seq, job, p, g, r0 + c, nh, nval, descr);
     }
 

@@ -150,12 +150,16 @@ def serving_section(dec, serve_ms, config3_gbps=None, inflight=3, loads=(0.6, 0.
     out = {"saturation": rep, "capacity_first_tx_tbs_per_s": round(cap_tbs, 1),
            "capacity_first_tx_gbps": round(cap_tbs * mean_bits / 1e9, 3),
            "capacity_first_tx_cbs_per_s": round(cap_tbs * mean_cbs, 1), "loads": []}
-    for x in loads:
+    loads = list(loads)
+    extra = bool(config3_cbs_per_s and resident)
+    if extra:  # one more point: 60 % of the 20-cell full-load slot's code blocks per second
+        loads.append(0.6 * config3_cbs_per_s / (cap_tbs * mean_cbs))
+    for i, x in enumerate(loads):
         arr = poisson_arrivals(32, x * cap_tbs, serve_ms * 1e3, n_templates, seed=int(100 * x))
         entries, outs, meta = build_trace(srv, templates, arr)
         done = srv.serve_trace(entries, rtt)
         r = trace_report(done, meta, templates, arr)
-        r["offered_load"] = x
+        r["offered_load"] = round(x, 4)
         r["offered_gbps"] = round(x * cap_tbs * mean_bits / 1e9, 3)
         if config3_gbps:
             r["offered_vs_config3"] = round(x * cap_tbs * mean_bits / 1e9 / config3_gbps, 4)
@@ -163,7 +167,9 @@ def serving_section(dec, serve_ms, config3_gbps=None, inflight=3, loads=(0.6, 0.
             r["offered_cbs_vs_config3"] = round(x * cap_tbs * mean_cbs / config3_cbs_per_s, 4)
         r["engine"] = srv.stats()
         out["loads"].append(r)
-        print("serving (%s) load %.1f: %s" % ("resident" if resident else "batched", x, json.dumps(r)),
+        if extra and i == len(loads) - 1:
+            out["at_60pct_of_config3_cbs"] = r
+        print("serving (%s) load %.3f: %s" % ("resident" if resident else "batched", x, json.dumps(r)),
               file=sys.stderr, flush=True)
     srv.close()
     out["workload"] = ("32 cells, Poisson TB arrivals, TB shapes as the heterogeneous slot (256 templates), "
