@@ -8,7 +8,10 @@ over the device work queue), and can capture the launches in a HIP graph
 
 Two regimes:
   * latency (fewer code blocks than `split_min`): every code block gets a
-    384-thread workgroup of its own, so each one runs at its own speed;
+    workgroup of its own, so each one runs at its own speed -- a 768-thread
+    row-pair workgroup (two lane groups taking the independent block rows
+    of each barrier segment at the same time) unless all code blocks are
+    high-rate (mb <= 8: no two consecutive rows are independent), then 384;
   * throughput (at least `split_min` code blocks): code blocks of one
     (BG, Z, mb) are packed several to a workgroup and workgroup sizes follow
     Z, as the tuning table chooses per (BG, Z, mb class)
@@ -70,8 +73,9 @@ class BatchDecoder:
     regime: groups and workgroup sizes from the tuning table, one launch per
     (mb class, workgroup size) on separate streams (fork/join)."""
 
-    def __init__(self, decoder, split_by_mb=True, split_min=256, pack=True, table=None):
+    def __init__(self, decoder, split_by_mb=True, split_min=256, pack=True, table=None, pair=True):
         self.dec = decoder
+        self.pair = pair  # latency regime: row-pair workgroups (768 threads)
         self._lib = L.lib()
         self.split_by_mb = split_by_mb
         self.split_min = split_min
@@ -93,8 +97,10 @@ class BatchDecoder:
     def workspaces(self):
         return [b.workspace for b in self.subs]
 
-    def _choice(self, bg, Z, mb, regime_throughput):
+    def _choice(self, bg, Z, mb, regime_throughput, pair_launch=False):
         """(code blocks per workgroup, workgroup size) of a configuration."""
+        if not regime_throughput and pair_launch:
+            return 1, 768
         if not (self.pack and regime_throughput):
             return 1, 384
         return tuning.choose(self.table, bg, Z, mb)
@@ -116,8 +122,8 @@ class BatchDecoder:
         launches = {}
         for t, tb in enumerate(tbs):
             Kb = getattr(tb, "Kb", None) or nsys(tb.bg)
-            P, block = self._choice(tb.bg, tb.Z, tb.mb, throughput)
             small = (tb.mb <= SMALL_MAXR) if split else False
+            P, block = self._choice(tb.bg, tb.Z, tb.mb, throughput, pair_launch=self.pair and not small)
             key = (small, block)
             # estimated group time: waves per SIMD x edges per row sweep
             waves = -(-P * tb.Z // 64)

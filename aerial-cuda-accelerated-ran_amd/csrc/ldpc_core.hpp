@@ -80,6 +80,40 @@ typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 
 template <int BG> constexpr uint64_t kBarrierMask = barrier_mask<bg_traits<BG>>();
 
+// Row-pair plan (two lane groups per code block): the rows between two
+// barriers share no column, so two lane groups may process them at the same
+// time.  Each row goes to one group: the rows of a barrier segment
+// alternate between the groups, and a segment's first row goes to the
+// group holding fewer edges so far (the per-lane message state is then
+// about half the graph's).  Within its group a row's edges take
+// consecutive message slots from edge_slot[r] and its compressed row state
+// the slot row_slot[r].  Rows r < NR only.
+template <int BG, int NR> struct pair_plan {
+    int group[64];
+    int edge_slot[64];
+    int row_slot[64];
+    int edges[2];
+    int rows[2];
+};
+template <int BG, int NR> constexpr pair_plan<BG, NR> make_pair_plan()
+{
+    using BGT = bg_traits<BG>;
+    pair_plan<BG, NR> p{};
+    int prev = 1;
+    for (int r = 0; r < NR; ++r) {
+        const bool first = (r == 0) || ((kBarrierMask<BG> >> r) & 1ull);
+        const int g = first ? (p.edges[1] < p.edges[0] ? 1 : 0) : 1 - prev;
+        p.group[r] = g;
+        p.edge_slot[r] = p.edges[g];
+        p.row_slot[r] = p.rows[g];
+        p.edges[g] += BGT::row_start(r + 1) - BGT::row_start(r);
+        p.rows[g] += 1;
+        prev = g;
+    }
+    return p;
+}
+template <int BG, int NR> constexpr pair_plan<BG, NR> kPairPlan = make_pair_plan<BG, NR>();
+
 __device__ __forceinline__ uint32_t f2u(float x) { return __builtin_bit_cast(uint32_t, x); }
 __device__ __forceinline__ float u2f(uint32_t x) { return __builtin_bit_cast(float, x); }
 __device__ __forceinline__ uint16_t h2u(_Float16 x) { return __builtin_bit_cast(uint16_t, x); }
@@ -113,7 +147,8 @@ __device__ __forceinline__ T& lds_at(unsigned char* base, uint32_t off)
 // (edge j at bit D-1-j).  Per edge: address 3, previous-message rebuild 4,
 // subtract 1, min/min2/parity 3, new message + write-back 6 VALU ops.
 // ---------------------------------------------------------------------------
-template <int BG, int R, int NR>
+// RS: the row's state slot (R, or its slot within a row-pair lane group).
+template <int BG, int R, int NR, int RS = R>
 __device__ __forceinline__ void process_row(float*, unsigned char* __restrict__ base, shift_ptr sh, uint32_t zb,
                                             uint32_t Zb, float alpha, float (&st_n1)[NR], float (&st_n2)[NR],
                                             uint32_t (&st_meta)[NR])
@@ -123,8 +158,8 @@ __device__ __forceinline__ void process_row(float*, unsigned char* __restrict__ 
     constexpr int D = BGT::row_start(R + 1) - E0;
     float t[D];
     uint32_t off[D];
-    const float on1 = st_n1[R], on2 = st_n2[R];
-    const uint32_t om = st_meta[R];
+    const float on1 = st_n1[RS], on2 = st_n2[RS];
+    const uint32_t om = st_meta[RS];
     const uint32_t oidx = om >> 27;
     static_for<D>([&](auto J) {
         constexpr int j = decltype(J)::value;
@@ -156,9 +191,9 @@ __device__ __forceinline__ void process_row(float*, unsigned char* __restrict__ 
         lds_at<float>(base, off[j]) = t[j] + u2f(cb);
         fs = __builtin_amdgcn_alignbit(fs, cb, 31);  // (fs << 1) | sign(c_j)
     });
-    st_n1[R] = a1;
-    st_n2[R] = a2;
-    st_meta[R] = fs | (idx << 27);
+    st_n1[RS] = a1;
+    st_n2[RS] = a2;
+    st_meta[RS] = fs | (idx << 27);
 }
 
 // ---------------------------------------------------------------------------
@@ -166,7 +201,9 @@ __device__ __forceinline__ void process_row(float*, unsigned char* __restrict__ 
 // two edges per VGPR (BG1: 158 registers).  Per edge: address 3, subtract
 // 1, min/min2/parity 3, new message + write-back 5 VALU ops; no argmin.
 // ---------------------------------------------------------------------------
-template <int BG, int R, int NW>
+// S0: message slot of the row's first edge (its edge index, or its slot
+// within a row-pair lane group).
+template <int BG, int R, int NW, int S0 = bg_traits<BG>::row_start(R)>
 __device__ __forceinline__ void process_row(_Float16*, unsigned char* __restrict__ base, shift_ptr sh, uint32_t zb,
                                             uint32_t Zb, _Float16 alpha, half2_t (&st)[NW])
 {
@@ -178,8 +215,9 @@ __device__ __forceinline__ void process_row(_Float16*, unsigned char* __restrict
     static_for<D>([&](auto J) {
         constexpr int j = decltype(J)::value;
         constexpr int e = E0 + j;
+        constexpr int sl = S0 + j;
         off[j] = (uint32_t)BGT::col(e) * Zb + rot_index(zb, sh[e] * 2u, Zb);
-        const _Float16 old = (e & 1) ? st[e >> 1].y : st[e >> 1].x;
+        const _Float16 old = (sl & 1) ? st[sl >> 1].y : st[sl >> 1].x;
         t[j] = lds_at<_Float16>(base, off[j]) - old;
     });
     _Float16 m1 = (_Float16)__builtin_huge_valf(), m2 = (_Float16)__builtin_huge_valf();
@@ -195,14 +233,14 @@ __device__ __forceinline__ void process_row(_Float16*, unsigned char* __restrict
     const _Float16 a1 = alpha * m1, a2 = alpha * m2;
     static_for<D>([&](auto J) {
         constexpr int j = decltype(J)::value;
-        constexpr int e = E0 + j;
+        constexpr int sl = S0 + j;
         const _Float16 mag = (__builtin_fabsf16(t[j]) == m1) ? a2 : a1;
         const _Float16 c = u2h((uint16_t)(h2u(mag) | ((h2u(t[j]) ^ par) & 0x8000u)));
         lds_at<_Float16>(base, off[j]) = t[j] + c;
-        if (e & 1)
-            st[e >> 1].y = c;
+        if (sl & 1)
+            st[sl >> 1].y = c;
         else
-            st[e >> 1].x = c;
+            st[sl >> 1].x = c;
     });
 }
 
@@ -280,24 +318,32 @@ template <> __device__ __forceinline__ _Float16 job_alpha<_Float16>(const cb_job
 // syndrome (its lanes freeze, exactly as if decoded alone); `cb_flags` is
 // LDS scratch of `count` words for that (unused when count == 1).
 // cb_iters (optional, LDS, `count` words): iterations each code block ran.
-template <int BG, typename T, int MAXR>
+// PAIR = 2: row-pair mode for one code block (count must be 1, blockDim.x
+// >= 2 * roundup64(Z)): lanes [0, Z) and [Zw, Zw + Z) (Zw = roundup64(Z),
+// wave-aligned) form two lane groups that process the rows of kPairPlan's
+// group 0 and group 1; rows of one barrier segment run at the same time,
+// and each lane holds the message state of its group's rows only.
+template <int BG, typename T, int MAXR, int PAIR = 1>
 __device__ __forceinline__ int run_iterations(T* __restrict__ app, const cb_job& job, int count = 1, int stride = 0,
                                               uint32_t* cb_flags = nullptr, uint32_t* cb_iters = nullptr)
 {
     using BGT = bg_traits<BG>;
     constexpr int NR = (MAXR < BGT::NROWS) ? MAXR : BGT::NROWS;
+    constexpr pair_plan<BG, NR> PP = kPairPlan<BG, NR>;
     const uint32_t Z = (uint32_t)job.Z;
     const int mb = job.mb;
     const uint32_t tid = threadIdx.x;
     const T alpha = job_alpha<T>(job);
 
     // C2V state: compressed per row (binary32) or explicit per edge (binary16)
-    constexpr int NW = (BGT::row_start(NR) + 1) / 2;
-    float st_n1[std::is_same<T, float>::value ? NR : 1], st_n2[std::is_same<T, float>::value ? NR : 1];
-    uint32_t st_meta[std::is_same<T, float>::value ? NR : 1];
+    constexpr int NRS = (PAIR == 2) ? (PP.rows[0] > PP.rows[1] ? PP.rows[0] : PP.rows[1]) : NR;
+    constexpr int NEDGE = (PAIR == 2) ? (PP.edges[0] > PP.edges[1] ? PP.edges[0] : PP.edges[1]) : BGT::row_start(NR);
+    constexpr int NW = (NEDGE + 1) / 2;
+    float st_n1[std::is_same<T, float>::value ? NRS : 1], st_n2[std::is_same<T, float>::value ? NRS : 1];
+    uint32_t st_meta[std::is_same<T, float>::value ? NRS : 1];
     half2_t st_h[std::is_same<T, float>::value ? 1 : NW];
     if constexpr (std::is_same<T, float>::value) {
-        static_for<NR>([&](auto R) {
+        static_for<NRS>([&](auto R) {
             st_n1[R] = 0.0f;
             st_n2[R] = 0.0f;
             st_meta[R] = 0;
@@ -306,10 +352,21 @@ __device__ __forceinline__ int run_iterations(T* __restrict__ app, const cb_job&
         static_for<NW>([&](auto W) { st_h[W] = half2_t{(_Float16)0, (_Float16)0}; });
     }
 
-    // lane -> (code block, check row)
-    const uint32_t cb = (count == 1) ? (tid < Z ? 0u : 1u) : tid / Z;
-    const uint32_t zlane = tid - (cb < (uint32_t)count ? cb : 0u) * Z;
-    const bool in_group = cb < (uint32_t)count;
+    // lane -> (code block, check row); row-pair mode: lane -> (lane group, check row)
+    uint32_t cb, zlane, lgrp = 0;
+    bool in_group;
+    if constexpr (PAIR == 2) {
+        const uint32_t Zw = (Z + 63u) & ~63u;
+        lgrp = tid >= Zw ? 1u : 0u;
+        zlane = tid - lgrp * Zw;
+        cb = 0;
+        in_group = zlane < Z && tid < 2u * Zw;
+        count = 1;
+    } else {
+        cb = (count == 1) ? (tid < Z ? 0u : 1u) : tid / Z;
+        zlane = tid - (cb < (uint32_t)count ? cb : 0u) * Z;
+        in_group = cb < (uint32_t)count;
+    }
     bool active = in_group;
     unsigned char* const lane_base =
         reinterpret_cast<unsigned char*>(app) + (size_t)(in_group ? cb : 0u) * (size_t)stride * sizeof(T);
@@ -336,11 +393,13 @@ __device__ __forceinline__ int run_iterations(T* __restrict__ app, const cb_job&
                 shift_ptr shr = sh;
                 uint32_t Zr = Zb;
                 asm volatile("" : "+s"(shr), "+s"(Zr));
-                if (active) {
+                if (active && (PAIR == 1 || lgrp == (uint32_t)PP.group[R])) {
+                    constexpr int RS = (PAIR == 2) ? PP.row_slot[R] : R;
+                    constexpr int S0 = (PAIR == 2) ? PP.edge_slot[R] : BGT::row_start(R);
                     if constexpr (std::is_same<T, float>::value)
-                        process_row<BG, R, NR>(app, lane_base, shr, zb, Zr, alpha, st_n1, st_n2, st_meta);
+                        process_row<BG, R, NRS, RS>(app, lane_base, shr, zb, Zr, alpha, st_n1, st_n2, st_meta);
                     else
-                        process_row<BG, R, NW>(app, lane_base, shr, zb, Zr, alpha, st_h);
+                        process_row<BG, R, NW, S0>(app, lane_base, shr, zb, Zr, alpha, st_h);
                 }
                 // barrier before the next row only where it shares a column
                 // with the rows since the last barrier, and at the iteration end
@@ -358,7 +417,8 @@ __device__ __forceinline__ int run_iterations(T* __restrict__ app, const cb_job&
                     shift_ptr shr = sh;
                     uint32_t Zr = Zb;
                     asm volatile("" : "+s"(shr), "+s"(Zr));
-                    if (active) fail |= row_syndrome<BG, R, T>(lane_base, shr, zb, Zr);
+                    if (active && (PAIR == 1 || lgrp == (uint32_t)PP.group[R]))
+                        fail |= row_syndrome<BG, R, T>(lane_base, shr, zb, Zr);
                 }
             });
             if (count == 1) {
@@ -463,6 +523,21 @@ __device__ __forceinline__ int decode_group(T* __restrict__ app, const cb_job& j
         if (b.soft)
             for (int n = threadIdx.x; n < nbits; n += blockDim.x) b.soft[n] = (_Float16)app[c * stride + n];
     }
+    return iters;
+}
+
+// One code block in row-pair mode (run_iterations PAIR = 2; blockDim.x >=
+// 2 * roundup64(Z)): LLRs into LDS, iterations, hard (and soft) outputs.
+template <int BG, typename T, typename TIn, int MAXR>
+__device__ __forceinline__ int decode_pair(T* __restrict__ app, const cb_job& job, const cb_buffers& b)
+{
+    const int nval = (bg_traits<BG>::NSYS + job.mb) * job.Z;
+    load_llrs<T, TIn>(app, static_cast<const TIn*>(b.llr), nval, job.clamp);
+    const int iters = run_iterations<BG, T, MAXR, 2>(app, job);
+    const int nbits = job.Kb * job.Z;
+    store_hard<T>(app, nbits, b.out);
+    if (b.soft)
+        for (int n = threadIdx.x; n < nbits; n += blockDim.x) b.soft[n] = (_Float16)app[n];
     return iters;
 }
 

@@ -116,10 +116,66 @@ __global__ __launch_bounds__(384) void ldpc_layered_persistent(const cuphyHipLDP
     }
 }
 
+// Row-pair persistent kernel (768 threads): every code block of the queue
+// decoded alone by two lane groups that take the independent block rows of
+// each barrier segment at the same time (run_iterations PAIR = 2), with 12
+// waves on the CU's 4 SIMDs.  For batches with fewer code blocks than CUs,
+// where the slot time is the slowest code block's decode time.  Same queue
+// protocol as ldpc_layered_persistent; a group's code blocks are decoded
+// one after another.
+template <typename T, typename TIn, int MAXR>
+__global__ __launch_bounds__(768) void ldpc_layered_persistent_pair(const cuphyHipLDPCWorkItem_t* __restrict__ items,
+                                                                    const uint32_t* __restrict__ group_first,
+                                                                    int n_groups, int lds_values,
+                                                                    uint32_t* __restrict__ queue)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __shared__ int s_item;
+    T* app = reinterpret_cast<T*>(lds_raw);
+    for (;;) {
+        if (threadIdx.x == 0) s_item = (int)atomicAdd(&queue[0], 1u);
+        __syncthreads();
+        const int g = __builtin_amdgcn_readfirstlane(s_item);
+        __syncthreads();
+        if (g >= n_groups) break;
+        const int first = group_first ? (int)group_first[g] : g;
+        const int count = group_first ? (int)group_first[g + 1] - first : 1;
+        for (int c = 0; c < count; ++c) {
+            const cuphyHipLDPCWorkItem_t& w = items[first + c];
+            const cb_job job = item_job(w);
+            const int nval = ((w.BG == 1) ? 22 : 10) * job.Z + job.mb * job.Z;
+            if (nval <= lds_values && 2 * ((job.Z + 63) & ~63) <= (int)blockDim.x) {
+                const cb_buffers b{w.llr, w.out, static_cast<_Float16*>(w.soft)};
+                if (w.BG == 1)
+                    decode_pair<1, T, TIn, MAXR>(app, job, b);
+                else
+                    decode_pair<2, T, TIn, MAXR>(app, job, b);
+            }
+            __syncthreads();
+        }
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const uint32_t done = atomicAdd(&queue[1], 1u);
+        if (done == gridDim.x - 1) {
+            atomicExch(&queue[0], 0u);
+            atomicExch(&queue[1], 0u);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 namespace {
+
+void* select_persistent_pair_kernel(cuphyDataType_t llr_type, bool fp16_math)
+{
+    if (llr_type == CUPHY_R_32F && !fp16_math) return (void*)&ldpc_layered_persistent_pair<float, float, 46>;
+    if (llr_type == CUPHY_R_32F) return (void*)&ldpc_layered_persistent_pair<_Float16, float, 46>;
+    if (llr_type == CUPHY_R_16F) return (void*)&ldpc_layered_persistent_pair<_Float16, _Float16, 46>;
+    return nullptr;
+}
 
 template <int MAXR>
 void* select_tb_kernel(int bg, cuphyDataType_t llr_type, bool fp16_math)
@@ -230,7 +286,8 @@ cuphyStatus_t decode_batch(const cuphyHipLDPCWorkItem_t* d_items, const uint32_t
                            int block_threads, uint32_t* d_queue, int cu_count, hipStream_t strm)
 {
     if (n_groups < 0 || !d_queue || (n_groups > 0 && !d_items)) return CUPHY_STATUS_INVALID_ARGUMENT;
-    if (block_threads < 64 || block_threads > 384 || block_threads % 64) return CUPHY_STATUS_INVALID_ARGUMENT;
+    const bool pair = block_threads == 768;  // row-pair mode
+    if (!pair && (block_threads < 64 || block_threads > 384 || block_threads % 64)) return CUPHY_STATUS_INVALID_ARGUMENT;
     const bool f16 = fp16_math || llr_type == CUPHY_R_16F;
     const size_t elem = f16 ? 2 : 4;
     // a group's LDS regions are at most 384 lanes' worth of code blocks
@@ -238,8 +295,9 @@ cuphyStatus_t decode_batch(const cuphyHipLDPCWorkItem_t* d_items, const uint32_t
         return CUPHY_STATUS_INVALID_ARGUMENT;
     if (n_groups == 0) return CUPHY_STATUS_SUCCESS;
     if (llr_type != CUPHY_R_32F && llr_type != CUPHY_R_16F) return CUPHY_STATUS_UNSUPPORTED_TYPE;
-    void* func = max_mb <= SMALL_MAXR ? select_persistent_kernel<SMALL_MAXR>(llr_type, f16)
-                                      : select_persistent_kernel<46>(llr_type, f16);
+    void* func = pair                  ? select_persistent_pair_kernel(llr_type, f16)
+                 : max_mb <= SMALL_MAXR ? select_persistent_kernel<SMALL_MAXR>(llr_type, f16)
+                                        : select_persistent_kernel<46>(llr_type, f16);
     const unsigned block = (unsigned)block_threads;
     const size_t lds = (size_t)max_values_per_group * elem;
     cuphyStatus_t s = ensure_lds_attr(func, lds);

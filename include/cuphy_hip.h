@@ -393,7 +393,12 @@ cuphyStatus_t CUPHYWINAPI cuphyHipLDPCBatchDecode(cuphyLDPCDecoder_t decoder, co
  * * Z) (sizes LDS).  The reference packs small-Z code blocks of one TB per
  * CTA (ldpc2_reg_index_fp_desc_dyn_small.cu:115-618); here any configuration
  * can be packed, with the group size chosen per (BG, Z) by the tuning tables
- * of the host layer.  Same workspace contract as cuphyHipLDPCBatchDecode. */
+ * of the host layer.  block_threads = 768 selects row-pair mode instead:
+ * every item is decoded alone (a group's items one after another) by two
+ * lane groups of a 768-thread workgroup that take the column-disjoint block
+ * rows of each barrier segment at the same time (12 waves per code block);
+ * for batches of fewer code blocks than CUs, whose time is that of the
+ * slowest code block.  Same workspace contract as cuphyHipLDPCBatchDecode. */
 cuphyStatus_t CUPHYWINAPI cuphyHipLDPCBatchDecodeGroups(cuphyLDPCDecoder_t decoder,
                                                         const cuphyHipLDPCWorkItem_t* d_items,
                                                         const uint32_t* d_group_first, int num_groups,

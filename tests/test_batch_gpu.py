@@ -66,6 +66,32 @@ def test_batch_bit_exact(dec, dtype):
     assert all(int(w.abs().sum().item()) == 0 for w in b.workspaces), "queue state not returned to zero"
 
 
+@pytest.mark.parametrize("dtype", ["fp32", "fp16"])
+def test_batch_row_pair_equals_one_lane_group(dec, dtype):
+    """Row-pair workgroups (768 threads, two lane groups on the independent
+    rows of each barrier segment) give the hard and soft outputs of the
+    384-thread decode, with early termination (per-CB stopping) and without."""
+    from aerial_amd import _lib as L
+    from aerial_amd.batch import BatchDecoder
+    llr_type = L.CUPHY_R_16F if dtype == "fp16" else L.CUPHY_R_32F
+    for et in (0, L.CUPHY_LDPC_DECODE_EARLY_TERM):
+        print("[case] batch row-pair vs one lane group %s et=%d" % (dtype, et), flush=True)
+        outs = []
+        for pair in (True, False):
+            tbs, _ = make_tbs(dtype, seed=70)
+            soft = [torch.zeros((tb.ncb, V.nsys(tb.bg) * tb.Z), dtype=torch.float16, device="cuda") for tb in tbs]
+            b = BatchDecoder(dec, pair=pair).build(tbs, llr_type=llr_type, fp16_math=(dtype != "fp32"),
+                                                   max_iterations=7, flags=et, soft=soft)
+            assert (max(sb.block for sb in b.subs) == 768) == pair
+            b.launch()
+            torch.cuda.synchronize()
+            outs.append([(tb.out.clone(), s_.clone()) for tb, s_ in zip(tbs, soft)])
+            assert all(int(w.abs().sum().item()) == 0 for w in b.workspaces)
+        for k, ((h1, s1), (h2, s2)) in enumerate(zip(*outs)):
+            assert torch.equal(h1, h2), (MIX[k], et)
+            assert torch.equal(s1.view(torch.int16), s2.view(torch.int16)), (MIX[k], et)
+
+
 def test_batch_order_independent(dec):
     from aerial_amd import _lib as L
     from aerial_amd.batch import BatchDecoder
