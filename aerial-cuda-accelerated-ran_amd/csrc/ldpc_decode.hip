@@ -58,6 +58,18 @@ __global__ __launch_bounds__(384) void ldpc_layered_tb(cuphyLDPCDecodeDesc_t des
         [&](int c) { return tb_buffers<TIn>(desc, first + c); }, s_cb_flags);
 }
 
+// Row-pair form of the transport-block kernel (one code block per
+// workgroup, Z > 32, mb > 8): workgroup b decodes code word b with two lane
+// groups of roundup64(Z) lanes that take the column-disjoint block rows of
+// each barrier segment at the same time (run_iterations PAIR = 2).
+template <int BG, typename T, typename TIn>
+__global__ __launch_bounds__(768) void ldpc_layered_tb_pair(cuphyLDPCDecodeDesc_t desc)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    decode_pair<BG, T, TIn, 46>(reinterpret_cast<T*>(lds_raw), tb_job<TIn>(desc),
+                                tb_buffers<TIn>(desc, (int)blockIdx.x));
+}
+
 // Persistent heterogeneous decoder: each workgroup repeatedly takes the
 // next group of work items from a device queue (atomic counter) and decodes
 // it, so code blocks of any (BG, Z, mb) share one launch.  Group g is items
@@ -192,6 +204,20 @@ void* select_tb_kernel(int bg, cuphyDataType_t llr_type, bool fp16_math)
     return nullptr;
 }
 
+void* select_tb_pair_kernel(int bg, cuphyDataType_t llr_type, bool fp16_math)
+{
+    if (bg == 1) {
+        if (llr_type == CUPHY_R_32F)
+            return fp16_math ? (void*)&ldpc_layered_tb_pair<1, _Float16, float> : (void*)&ldpc_layered_tb_pair<1, float, float>;
+        if (llr_type == CUPHY_R_16F) return (void*)&ldpc_layered_tb_pair<1, _Float16, _Float16>;
+    } else if (bg == 2) {
+        if (llr_type == CUPHY_R_32F)
+            return fp16_math ? (void*)&ldpc_layered_tb_pair<2, _Float16, float> : (void*)&ldpc_layered_tb_pair<2, float, float>;
+        if (llr_type == CUPHY_R_16F) return (void*)&ldpc_layered_tb_pair<2, _Float16, _Float16>;
+    }
+    return nullptr;
+}
+
 void* select_kernel(int bg, cuphyDataType_t llr_type, bool fp16_math, int mb)
 {
     return mb <= SMALL_MAXR ? select_tb_kernel<SMALL_MAXR>(bg, llr_type, fp16_math)
@@ -236,13 +262,17 @@ cuphyStatus_t fill_launch(const cuphyLDPCDecodeDesc_t& desc, launch_geometry& g)
     if (!config_supported(c)) return CUPHY_STATUS_UNSUPPORTED_CONFIG;
     if (desc.num_tbs < 1 || desc.num_tbs > CUPHY_LDPC_DECODE_DESC_MAX_TB) return CUPHY_STATUS_INVALID_ARGUMENT;
     const bool fp16 = use_fp16_math(c);
-    g.func = select_kernel(c.BG, c.llr_type, fp16, c.num_parity_nodes);
+    const int P = tb_group_size(c.Z);
+    // one code block per workgroup and rows past the high-rate core: row-pair
+    // form (two lane groups; the 46-row variant holds one 6-wave workgroup
+    // per CU anyway, the row-pair one 12 waves)
+    const bool pair = P == 1 && c.num_parity_nodes > SMALL_MAXR;
+    g.func = pair ? select_tb_pair_kernel(c.BG, c.llr_type, fp16) : select_kernel(c.BG, c.llr_type, fp16, c.num_parity_nodes);
     if (!g.func) return CUPHY_STATUS_UNSUPPORTED_CONFIG;
     const int nsys = (c.BG == 1) ? 22 : 10;
-    const int P = tb_group_size(c.Z);
     const size_t stride = (size_t)(((nsys + c.num_parity_nodes) * c.Z + 7) & ~7);
     g.lds_bytes = (size_t)P * stride * (fp16 ? 2u : 4u);
-    g.block = dim3((unsigned)((P * c.Z + 63) / 64 * 64));
+    g.block = pair ? dim3((unsigned)(2 * ((c.Z + 63) / 64 * 64))) : dim3((unsigned)((P * c.Z + 63) / 64 * 64));
     g.grid = dim3((unsigned)((desc_codewords(desc) + P - 1) / P));
     return CUPHY_STATUS_SUCCESS;
 }

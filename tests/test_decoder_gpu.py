@@ -278,3 +278,21 @@ def test_launch_descriptor_runs(dec):
     assert rc == 0
     torch.cuda.synchronize()
     assert np.array_equal(out.cpu().numpy().view(np.uint32), h_ref)
+    # mb > 8: the descriptor carries the row-pair kernel (two lane groups of
+    # roundup64(Z) lanes), with the same results as the direct call
+    _log("launch descriptor BG1 Z=192 mb=20 2 CBs (row-pair)")
+    llr, _ = V.make_cbs(1, 192, 20, 2, seed=42, snr_db=1.0)
+    h_ref, _ = run_gpu(dec, 1, 192, 20, llr, soft=False)
+    cfg = LdpcDecodeConfig(1, 192, 20, llr_type=L.CUPHY_R_32F, max_iterations=10, alpha=0.8125)
+    x = torch.from_numpy(np.ascontiguousarray(llr)).cuda()
+    out = torch.zeros((2, cfg.out_words()), dtype=torch.int32, device="cuda")
+    lc = L.LDPCDecodeLaunchConfig()
+    lc.decode_desc = dec.build_desc(cfg, [x], [out])
+    L.check("GetLaunchDescriptor", lib.cuphyErrorCorrectionLDPCDecodeGetLaunchDescriptor(dec.dec, C.byref(lc)))
+    p = lc.kernel_node_params_driver
+    assert p.gridDim.x == 2 and p.blockDim.x == 384 and p.func
+    rc = hip.hipLaunchKernel(p.func, p.gridDim, p.blockDim, p.kernelParams, p.sharedMemBytes,
+                             C.c_void_p(torch.cuda.current_stream().cuda_stream))
+    assert rc == 0
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy().view(np.uint32), h_ref)
