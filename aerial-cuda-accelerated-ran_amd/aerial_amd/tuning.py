@@ -9,7 +9,10 @@ smaller workgroup lets more workgroups share a CU.  The best trade depends
 on registers, LDS and wave scheduling, so `autotune` measures it: for each
 (BG, Z, mb class) it times one persistent launch over a few thousand
 workgroups' worth of code blocks for every candidate P (the largest P for
-each workgroup size of 1..6 waves, and P = 1) and keeps the fastest.  The
+each workgroup size of 1..6 waves, and P = 1; for the full class also the
+768-thread row-pair workgroup, one code block decoded by two lane groups
+on the independent block rows of each barrier segment) and keeps the
+fastest.  The
 result is a JSON table shipped with the package (tuning_gfx950.json);
 without a table entry `model_choice` picks by a residency model.
 
@@ -160,15 +163,21 @@ def autotune(decoder, zs=None, log=sys.stderr):
                 for P in candidates(Z):
                     res[P] = measure(decoder, bg, Z, mb, P)
                 best = max(res, key=lambda p: res[p])
+                best_rate, block = res[best], block_of(best, Z)
                 # one code block per 384-thread workgroup (the latency-regime
                 # launch shape), for reference
                 wide = measure(decoder, bg, Z, mb, 1, block=384)
+                pair = measure(decoder, bg, Z, mb, 1, block=768) if cls == "full" else None
+                if pair is not None and pair > best_rate:
+                    best, best_rate, block = 1, pair, 768
                 entries["%d,%d,%s" % (bg, Z, cls)] = {
-                    "P": best, "block": block_of(best, Z), "cbs_per_s": round(res[best], 1),
+                    "P": best, "block": block, "cbs_per_s": round(best_rate, 1),
                     "model_P": model_choice(bg, Z, mb)[0], "one_per_384_threads": round(wide, 1),
+                    "row_pair_768_threads": round(pair, 1) if pair is not None else None,
                     "candidates": {str(p): round(v, 1) for p, v in res.items()}}
-                print("BG%d Z=%d %s: P=%d (%.3g CB/s; P=1 %.3g; one per 384 threads %.3g)"
-                      % (bg, Z, cls, best, res[best], res[1], wide), file=log, flush=True)
+                print("BG%d Z=%d %s: P=%d block %d (%.3g CB/s; P=1 %.3g; one per 384 threads %.3g; row pair %s)"
+                      % (bg, Z, cls, best, block, best_rate, res[1], wide,
+                         "%.3g" % pair if pair is not None else "-"), file=log, flush=True)
     return entries
 
 

@@ -157,7 +157,7 @@ def test_batch_single_and_empty(dec):
 
 # Packed groups: several code blocks of one (BG, Z, mb) per workgroup.
 PACK_MIX = [(1, 2, 20, 70), (2, 15, 4, 30), (1, 36, 8, 12), (2, 64, 42, 9), (1, 72, 4, 7), (2, 104, 10, 5),
-            (1, 176, 12, 4), (1, 384, 6, 2), (2, 7, 42, 25)]
+            (1, 176, 12, 4), (1, 384, 6, 2), (2, 7, 42, 25), (1, 320, 30, 2)]  # last: a row-pair launch
 
 
 def _max_pack_table(mix):
