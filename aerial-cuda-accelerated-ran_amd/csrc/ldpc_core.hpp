@@ -376,7 +376,7 @@ __device__ __forceinline__ int run_iterations(T* __restrict__ app, const cb_job&
     int it = 0;
     while (it < job.max_iter) {
         ++it;
-        if (cb_iters && active && zlane == 0u) cb_iters[cb] = (uint32_t)it;
+        if (cb_iters && active && zlane == 0u && lgrp == 0u) cb_iters[cb] = (uint32_t)it;
         // Opaque copies of the lane offset and of the shift pointer, renewed
         // per row: the 316 (BG1) per-edge shifts and addresses are reloaded
         // and recomputed where used instead of being hoisted out of the

@@ -679,7 +679,8 @@ int serve_resident_grid(int cu_count)
     const size_t lds = ((size_t)SERVE_LDS_VALUES * 2 + 15) / 16 * 16 + 64u;
     if (ensure_lds_attr(func, lds) != CUPHY_STATUS_SUCCESS) return 0;
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, func, 384, lds) != hipSuccess || per_cu < 1) per_cu = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, func, FUSED_BLOCK, lds) != hipSuccess || per_cu < 1)
+        per_cu = 1;
     return std::max(1, cu_count) * per_cu;
 }
 
@@ -702,7 +703,7 @@ cuphyStatus_t serve_resident_launch(const serve_ring_hdr* ring, const uint32_t* 
     void* args[11] = {const_cast<serve_ring_hdr**>(&ring), const_cast<uint32_t**>(&ring_items), &ring_mask,
                       const_cast<cuphyHipPuschSlotTb_t**>(&jobs), &n_slots, &st, &abandoned, &tbw, &lds_values,
                       &idle_ticks, &life_ticks};
-    if (hipLaunchKernel(func, dim3((unsigned)grid), dim3(384), args, lds, strm) != hipSuccess)
+    if (hipLaunchKernel(func, dim3((unsigned)grid), dim3(FUSED_BLOCK), args, lds, strm) != hipSuccess)
         return CUPHY_STATUS_INTERNAL_ERROR;
     return CUPHY_STATUS_SUCCESS;
 }

@@ -18,7 +18,7 @@ constexpr int FUSED_WS_QUEUE_WORDS = 64;
 // the group of `count` consecutive CBs CB .. CB + count - 1 of the TB,
 // decoded side by side in one workgroup (count * Zc <= 384).
 constexpr int FUSED_MAX_GROUP = 16;
-constexpr int FUSED_BLOCK = 384;
+constexpr int FUSED_BLOCK = 384;  // threads of a fused-kernel workgroup
 __host__ __device__ inline int item_cb(uint32_t it) { return (int)(it & 0xFFFu); }
 __host__ __device__ inline int item_count(uint32_t it) { return (int)((it >> 12) & 0xFu) + 1; }
 __host__ __device__ inline uint32_t make_item(int tb, int cb, int count)
