@@ -3,7 +3,7 @@
 // LLRs and prints one line per workload.  Used for hardware-counter runs
 // (rocprofv3 --pmc), where the program under the profiler should be small.
 //
-//   decode_bench [tb|batch|wide|all] [repetitions]
+//   decode_bench [tb|batch|wide|widepair|all] [repetitions]
 //
 //   tb    : 1 TB x 8 CBs, BG1 Z=384 mb=46, binary32 LLRs, 10 iterations
 //           (the plain per-TB launch of cuphyErrorCorrectionLDPCTransportBlockDecode)
@@ -11,6 +11,8 @@
 //           persistent launch (cuphyHipLDPCBatchDecode)
 //   wide  : 256 CBs BG1 Z=384 mb=46, binary16 LLRs, 10 iterations, one
 //           persistent launch (one code block per CU: the latency case)
+//   widepair : the same in row-pair mode (cuphyHipLDPCBatchDecodeGroups,
+//           768-thread workgroups, one code block per group)
 //
 // LLRs are pseudo-random (+-0.5..8.5, xorshift from a fixed seed): decode
 // time does not depend on them without early termination.
@@ -120,7 +122,9 @@ void run_tb(cuphyLDPCDecoder_t dec, hipStream_t strm, int reps)
     (void)hipFree(d_out);
 }
 
-void run_batch(cuphyLDPCDecoder_t dec, hipStream_t strm, int reps, int ncb, int mb, const char* name)
+// block 0: cuphyHipLDPCBatchDecode; otherwise cuphyHipLDPCBatchDecodeGroups
+// with one code block per group and `block` threads per workgroup
+void run_batch(cuphyLDPCDecoder_t dec, hipStream_t strm, int reps, int ncb, int mb, const char* name, int block = 0)
 {
     const int Z = 384, nval = (22 + mb) * Z, stride = (nval + 7) & ~7;
     std::vector<uint16_t> h((size_t)ncb * stride, 0);
@@ -154,8 +158,17 @@ void run_batch(cuphyLDPCDecoder_t dec, hipStream_t strm, int reps, int ncb, int 
     void* d_ws = nullptr;
     CHECK_HIP(hipMalloc(&d_ws, ws_bytes));
     CHECK_HIP(hipMemset(d_ws, 0, ws_bytes));
+    std::vector<uint32_t> first(ncb + 1);
+    for (int c = 0; c <= ncb; ++c) first[c] = (uint32_t)c;
+    uint32_t* d_first = nullptr;
+    CHECK_HIP(hipMalloc(&d_first, first.size() * sizeof(uint32_t)));
+    CHECK_HIP(hipMemcpy(d_first, first.data(), first.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     auto launch = [&]() {
-        CHECK_CUPHY(cuphyHipLDPCBatchDecode(dec, d_items, ncb, CUPHY_R_16F, 1, nval, mb, d_ws, strm));
+        if (block == 0)
+            CHECK_CUPHY(cuphyHipLDPCBatchDecode(dec, d_items, ncb, CUPHY_R_16F, 1, nval, mb, d_ws, strm));
+        else
+            CHECK_CUPHY(cuphyHipLDPCBatchDecodeGroups(dec, d_items, d_first, ncb, CUPHY_R_16F, 1, stride, mb, block,
+                                                      d_ws, strm));
     };
     launch();
     CHECK_HIP(hipStreamSynchronize(strm));
@@ -171,6 +184,7 @@ void run_batch(cuphyLDPCDecoder_t dec, hipStream_t strm, int reps, int ncb, int 
     std::printf("%s: %d CBs BG1 Z=384 mb=%d fp16 10 it: %.2f us per launch, %.3f Gbps (%d launches)\n", name, ncb,
                 mb, us, gbps, reps);
     (void)hipFree(d_items);
+    (void)hipFree(d_first);
     (void)hipFree(d_ws);
     (void)hipFree(d_llr);
     (void)hipFree(d_out);
@@ -182,8 +196,8 @@ int main(int argc, char** argv)
 {
     const std::string what = argc > 1 ? argv[1] : "all";
     const int reps = argc > 2 ? std::atoi(argv[2]) : 5;
-    if (reps < 1 || (what != "tb" && what != "batch" && what != "wide" && what != "all")) {
-        std::fprintf(stderr, "usage: decode_bench [tb|batch|wide|all] [repetitions >= 1]\n");
+    if (reps < 1 || (what != "tb" && what != "batch" && what != "wide" && what != "widepair" && what != "all")) {
+        std::fprintf(stderr, "usage: decode_bench [tb|batch|wide|widepair|all] [repetitions >= 1]\n");
         return 2;
     }
     cuphyContext_t ctx = nullptr;
@@ -195,6 +209,7 @@ int main(int argc, char** argv)
     if (what == "tb" || what == "all") run_tb(dec, strm, reps);
     if (what == "batch" || what == "all") run_batch(dec, strm, reps, 2800, 4, "batch");
     if (what == "wide" || what == "all") run_batch(dec, strm, reps, 256, 46, "wide");
+    if (what == "widepair" || what == "all") run_batch(dec, strm, reps, 256, 46, "widepair", 768);
     CHECK_HIP(hipStreamDestroy(strm));
     (void)cuphyDestroyLDPCDecoder(dec);
     (void)cuphyDestroyContext(ctx);

@@ -346,11 +346,14 @@ def main():
 
     # ---- roofline of the decoder kernel (VALU-bound) ----------------------
     # dominant kernel = the batched persistent decoder: one launch per step
+    # (row-pair workgroups of 768 threads in the latency regime)
+    kernel_name = ("nrldpc::ldpc_layered_persistent_pair<_Float16, _Float16, 46>"
+                   if any(sb.block == 768 for sb in bdec.subs) else "nrldpc::ldpc_layered_persistent<_Float16, _Float16>")
     achieved_tops = edge_updates * OPS_PER_EDGE / (batched_ms * 1e-3) / 1e12
     plain_tops = edge_updates * OPS_PER_EDGE / (kernel_ms * 1e-3) / 1e12
     roofline = {"bound": "valu", "achieved": round(achieved_tops, 3), "peak": round(VALU_PEAK_TOPS, 2),
                 "unit": "TOP/s", "frac": round(achieved_tops / VALU_PEAK_TOPS, 4), "traffic": measured_traffic(),
-                "kernel": "nrldpc::ldpc_layered_persistent<_Float16, _Float16>",
+                "kernel": kernel_name,
                 "work": "%d edge updates x %d VALU ops per launch" % (edge_updates, OPS_PER_EDGE),
                 "avg_launch_us": round(batched_ms * 1e3, 2),
                 "plain_kernel": {"achieved": round(plain_tops, 3), "avg_launch_us": round(kernel_ms * 1e3 / len(descs), 2)}}
