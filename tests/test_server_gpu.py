@@ -272,7 +272,7 @@ def test_descrambled_requests(dec, resident):
     from aerial_amd.slot import SlotDecoder, SlotTb
     print("[case] server: descrambled requests, resident=%s" % resident, flush=True)
     specs = [(1, 384, 4, 8448, 8, 1, 9088, 12.0), (2, 104, 1, 1040, 4, 1, 2080, 8.0),
-             (1, 208, 3, 4576, 2, 1, 9152, 8.0)]
+             (1, 208, 3, 4576, 2, 1, 9152, 8.0), (2, 64, 4, 640, 2, 1, 1280, 8.0)]  # last: one item of 4 CBs
     tt = TraceTemplates(specs, seed=11)
     srv = HarqServer(dec, pool_bytes=1 << 30, max_inflight=2, max_tbs=16, max_cbs=256, resident=resident)
     reqs = []
