@@ -494,8 +494,13 @@ __device__ __forceinline__ void complete_cb(const cuphyHipPuschSlotTb_t& job, ui
 // each TB uses the pair at its first slot (TB-CRC accumulator and count of
 // finished CBs; see complete_cb).  The next item index is fetched while the current CB is
 // worked on.  The queue is returned to 0 by the last workgroup to leave.
+// The high-rate variant (MAXR <= 8) is held to 4 waves per SIMD (at most
+// 128 VGPRs): a 384-thread workgroup puts its 6 waves 2/2/1/1 on the CU's
+// SIMDs, so a second workgroup on the CU needs 4 wave slots on two SIMDs;
+// at its natural 157 VGPRs (3 slots) one workgroup per CU ran.
 template <typename TI, int MAXR, bool GROUPED>
-__global__ __launch_bounds__(384) void pusch_slot_fused(const cuphyHipPuschSlotTb_t* __restrict__ tbs, int n_tbs,
+__global__ __launch_bounds__(384) __attribute__((amdgpu_waves_per_eu(MAXR <= SMALL_MAXR ? 4 : 1)))
+void pusch_slot_fused(const cuphyHipPuschSlotTb_t* __restrict__ tbs, int n_tbs,
                                                         const uint32_t* __restrict__ items, int n_items,
                                                         int lds_values, uint32_t* __restrict__ ws)
 {
