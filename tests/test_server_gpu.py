@@ -294,7 +294,8 @@ def test_descrambled_requests(dec, resident):
         chk = HarqServer(dec, pool_bytes=1 << 26, max_inflight=1, max_tbs=4, max_cbs=16, resident=True)
         p, llr, o = reqs[1]
         r = chk.request(399, p, llr, o, tt.mbs[1][0], descramble=True)
-        r.prm.encodedSize = r.prm.G = 500000  # 15,627 sequence words for its one CB
+        r.prm.encodedSize = r.prm.G = 1000000  # 31,252 sequence words for its one CB
+        # beyond the resident launch's LDS: one 68 x 384 array and 2048 words
         assert lds_values(r.prm, tt.mbs[1][0], True) > 68 * 384 + 2 * 2048
         rc = L.lib().cuphyHipHarqServerSubmit(chk.h, r)
         assert chk.stats()["submitted"] == 0
