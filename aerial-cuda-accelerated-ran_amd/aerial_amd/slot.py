@@ -162,6 +162,9 @@ class SlotDecoder:
         self.llr_type = _LLR_TYPES[dtypes.pop()]
         n = len(tbs)
         assert n <= 65536
+        # work items hold a 12-bit CB index (make_item)
+        assert all(tb.params.num_CBs <= 4096 for tb in tbs), "at most 4096 code blocks per TB"
+
         parr = (L.PerTbParams * n)(*[tb.params for tb in tbs])
         self._dev_params = torch.from_numpy(np.frombuffer(bytes(parr), dtype=np.uint8).copy()).cuda()
         psize = C.sizeof(L.PerTbParams)
