@@ -57,7 +57,8 @@ def build(verbose=True):
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
     os.replace(tmp, LIB)
-    cmd = ["hipcc", "-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-O2", "-std=c++17",
+    # the driver holds one device kernel of its own (the VALU peak probe)
+    cmd = ["hipcc", "-x", "hip"] + HIP_ONLY + ["-O2", "-std=c++17",
            "-I" + os.path.join(ROOT, "include"), os.path.join(CSRC, "decode_bench.cpp"), "-o", BENCH + ".tmp",
            "-L" + OUT_DIR, "-lcuphy_hip", "-Wl,-rpath,$ORIGIN", "-L/opt/rocm/lib", "-lamdhip64"]
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -3,7 +3,7 @@
 // LLRs and prints one line per workload.  Used for hardware-counter runs
 // (rocprofv3 --pmc), where the program under the profiler should be small.
 //
-//   decode_bench [tb|batch|wide|widepair|all] [repetitions]
+//   decode_bench [tb|batch|wide|widepair|all|valupeak] [repetitions]
 //
 //   tb    : 1 TB x 8 CBs, BG1 Z=384 mb=46, binary32 LLRs, 10 iterations
 //           (the plain per-TB launch of cuphyErrorCorrectionLDPCTransportBlockDecode)
@@ -13,6 +13,10 @@
 //           persistent launch (one code block per CU: the latency case)
 //   widepair : the same in row-pair mode (cuphyHipLDPCBatchDecodeGroups,
 //           768-thread workgroups, one code block per group)
+//
+//   valupeak : VALU issue rate of the device in 32-bit lane-ops per second
+//           (v_add_u32 in 8 independent chains per lane, 8 workgroups of 256
+//           threads per CU): the denominator of the decoder's VALU roofline
 //
 // LLRs are pseudo-random (+-0.5..8.5, xorshift from a fixed seed): decode
 // time does not depend on them without early termination.
@@ -190,14 +194,67 @@ void run_batch(cuphyLDPCDecoder_t dec, hipStream_t strm, int reps, int ncb, int 
     (void)hipFree(d_out);
 }
 
+constexpr int PEAK_UNROLL = 4;
+
+// 8 chains x PEAK_UNROLL adds per loop trip; the adds are inline asm so that
+// the count of VALU instructions is exactly what is timed.
+__global__ __launch_bounds__(256) void valu_peak_kernel(uint32_t* out, int iters)
+{
+    uint32_t a0 = threadIdx.x, a1 = a0 + 1, a2 = a0 + 2, a3 = a0 + 3, a4 = a0 + 4, a5 = a0 + 5, a6 = a0 + 6,
+             a7 = a0 + 7;
+    const uint32_t b = blockIdx.x | 1u;
+    for (int i = 0; i < iters; ++i) {
+#pragma unroll
+        for (int u = 0; u < PEAK_UNROLL; ++u) {
+            asm volatile("v_add_u32 %0, %0, %1" : "+v"(a0) : "v"(b));
+            asm volatile("v_add_u32 %0, %0, %1" : "+v"(a1) : "v"(b));
+            asm volatile("v_add_u32 %0, %0, %1" : "+v"(a2) : "v"(b));
+            asm volatile("v_add_u32 %0, %0, %1" : "+v"(a3) : "v"(b));
+            asm volatile("v_add_u32 %0, %0, %1" : "+v"(a4) : "v"(b));
+            asm volatile("v_add_u32 %0, %0, %1" : "+v"(a5) : "v"(b));
+            asm volatile("v_add_u32 %0, %0, %1" : "+v"(a6) : "v"(b));
+            asm volatile("v_add_u32 %0, %0, %1" : "+v"(a7) : "v"(b));
+        }
+    }
+    out[blockIdx.x * blockDim.x + threadIdx.x] = a0 ^ a1 ^ a2 ^ a3 ^ a4 ^ a5 ^ a6 ^ a7;
+}
+
+void run_valu_peak(hipStream_t strm, int reps)
+{
+    int dev = 0, cus = 0, clk_khz = 0;
+    CHECK_HIP(hipGetDevice(&dev));
+    CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    CHECK_HIP(hipDeviceGetAttribute(&clk_khz, hipDeviceAttributeClockRate, dev));
+    const int blocks = cus * 8, threads = 256, iters = 8192;
+    uint32_t* d_out = nullptr;
+    CHECK_HIP(hipMalloc(&d_out, (size_t)blocks * threads * sizeof(uint32_t)));
+    hipLaunchKernelGGL(valu_peak_kernel, dim3(blocks), dim3(threads), 0, strm, d_out, iters);
+    CHECK_HIP(hipStreamSynchronize(strm));
+    timer t;
+    CHECK_HIP(hipEventRecord(t.a, strm));
+    for (int r = 0; r < reps; ++r)
+        hipLaunchKernelGGL(valu_peak_kernel, dim3(blocks), dim3(threads), 0, strm, d_out, iters);
+    CHECK_HIP(hipEventRecord(t.b, strm));
+    CHECK_HIP(hipEventSynchronize(t.b));
+    float ms = 0;
+    CHECK_HIP(hipEventElapsedTime(&ms, t.a, t.b));
+    const double lane_ops = (double)blocks * threads * iters * 8 * PEAK_UNROLL * reps;
+    const double rate = lane_ops / (ms * 1e-3);
+    std::printf("valupeak: %d CUs, reported peak clock %.0f MHz: %.2f T 32-bit lane-ops/s "
+                "(%.1f lane-ops per CU per clock at the reported clock; %d launches)\n",
+                cus, clk_khz / 1e3, rate / 1e12, rate / cus / (clk_khz * 1e3), reps);
+    (void)hipFree(d_out);
+}
+
 }  // namespace
 
 int main(int argc, char** argv)
 {
     const std::string what = argc > 1 ? argv[1] : "all";
     const int reps = argc > 2 ? std::atoi(argv[2]) : 5;
-    if (reps < 1 || (what != "tb" && what != "batch" && what != "wide" && what != "widepair" && what != "all")) {
-        std::fprintf(stderr, "usage: decode_bench [tb|batch|wide|widepair|all] [repetitions >= 1]\n");
+    if (reps < 1 || (what != "tb" && what != "batch" && what != "wide" && what != "widepair" && what != "all" &&
+                     what != "valupeak")) {
+        std::fprintf(stderr, "usage: decode_bench [tb|batch|wide|widepair|all|valupeak] [repetitions >= 1]\n");
         return 2;
     }
     cuphyContext_t ctx = nullptr;
@@ -210,6 +267,7 @@ int main(int argc, char** argv)
     if (what == "batch" || what == "all") run_batch(dec, strm, reps, 2800, 4, "batch");
     if (what == "wide" || what == "all") run_batch(dec, strm, reps, 256, 46, "wide");
     if (what == "widepair" || what == "all") run_batch(dec, strm, reps, 256, 46, "widepair", 768);
+    if (what == "valupeak") run_valu_peak(strm, reps);
     CHECK_HIP(hipStreamDestroy(strm));
     (void)cuphyDestroyLDPCDecoder(dec);
     (void)cuphyDestroyContext(ctx);

@@ -28,7 +28,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "aerial-cuda-accelerated-ran_amd"))
 
 METRIC = "decoded info-bit Gbps @10 iters (batched vs plain); p99 TB latency vs 500µs"
-VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # 32-bit VALU lane-ops/s: 256 CUs x 64 lanes x 2.4 GHz
+# 32-bit VALU lane-ops/s: 256 CUs x 128 lane-ops per clock x 2.4 GHz (the published 157.3 TFLOP/s FP32
+# vector rate counts an FMA as 2 flops; a v_add_u32 issue probe, decode_bench valupeak, measures 62.6 T)
+VALU_PEAK_TOPS = 256 * 128 * 2.4e9 / 1e12
 OPS_PER_EDGE = 8                           # VALU lane-ops per edge update (algorithmic count)
 ITERS = 10
 
