@@ -14,9 +14,10 @@
 //   widepair : the same in row-pair mode (cuphyHipLDPCBatchDecodeGroups,
 //           768-thread workgroups, one code block per group)
 //
-//   valupeak : VALU issue rate of the device in 32-bit lane-ops per second
-//           (v_add_u32 in 8 independent chains per lane, 8 workgroups of 256
-//           threads per CU): the denominator of the decoder's VALU roofline
+//   valupeak : VALU issue rate of the device in lane-ops per second for the
+//           instruction types of the decoder's row body (each in 8 independent
+//           chains per lane, 8 workgroups of 256 threads per CU); v_add_u32
+//           gives the denominator of the decoder's VALU roofline
 //
 // LLRs are pseudo-random (+-0.5..8.5, xorshift from a fixed seed): decode
 // time does not depend on them without early termination.
@@ -196,27 +197,60 @@ void run_batch(cuphyLDPCDecoder_t dec, hipStream_t strm, int reps, int ncb, int 
 
 constexpr int PEAK_UNROLL = 4;
 
-// 8 chains x PEAK_UNROLL adds per loop trip; the adds are inline asm so that
-// the count of VALU instructions is exactly what is timed.
+// One VALU instruction of the probe (the operand registers hold 32-bit
+// values; for the 16-bit forms only the low halves are used): K selects it.
+template <int K>
+__device__ __forceinline__ void peak_op(uint32_t& a, uint32_t b)
+{
+    if constexpr (K == 0) asm volatile("v_add_u32 %0, %0, %1" : "+v"(a) : "v"(b));
+    if constexpr (K == 1) asm volatile("v_add_f16 %0, %0, %1" : "+v"(a) : "v"(b));
+    if constexpr (K == 2) asm volatile("v_pk_add_f16 %0, %0, %1" : "+v"(a) : "v"(b));
+    if constexpr (K == 3) asm volatile("v_min_f16 %0, %0, %1" : "+v"(a) : "v"(b));
+    if constexpr (K == 4) asm volatile("v_med3_f16 %0, %0, %1, %1" : "+v"(a) : "v"(b));
+    if constexpr (K == 5) asm volatile("v_lshl_add_u32 %0, %1, 1, %0" : "+v"(a) : "v"(b));
+    if constexpr (K == 6) asm volatile("v_add3_u32 %0, %0, %1, %1" : "+v"(a) : "v"(b));
+    if constexpr (K == 7) asm volatile("v_cndmask_b32 %0, %0, %1, vcc" : "+v"(a) : "v"(b) : "vcc");
+}
+constexpr const char* PEAK_NAMES[8] = {"v_add_u32", "v_add_f16", "v_pk_add_f16", "v_min_f16",
+                                       "v_med3_f16", "v_lshl_add_u32", "v_add3_u32", "v_cndmask_b32"};
+
+// 8 chains x PEAK_UNROLL instructions per loop trip; the instructions are
+// inline asm so that the count of VALU instructions is exactly what is timed.
+template <int K>
 __global__ __launch_bounds__(256) void valu_peak_kernel(uint32_t* out, int iters)
 {
-    uint32_t a0 = threadIdx.x, a1 = a0 + 1, a2 = a0 + 2, a3 = a0 + 3, a4 = a0 + 4, a5 = a0 + 5, a6 = a0 + 6,
-             a7 = a0 + 7;
+    uint32_t a[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) a[c] = threadIdx.x + (uint32_t)c;
     const uint32_t b = blockIdx.x | 1u;
     for (int i = 0; i < iters; ++i) {
 #pragma unroll
         for (int u = 0; u < PEAK_UNROLL; ++u) {
-            asm volatile("v_add_u32 %0, %0, %1" : "+v"(a0) : "v"(b));
-            asm volatile("v_add_u32 %0, %0, %1" : "+v"(a1) : "v"(b));
-            asm volatile("v_add_u32 %0, %0, %1" : "+v"(a2) : "v"(b));
-            asm volatile("v_add_u32 %0, %0, %1" : "+v"(a3) : "v"(b));
-            asm volatile("v_add_u32 %0, %0, %1" : "+v"(a4) : "v"(b));
-            asm volatile("v_add_u32 %0, %0, %1" : "+v"(a5) : "v"(b));
-            asm volatile("v_add_u32 %0, %0, %1" : "+v"(a6) : "v"(b));
-            asm volatile("v_add_u32 %0, %0, %1" : "+v"(a7) : "v"(b));
+#pragma unroll
+            for (int c = 0; c < 8; ++c) peak_op<K>(a[c], b);
         }
     }
-    out[blockIdx.x * blockDim.x + threadIdx.x] = a0 ^ a1 ^ a2 ^ a3 ^ a4 ^ a5 ^ a6 ^ a7;
+    out[blockIdx.x * blockDim.x + threadIdx.x] = a[0] ^ a[1] ^ a[2] ^ a[3] ^ a[4] ^ a[5] ^ a[6] ^ a[7];
+}
+
+template <int K>
+void run_valu_peak_one(hipStream_t strm, int reps, int cus, int clk_khz, uint32_t* d_out)
+{
+    const int blocks = cus * 8, threads = 256, iters = 8192;
+    hipLaunchKernelGGL(valu_peak_kernel<K>, dim3(blocks), dim3(threads), 0, strm, d_out, iters);
+    CHECK_HIP(hipStreamSynchronize(strm));
+    timer t;
+    CHECK_HIP(hipEventRecord(t.a, strm));
+    for (int r = 0; r < reps; ++r)
+        hipLaunchKernelGGL(valu_peak_kernel<K>, dim3(blocks), dim3(threads), 0, strm, d_out, iters);
+    CHECK_HIP(hipEventRecord(t.b, strm));
+    CHECK_HIP(hipEventSynchronize(t.b));
+    float ms = 0;
+    CHECK_HIP(hipEventElapsedTime(&ms, t.a, t.b));
+    const double lane_ops = (double)blocks * threads * iters * 8 * PEAK_UNROLL * reps;
+    const double rate = lane_ops / (ms * 1e-3);
+    std::printf("valupeak %-15s: %6.2f T lane-ops/s (%5.1f lane-ops per CU per clock at the reported %.0f MHz)\n",
+                PEAK_NAMES[K], rate / 1e12, rate / cus / (clk_khz * 1e3), clk_khz / 1e3);
 }
 
 void run_valu_peak(hipStream_t strm, int reps)
@@ -225,24 +259,16 @@ void run_valu_peak(hipStream_t strm, int reps)
     CHECK_HIP(hipGetDevice(&dev));
     CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     CHECK_HIP(hipDeviceGetAttribute(&clk_khz, hipDeviceAttributeClockRate, dev));
-    const int blocks = cus * 8, threads = 256, iters = 8192;
     uint32_t* d_out = nullptr;
-    CHECK_HIP(hipMalloc(&d_out, (size_t)blocks * threads * sizeof(uint32_t)));
-    hipLaunchKernelGGL(valu_peak_kernel, dim3(blocks), dim3(threads), 0, strm, d_out, iters);
-    CHECK_HIP(hipStreamSynchronize(strm));
-    timer t;
-    CHECK_HIP(hipEventRecord(t.a, strm));
-    for (int r = 0; r < reps; ++r)
-        hipLaunchKernelGGL(valu_peak_kernel, dim3(blocks), dim3(threads), 0, strm, d_out, iters);
-    CHECK_HIP(hipEventRecord(t.b, strm));
-    CHECK_HIP(hipEventSynchronize(t.b));
-    float ms = 0;
-    CHECK_HIP(hipEventElapsedTime(&ms, t.a, t.b));
-    const double lane_ops = (double)blocks * threads * iters * 8 * PEAK_UNROLL * reps;
-    const double rate = lane_ops / (ms * 1e-3);
-    std::printf("valupeak: %d CUs, reported peak clock %.0f MHz: %.2f T 32-bit lane-ops/s "
-                "(%.1f lane-ops per CU per clock at the reported clock; %d launches)\n",
-                cus, clk_khz / 1e3, rate / 1e12, rate / cus / (clk_khz * 1e3), reps);
+    CHECK_HIP(hipMalloc(&d_out, (size_t)cus * 8 * 256 * sizeof(uint32_t)));
+    run_valu_peak_one<0>(strm, reps, cus, clk_khz, d_out);
+    run_valu_peak_one<1>(strm, reps, cus, clk_khz, d_out);
+    run_valu_peak_one<2>(strm, reps, cus, clk_khz, d_out);
+    run_valu_peak_one<3>(strm, reps, cus, clk_khz, d_out);
+    run_valu_peak_one<4>(strm, reps, cus, clk_khz, d_out);
+    run_valu_peak_one<5>(strm, reps, cus, clk_khz, d_out);
+    run_valu_peak_one<6>(strm, reps, cus, clk_khz, d_out);
+    run_valu_peak_one<7>(strm, reps, cus, clk_khz, d_out);
     (void)hipFree(d_out);
 }
 

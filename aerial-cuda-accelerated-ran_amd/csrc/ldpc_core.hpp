@@ -173,7 +173,9 @@ __device__ __forceinline__ void process_row(float*, unsigned char* __restrict__ 
     static_for<D>([&](auto J) {
         constexpr int j = decltype(J)::value;
         const float a = __builtin_fabsf(t[j]);
-        m2 = __builtin_amdgcn_fmed3f(m1, a, m2);  // min(m2, max(m1, a)); m1 <= m2
+        // min(m2, max(m1, a)) (= med3(m1, a, m2) as m1 <= m2, but as two
+        // full-rate operations: v_med3 issues at a quarter of v_min's rate)
+        m2 = __builtin_fminf(m2, __builtin_fmaxf(m1, a));
         m1 = __builtin_fminf(m1, a);
         par ^= f2u(t[j]);
     });
@@ -225,7 +227,7 @@ __device__ __forceinline__ void process_row(_Float16*, unsigned char* __restrict
     static_for<D>([&](auto J) {
         constexpr int j = decltype(J)::value;
         const _Float16 a = __builtin_fabsf16(t[j]);
-        m2 = __builtin_amdgcn_fmed3h(m1, a, m2);
+        m2 = __builtin_fminf16(m2, __builtin_fmaxf16(m1, a));  // as in the binary32 row
         m1 = __builtin_fminf16(m1, a);
         par ^= h2u(t[j]);
     });
