@@ -32,6 +32,10 @@ METRIC = "decoded info-bit Gbps @10 iters (batched vs plain); p99 TB latency vs 
 # vector rate counts an FMA as 2 flops; a v_add_u32 issue probe, decode_bench valupeak, measures 62.6 T)
 VALU_PEAK_TOPS = 256 * 128 * 2.4e9 / 1e12
 OPS_PER_EDGE = 8                           # VALU lane-ops per edge update (algorithmic count)
+# LDS bytes per edge update with binary16 APP values: one read and one write of 2 B
+LDS_BYTES_PER_EDGE = 4
+# LDS bandwidth: 256 CUs x 128 B per clock x 2.4 GHz (nominal)
+LDS_PEAK_TBS = 256 * 128 * 2.4e9 / 1e12
 ITERS = 10
 
 
@@ -207,10 +211,30 @@ def cpu_baseline(tbs, budget_s):
         if t_total * ncores >= budget_s or passes >= 50:
             break
     gbps = bits_per_pass * passes / t_total / 1e9
+    # one pass on one core, for the per-core rate
+    t0 = time.perf_counter()
+    for cfg, llr, _ in jobs:
+        O.decode_cbs(cfg, llr, nthreads=1)
+    gbps_1 = bits_per_pass / (time.perf_counter() - t0) / 1e9
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": round(gbps, 4), "unit": "Gbps", "cores": ncores, "kind": "port",
+            "value_1_core": round(gbps_1, 5), "cpu_model": model,
             "sample": "%d pass(es) over the slot's %d code blocks (%d info bits/pass), oracle binary32 "
                       "layered NMS, 10 iterations, one thread per core" %
                       (passes, sum(tb.ncb for tb in tbs), bits_per_pass)}
+
+
+def lds_fraction(edges, ms):
+    """Algorithmic LDS bandwidth of a binary16 decode (APP read + write per
+    edge update) against the nominal LDS peak."""
+    tbs = edges * LDS_BYTES_PER_EDGE / (ms * 1e-3) / 1e12
+    return {"achieved": round(tbs, 3), "peak": round(LDS_PEAK_TBS, 2), "unit": "TB/s",
+            "frac": round(tbs / LDS_PEAK_TBS, 4)}
 
 
 def measured_traffic():
@@ -358,6 +382,7 @@ def main():
                 "kernel": kernel_name,
                 "work": "%d edge updates x %d VALU ops per launch" % (edge_updates, OPS_PER_EDGE),
                 "avg_launch_us": round(batched_ms * 1e3, 2),
+                "lds": lds_fraction(edge_updates, batched_ms),
                 "plain_kernel": {"achieved": round(plain_tops, 3), "avg_launch_us": round(kernel_ms * 1e3 / len(descs), 2)}}
 
     config1 = None
@@ -446,6 +471,7 @@ def config3_section(dec, stream, steps):
                "plain_ms": round(c3_plain_ms, 4), "plain_gbps": round(c3_bits / (c3_plain_ms * 1e-3) / 1e9, 3),
                "batched_ms": round(c3_batch_ms, 4), "batched_gbps": round(c3_bits / (c3_batch_ms * 1e-3) / 1e9, 3),
                "batched_valu_frac": round(c3_edges * OPS_PER_EDGE / (c3_batch_ms * 1e-3) / 1e12 / VALU_PEAK_TOPS, 4),
+               "batched_lds_frac": lds_fraction(c3_edges, c3_batch_ms)["frac"],
                "slot_budget_ms": 0.5}
     return config3
 
