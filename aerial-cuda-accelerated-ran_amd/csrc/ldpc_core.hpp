@@ -134,12 +134,51 @@ __device__ __forceinline__ T& lds_at(unsigned char* base, uint32_t off)
     return *reinterpret_cast<T*>(base + off);
 }
 
+// LDS addressed from LDS address 0 (row-pair kernels, whose APP values start
+// there: they hold no static LDS, which the host checks before a launch).  A
+// known base lets an edge's constant column offset go into the LDS
+// instruction's offset field.
+typedef __attribute__((address_space(3))) unsigned char lds_byte;
+__device__ __forceinline__ lds_byte* lds_origin() { return (lds_byte*)(uintptr_t)0u; }
+template <typename T> __device__ __forceinline__ T lds_ld(unsigned char* b, uint32_t off) { return lds_at<T>(b, off); }
+template <typename T> __device__ __forceinline__ void lds_st(unsigned char* b, uint32_t off, T v) { lds_at<T>(b, off) = v; }
+template <typename T> __device__ __forceinline__ T lds_ld(lds_byte* b, uint32_t off)
+{
+    return *reinterpret_cast<const __attribute__((address_space(3))) T*>(b + off);
+}
+template <typename T> __device__ __forceinline__ void lds_st(lds_byte* b, uint32_t off, T v)
+{
+    *reinterpret_cast<__attribute__((address_space(3))) T*>(b + off) = v;
+}
+
+// Lane-major APP layout (row-pair kernels): lane z's values of every column
+// at z * pitch + column * sizeof(T), the pitch an odd number of 4-byte words
+// so that 64 consecutive lanes touch 64 distinct LDS banks.  The column term
+// is then a constant, which the LDS instruction takes as its offset.
+__host__ __device__ constexpr uint32_t lm_pitch_bytes(int ncol, int elem)
+{
+    return ((((uint32_t)(ncol * elem) + 3u) >> 2) | 1u) << 2;
+}
+
 // ---------------------------------------------------------------------------
 // Row addressing: lane z of a code block whose APP values start at byte
 // `base` reads APP[col][(z + s) mod Z] at base + col*Zb + rot_index(zb, sb, Zb),
 // with zb, sb, Zb the lane, shift and lifting size in bytes (x sizeof(T)):
 // add, subtract, min and one three-operand add per edge.
 // ---------------------------------------------------------------------------
+
+// Byte offset of edge E's APP value for the lane.  Column-major (LM false):
+// zb, Zb = lane and lifting size x sizeof(T).  Lane-major (LM true): zb, Zb
+// = lane and lifting size x pitch Pb.
+template <int BG, int E, typename T, bool LM>
+__device__ __forceinline__ uint32_t edge_off(shift_ptr sh, uint32_t zb, uint32_t Zb, uint32_t Pb)
+{
+    using BGT = bg_traits<BG>;
+    if constexpr (LM)
+        return rot_index(zb, sh[E] * Pb, Zb) + (uint32_t)BGT::col(E) * (uint32_t)sizeof(T);
+    else
+        return (uint32_t)BGT::col(E) * Zb + rot_index(zb, sh[E] * (uint32_t)sizeof(T), Zb);
+}
 
 // ---------------------------------------------------------------------------
 // binary32 rows: compressed C2V per (row, lane) -- normalised min1/min2 and a
@@ -148,10 +187,10 @@ __device__ __forceinline__ T& lds_at(unsigned char* base, uint32_t off)
 // subtract 1, min/min2/parity 3, new message + write-back 6 VALU ops.
 // ---------------------------------------------------------------------------
 // RS: the row's state slot (R, or its slot within a row-pair lane group).
-template <int BG, int R, int NR, int RS = R>
-__device__ __forceinline__ void process_row(float*, unsigned char* __restrict__ base, shift_ptr sh, uint32_t zb,
+template <int BG, int R, int NR, int RS = R, bool LM = false, typename BP = unsigned char*>
+__device__ __forceinline__ void process_row(float*, BP __restrict__ base, shift_ptr sh, uint32_t zb,
                                             uint32_t Zb, float alpha, float (&st_n1)[NR], float (&st_n2)[NR],
-                                            uint32_t (&st_meta)[NR])
+                                            uint32_t (&st_meta)[NR], uint32_t Pb = 0)
 {
     using BGT = bg_traits<BG>;
     constexpr int E0 = BGT::row_start(R);
@@ -163,10 +202,10 @@ __device__ __forceinline__ void process_row(float*, unsigned char* __restrict__ 
     const uint32_t oidx = om >> 27;
     static_for<D>([&](auto J) {
         constexpr int j = decltype(J)::value;
-        off[j] = (uint32_t)BGT::col(E0 + j) * Zb + rot_index(zb, sh[E0 + j] * 4u, Zb);
+        off[j] = edge_off<BG, E0 + j, float, LM>(sh, zb, Zb, Pb);
         const float mag = (oidx == (uint32_t)j) ? on2 : on1;
         const float old = u2f(f2u(mag) | ((om << (32 - D + j)) & 0x80000000u));
-        t[j] = lds_at<float>(base, off[j]) - old;
+        t[j] = lds_ld<float>(base, off[j]) - old;
     });
     float m1 = __builtin_huge_valf(), m2 = __builtin_huge_valf();
     uint32_t par = 0;
@@ -190,7 +229,7 @@ __device__ __forceinline__ void process_row(float*, unsigned char* __restrict__ 
         const float mag = is_min ? a2 : a1;
         idx = is_min ? (uint32_t)j : idx;
         const uint32_t cb = f2u(mag) | ((f2u(t[j]) ^ par) & 0x80000000u);
-        lds_at<float>(base, off[j]) = t[j] + u2f(cb);
+        lds_st<float>(base, off[j], t[j] + u2f(cb));
         fs = __builtin_amdgcn_alignbit(fs, cb, 31);  // (fs << 1) | sign(c_j)
     });
     st_n1[RS] = a1;
@@ -205,9 +244,10 @@ __device__ __forceinline__ void process_row(float*, unsigned char* __restrict__ 
 // ---------------------------------------------------------------------------
 // S0: message slot of the row's first edge (its edge index, or its slot
 // within a row-pair lane group).
-template <int BG, int R, int NW, int S0 = bg_traits<BG>::row_start(R)>
-__device__ __forceinline__ void process_row(_Float16*, unsigned char* __restrict__ base, shift_ptr sh, uint32_t zb,
-                                            uint32_t Zb, _Float16 alpha, half2_t (&st)[NW])
+template <int BG, int R, int NW, int S0 = bg_traits<BG>::row_start(R), bool LM = false,
+          typename BP = unsigned char*>
+__device__ __forceinline__ void process_row(_Float16*, BP __restrict__ base, shift_ptr sh, uint32_t zb,
+                                            uint32_t Zb, _Float16 alpha, half2_t (&st)[NW], uint32_t Pb = 0)
 {
     using BGT = bg_traits<BG>;
     constexpr int E0 = BGT::row_start(R);
@@ -218,9 +258,9 @@ __device__ __forceinline__ void process_row(_Float16*, unsigned char* __restrict
         constexpr int j = decltype(J)::value;
         constexpr int e = E0 + j;
         constexpr int sl = S0 + j;
-        off[j] = (uint32_t)BGT::col(e) * Zb + rot_index(zb, sh[e] * 2u, Zb);
+        off[j] = edge_off<BG, e, _Float16, LM>(sh, zb, Zb, Pb);
         const _Float16 old = (sl & 1) ? st[sl >> 1].y : st[sl >> 1].x;
-        t[j] = lds_at<_Float16>(base, off[j]) - old;
+        t[j] = lds_ld<_Float16>(base, off[j]) - old;
     });
     _Float16 m1 = (_Float16)__builtin_huge_valf(), m2 = (_Float16)__builtin_huge_valf();
     uint32_t par = 0;
@@ -238,7 +278,7 @@ __device__ __forceinline__ void process_row(_Float16*, unsigned char* __restrict
         constexpr int sl = S0 + j;
         const _Float16 mag = (__builtin_fabsf16(t[j]) == m1) ? a2 : a1;
         const _Float16 c = u2h((uint16_t)(h2u(mag) | ((h2u(t[j]) ^ par) & 0x8000u)));
-        lds_at<_Float16>(base, off[j]) = t[j] + c;
+        lds_st<_Float16>(base, off[j], (_Float16)(t[j] + c));
         if (sl & 1)
             st[sl >> 1].y = c;
         else
@@ -247,8 +287,9 @@ __device__ __forceinline__ void process_row(_Float16*, unsigned char* __restrict
 }
 
 // Parity of check row R, lane z: hard bit = APP < 0 (syndrome_check.m:38-40).
-template <int BG, int R, typename T>
-__device__ __forceinline__ uint32_t row_syndrome(unsigned char* __restrict__ base, shift_ptr sh, uint32_t zb, uint32_t Zb)
+template <int BG, int R, typename T, bool LM = false, typename BP = unsigned char*>
+__device__ __forceinline__ uint32_t row_syndrome(BP __restrict__ base, shift_ptr sh, uint32_t zb, uint32_t Zb,
+                                                 uint32_t Pb = 0)
 {
     using BGT = bg_traits<BG>;
     constexpr int E0 = BGT::row_start(R);
@@ -256,8 +297,8 @@ __device__ __forceinline__ uint32_t row_syndrome(unsigned char* __restrict__ bas
     uint32_t par = 0;
     static_for<D>([&](auto J) {
         constexpr int j = decltype(J)::value;
-        const uint32_t off = (uint32_t)BGT::col(E0 + j) * Zb + rot_index(zb, sh[E0 + j] * (uint32_t)sizeof(T), Zb);
-        par ^= (uint32_t)(lds_at<T>(base, off) < (T)0);
+        const uint32_t off = edge_off<BG, E0 + j, T, LM>(sh, zb, Zb, Pb);
+        par ^= (uint32_t)(lds_ld<T>(base, off) < (T)0);
     });
     return par;
 }
@@ -370,9 +411,19 @@ __device__ __forceinline__ int run_iterations(T* __restrict__ app, const cb_job&
         in_group = cb < (uint32_t)count;
     }
     bool active = in_group;
-    unsigned char* const lane_base =
+    unsigned char* const cm_base =
         reinterpret_cast<unsigned char*>(app) + (size_t)(in_group ? cb : 0u) * (size_t)stride * sizeof(T);
-    const uint32_t Zb = Z * (uint32_t)sizeof(T);
+    // row-pair: the APP values start at LDS address 0 (app == LDS origin)
+    const auto lane_base = [&]() {
+        if constexpr (PAIR == 2)
+            return lds_origin();
+        else
+            return cm_base;
+    }();
+    // row-pair mode keeps the APP values lane-major (lm_pitch_bytes)
+    constexpr bool LM = (PAIR == 2);
+    const uint32_t Pb = LM ? lm_pitch_bytes(BGT::NSYS + mb, (int)sizeof(T)) : (uint32_t)sizeof(T);
+    const uint32_t Zb = Z * Pb;
 
     shift_ptr sh = (shift_ptr)(const void*)&g_shift_tables.v[BG - 1][z_index((int)Z)][0];
     int it = 0;
@@ -383,7 +434,7 @@ __device__ __forceinline__ int run_iterations(T* __restrict__ app, const cb_job&
         // per row: the 316 (BG1) per-edge shifts and addresses are reloaded
         // and recomputed where used instead of being hoisted out of the
         // loops into (spilled) registers.
-        uint32_t zb = zlane * (uint32_t)sizeof(T);
+        uint32_t zb = zlane * Pb;
         asm volatile("" : "+v"(zb));
         static_for<NR>([&](auto Rc) {
             constexpr int R = decltype(Rc)::value;
@@ -399,9 +450,9 @@ __device__ __forceinline__ int run_iterations(T* __restrict__ app, const cb_job&
                     constexpr int RS = (PAIR == 2) ? PP.row_slot[R] : R;
                     constexpr int S0 = (PAIR == 2) ? PP.edge_slot[R] : BGT::row_start(R);
                     if constexpr (std::is_same<T, float>::value)
-                        process_row<BG, R, NRS, RS>(app, lane_base, shr, zb, Zr, alpha, st_n1, st_n2, st_meta);
+                        process_row<BG, R, NRS, RS, LM>(app, lane_base, shr, zb, Zr, alpha, st_n1, st_n2, st_meta, Pb);
                     else
-                        process_row<BG, R, NW, S0>(app, lane_base, shr, zb, Zr, alpha, st_h);
+                        process_row<BG, R, NW, S0, LM>(app, lane_base, shr, zb, Zr, alpha, st_h, Pb);
                 }
                 // barrier before the next row only where it shares a column
                 // with the rows since the last barrier, and at the iteration end
@@ -420,10 +471,19 @@ __device__ __forceinline__ int run_iterations(T* __restrict__ app, const cb_job&
                     uint32_t Zr = Zb;
                     asm volatile("" : "+s"(shr), "+s"(Zr));
                     if (active && (PAIR == 1 || lgrp == (uint32_t)PP.group[R]))
-                        fail |= row_syndrome<BG, R, T>(lane_base, shr, zb, Zr);
+                        fail |= row_syndrome<BG, R, T, LM>(lane_base, shr, zb, Zr, Pb);
                 }
             });
-            if (count == 1) {
+            if constexpr (PAIR == 2) {
+                // OR over the workgroup through the LDS word cb_flags[0]
+                // (the row-pair kernels hold no static LDS, so that their
+                // APP values start at LDS address 0)
+                if (tid == 0u) cb_flags[0] = 0u;
+                __syncthreads();
+                if (fail) cb_flags[0] = 1u;
+                __syncthreads();
+                if (__builtin_amdgcn_readfirstlane(cb_flags[0]) == 0u) break;
+            } else if (count == 1) {
                 if (!__syncthreads_or((int)fail)) break;
             } else {
                 // per code block: any failing check row keeps it iterating
@@ -530,16 +590,60 @@ __device__ __forceinline__ int decode_group(T* __restrict__ app, const cb_job& j
 
 // One code block in row-pair mode (run_iterations PAIR = 2; blockDim.x >=
 // 2 * roundup64(Z)): LLRs into LDS, iterations, hard (and soft) outputs.
+// The APP values are lane-major (lm_pitch_bytes; LDS of Z * pitch bytes):
+// value n = column * Z + z of the code word at z * pitch + column * sizeof(T).
+// flag: one LDS word outside them (the early-termination vote).
 template <int BG, typename T, typename TIn, int MAXR>
-__device__ __forceinline__ int decode_pair(T* __restrict__ app, const cb_job& job, const cb_buffers& b)
+__device__ __forceinline__ int decode_pair(T* __restrict__ app, const cb_job& job, const cb_buffers& b,
+                                           uint32_t* flag)
 {
-    const int nval = (bg_traits<BG>::NSYS + job.mb) * job.Z;
-    load_llrs<T, TIn>(app, static_cast<const TIn*>(b.llr), nval, job.clamp);
-    const int iters = run_iterations<BG, T, MAXR, 2>(app, job);
-    const int nbits = job.Kb * job.Z;
-    store_hard<T>(app, nbits, b.out);
+    const int Z = job.Z;
+    const int ncol = bg_traits<BG>::NSYS + job.mb;
+    const uint32_t pitch = lm_pitch_bytes(ncol, (int)sizeof(T));
+    unsigned char* const lds = reinterpret_cast<unsigned char*>(app);
+    auto at = [&](int n) -> T& {
+        const int col = n / Z;
+        return lds_at<T>(lds, (uint32_t)(n - col * Z) * pitch + (uint32_t)col * (uint32_t)sizeof(T));
+    };
+    // LLRs: thread t takes lane z = t mod Z of columns t / Z, t / Z + per, ...
+    // (per = blockDim.x / Z >= 2 column streams; coalesced over z)
+    {
+        const TIn* src = static_cast<const TIn*>(b.llr);
+        const int per = (int)blockDim.x / Z;
+        const int t = (int)threadIdx.x;
+        const T clampv = (T)job.clamp;
+        const bool do_clamp = job.clamp > 0.0f;
+        if (t < per * Z) {
+            const int c0 = t / Z;
+            const int z = t - c0 * Z;
+            for (int col = c0; col < ncol; col += per) {
+                T v = llr_in<TIn, T>::cvt(src[col * Z + z]);
+                if (do_clamp) v = num<T>::vmax(num<T>::vmin(v, clampv), -clampv);
+                lds_at<T>(lds, (uint32_t)z * pitch + (uint32_t)col * (uint32_t)sizeof(T)) = (v == (T)0) ? (T)0 : v;
+            }
+        }
+        __syncthreads();
+    }
+    const int iters = run_iterations<BG, T, MAXR, 2>(app, job, 1, 0, flag);
+    const int nbits = job.Kb * Z;
+    // hard decisions, 64 per wave per step by ballot (as store_hard)
+    {
+        const int nwords = (nbits + 31) >> 5;
+        const int tid = (int)threadIdx.x;
+        const int lane = tid & 63;
+        const int nwaves = (int)((blockDim.x + 63u) >> 6);
+        for (int c = tid >> 6; c * 64 < nbits; c += nwaves) {
+            const int n = c * 64 + lane;
+            const bool bit = (n < nbits) && (at(n < nbits ? n : 0) <= (T)0);
+            const uint64_t m = __ballot(bit);
+            if (lane == 0) {
+                b.out[2 * c] = (uint32_t)m;
+                if (2 * c + 1 < nwords) b.out[2 * c + 1] = (uint32_t)(m >> 32);
+            }
+        }
+    }
     if (b.soft)
-        for (int n = threadIdx.x; n < nbits; n += blockDim.x) b.soft[n] = (_Float16)app[n];
+        for (int n = threadIdx.x; n < nbits; n += blockDim.x) b.soft[n] = (_Float16)at(n);
     return iters;
 }
 

@@ -66,8 +66,11 @@ template <int BG, typename T, typename TIn>
 __global__ __launch_bounds__(768) void ldpc_layered_tb_pair(cuphyLDPCDecodeDesc_t desc)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    decode_pair<BG, T, TIn, 46>(reinterpret_cast<T*>(lds_raw), tb_job<TIn>(desc),
-                                tb_buffers<TIn>(desc, (int)blockIdx.x));
+    const cb_job job = tb_job<TIn>(desc);
+    // the vote word after the lane-major APP rows (Z * pitch bytes, a multiple of 4)
+    uint32_t* flag = reinterpret_cast<uint32_t*>(
+        lds_raw + (size_t)job.Z * lm_pitch_bytes(bg_traits<BG>::NSYS + job.mb, (int)sizeof(T)));
+    decode_pair<BG, T, TIn, 46>(reinterpret_cast<T*>(lds_raw), job, tb_buffers<TIn>(desc, (int)blockIdx.x), flag);
 }
 
 // Persistent heterogeneous decoder: each workgroup repeatedly takes the
@@ -141,13 +144,17 @@ __global__ __launch_bounds__(768) void ldpc_layered_persistent_pair(const cuphyH
                                                                     int n_groups, int lds_values,
                                                                     uint32_t* __restrict__ queue)
 {
+    // no static LDS: the lane-major APP values start at LDS address 0; the
+    // queue slot and the vote word are the last 16 bytes of the allocation
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    __shared__ int s_item;
+    const uint32_t lds_bytes = (uint32_t)lds_values * (uint32_t)sizeof(T);
+    int* const s_item = reinterpret_cast<int*>(lds_raw + lds_bytes - 16u);
+    uint32_t* const flag = reinterpret_cast<uint32_t*>(lds_raw + lds_bytes - 8u);
     T* app = reinterpret_cast<T*>(lds_raw);
     for (;;) {
-        if (threadIdx.x == 0) s_item = (int)atomicAdd(&queue[0], 1u);
+        if (threadIdx.x == 0) *s_item = (int)atomicAdd(&queue[0], 1u);
         __syncthreads();
-        const int g = __builtin_amdgcn_readfirstlane(s_item);
+        const int g = __builtin_amdgcn_readfirstlane(*s_item);
         __syncthreads();
         if (g >= n_groups) break;
         const int first = group_first ? (int)group_first[g] : g;
@@ -155,13 +162,14 @@ __global__ __launch_bounds__(768) void ldpc_layered_persistent_pair(const cuphyH
         for (int c = 0; c < count; ++c) {
             const cuphyHipLDPCWorkItem_t& w = items[first + c];
             const cb_job job = item_job(w);
-            const int nval = ((w.BG == 1) ? 22 : 10) * job.Z + job.mb * job.Z;
-            if (nval <= lds_values && 2 * ((job.Z + 63) & ~63) <= (int)blockDim.x) {
+            // lane-major APP values: Z rows of lm_pitch_bytes, below the last 16 bytes
+            const uint32_t need = (uint32_t)job.Z * lm_pitch_bytes(((w.BG == 1) ? 22 : 10) + job.mb, (int)sizeof(T));
+            if (need + 16u <= lds_bytes && 2 * ((job.Z + 63) & ~63) <= (int)blockDim.x) {
                 const cb_buffers b{w.llr, w.out, static_cast<_Float16*>(w.soft)};
                 if (w.BG == 1)
-                    decode_pair<1, T, TIn, MAXR>(app, job, b);
+                    decode_pair<1, T, TIn, MAXR>(app, job, b, flag);
                 else
-                    decode_pair<2, T, TIn, MAXR>(app, job, b);
+                    decode_pair<2, T, TIn, MAXR>(app, job, b, flag);
             }
             __syncthreads();
         }
@@ -256,6 +264,8 @@ static bool use_fp16_math(const cuphyLDPCDecodeConfigDesc_t& c)
     return c.llr_type == CUPHY_R_16F || c.algo == 2;
 }
 
+cuphyStatus_t check_pair_kernel(void* func);
+
 cuphyStatus_t fill_launch(const cuphyLDPCDecodeDesc_t& desc, launch_geometry& g)
 {
     const cuphyLDPCDecodeConfigDesc_t& c = desc.config;
@@ -269,9 +279,15 @@ cuphyStatus_t fill_launch(const cuphyLDPCDecodeDesc_t& desc, launch_geometry& g)
     const bool pair = P == 1 && c.num_parity_nodes > SMALL_MAXR;
     g.func = pair ? select_tb_pair_kernel(c.BG, c.llr_type, fp16) : select_kernel(c.BG, c.llr_type, fp16, c.num_parity_nodes);
     if (!g.func) return CUPHY_STATUS_UNSUPPORTED_CONFIG;
+    if (pair) {
+        const cuphyStatus_t s = check_pair_kernel(g.func);
+        if (s != CUPHY_STATUS_SUCCESS) return s;
+    }
     const int nsys = (c.BG == 1) ? 22 : 10;
     const size_t stride = (size_t)(((nsys + c.num_parity_nodes) * c.Z + 7) & ~7);
-    g.lds_bytes = (size_t)P * stride * (fp16 ? 2u : 4u);
+    // row-pair kernels keep the APP values lane-major: Z rows of lm_pitch_bytes
+    g.lds_bytes = pair ? (size_t)c.Z * lm_pitch_bytes(nsys + c.num_parity_nodes, fp16 ? 2 : 4) + 16u
+                       : (size_t)P * stride * (fp16 ? 2u : 4u);
     g.block = pair ? dim3((unsigned)(2 * ((c.Z + 63) / 64 * 64))) : dim3((unsigned)((P * c.Z + 63) / 64 * 64));
     g.grid = dim3((unsigned)((desc_codewords(desc) + P - 1) / P));
     return CUPHY_STATUS_SUCCESS;
@@ -298,6 +314,21 @@ cuphyStatus_t ensure_lds_attr(void* func, size_t bytes)
     return CUPHY_STATUS_SUCCESS;
 }
 
+// Row-pair kernels address their APP values from LDS address 0, so they
+// must hold no static LDS (which would come first); checked once per kernel.
+cuphyStatus_t check_pair_kernel(void* func)
+{
+    static std::mutex mu;
+    static std::vector<void*> ok;
+    std::lock_guard<std::mutex> g(mu);
+    for (void* f : ok)
+        if (f == func) return CUPHY_STATUS_SUCCESS;
+    hipFuncAttributes attr{};
+    if (hipFuncGetAttributes(&attr, func) != hipSuccess || attr.sharedSizeBytes != 0) return CUPHY_STATUS_INTERNAL_ERROR;
+    ok.push_back(func);
+    return CUPHY_STATUS_SUCCESS;
+}
+
 cuphyStatus_t decode_tb(const cuphyLDPCDecodeDesc_t& desc, hipStream_t strm)
 {
     launch_geometry g;
@@ -321,7 +352,7 @@ cuphyStatus_t decode_batch(const cuphyHipLDPCWorkItem_t* d_items, const uint32_t
     const bool f16 = fp16_math || llr_type == CUPHY_R_16F;
     const size_t elem = f16 ? 2 : 4;
     // a group's LDS regions are at most 384 lanes' worth of code blocks
-    if (max_values_per_group <= 0 || (size_t)max_values_per_group * elem > 160u * 1024u)
+    if (max_values_per_group <= 0 || (size_t)max_values_per_group * elem + (pair ? 8u * 384u : 0u) > 160u * 1024u)
         return CUPHY_STATUS_INVALID_ARGUMENT;
     if (n_groups == 0) return CUPHY_STATUS_SUCCESS;
     if (llr_type != CUPHY_R_32F && llr_type != CUPHY_R_16F) return CUPHY_STATUS_UNSUPPORTED_TYPE;
@@ -329,14 +360,17 @@ cuphyStatus_t decode_batch(const cuphyHipLDPCWorkItem_t* d_items, const uint32_t
                  : max_mb <= SMALL_MAXR ? select_persistent_kernel<SMALL_MAXR>(llr_type, f16)
                                         : select_persistent_kernel<46>(llr_type, f16);
     const unsigned block = (unsigned)block_threads;
-    const size_t lds = (size_t)max_values_per_group * elem;
-    cuphyStatus_t s = ensure_lds_attr(func, lds);
+    // row-pair: lane-major APP rows padded to an odd number of words, at
+    // most 7 bytes per lane beyond the values (Z <= 384)
+    const size_t lds = (size_t)max_values_per_group * elem + (pair ? 8u * 384u : 0u);
+    cuphyStatus_t s = pair ? check_pair_kernel(func) : CUPHY_STATUS_SUCCESS;
+    if (s == CUPHY_STATUS_SUCCESS) s = ensure_lds_attr(func, lds);
     if (s != CUPHY_STATUS_SUCCESS) return s;
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, func, (int)block, lds) != hipSuccess || per_cu < 1)
         per_cu = 1;
     const int grid = std::min(n_groups, std::max(1, cu_count) * per_cu);
-    int lds_values = max_values_per_group;
+    int lds_values = (int)(lds / elem);
     void* args[5] = {const_cast<cuphyHipLDPCWorkItem_t**>(&d_items), const_cast<uint32_t**>(&d_group_first),
                      &n_groups, &lds_values, &d_queue};
     if (hipLaunchKernel(func, dim3((unsigned)grid), dim3(block), args, lds, strm) != hipSuccess)
