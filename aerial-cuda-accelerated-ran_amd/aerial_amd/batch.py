@@ -164,7 +164,10 @@ class BatchDecoder:
                 i += cnt
             first[len(groups)] = i
             sb.items = torch.from_numpy(arr.view(np.uint8).copy()).cuda()
-            sb.group_first = torch.from_numpy(first.view(np.int32).copy()).cuda()
+            # no group list when every group is one code block: the library then
+            # uses its lane-major one-code-block kernel
+            if any(g[3] > 1 for g in groups) or sb.block == 768:
+                sb.group_first = torch.from_numpy(first.view(np.int32).copy()).cuda()
             sb.n_items = n_items
             sb.n_groups = len(groups)
             self.subs.append(sb)
@@ -176,7 +179,8 @@ class BatchDecoder:
 
     def _launch_one(self, sb, stream):
         st = self._lib.cuphyHipLDPCBatchDecodeGroups(
-            self.dec.dec, C.c_void_p(sb.items.data_ptr()), C.c_void_p(sb.group_first.data_ptr()), sb.n_groups,
+            self.dec.dec, C.c_void_p(sb.items.data_ptr()),
+            C.c_void_p(sb.group_first.data_ptr() if sb.group_first is not None else None), sb.n_groups,
             self.llr_type, int(self.fp16_math), sb.max_values, sb.max_mb, sb.block,
             C.c_void_p(sb.workspace.data_ptr()), _stream_handle(stream))
         return L.check("cuphyHipLDPCBatchDecodeGroups", st)

@@ -600,7 +600,7 @@ cuphyStatus_t CUPHYWINAPI cuphyHipLDPCBatchDecodeGroups(cuphyLDPCDecoder_t decod
                                                         int max_values_per_group, int max_parity_nodes,
                                                         int block_threads, void* d_workspace, hipStream_t strm)
 {
-    if (!decoder || !d_workspace || !d_group_first || max_parity_nodes < 4 || max_parity_nodes > 46)
+    if (!decoder || !d_workspace || max_parity_nodes < 4 || max_parity_nodes > 46)
         return CUPHY_STATUS_INVALID_ARGUMENT;
     return nrldpc::decode_batch(d_items, d_group_first, num_groups, llr_type, fp16_math != 0, max_values_per_group,
                                 max_parity_nodes, block_threads, static_cast<uint32_t*>(d_workspace),

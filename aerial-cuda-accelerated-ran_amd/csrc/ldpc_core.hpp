@@ -366,7 +366,9 @@ template <> __device__ __forceinline__ _Float16 job_alpha<_Float16>(const cb_job
 // wave-aligned) form two lane groups that process the rows of kPairPlan's
 // group 0 and group 1; rows of one barrier segment run at the same time,
 // and each lane holds the message state of its group's rows only.
-template <int BG, typename T, int MAXR, int PAIR = 1>
+// LM: lane-major APP values from LDS address 0 (lm_pitch_bytes; count must
+// be 1, cb_flags one LDS word for the early-termination vote).
+template <int BG, typename T, int MAXR, int PAIR = 1, bool LM = (PAIR == 2)>
 __device__ __forceinline__ int run_iterations(T* __restrict__ app, const cb_job& job, int count = 1, int stride = 0,
                                               uint32_t* cb_flags = nullptr, uint32_t* cb_iters = nullptr)
 {
@@ -413,15 +415,13 @@ __device__ __forceinline__ int run_iterations(T* __restrict__ app, const cb_job&
     bool active = in_group;
     unsigned char* const cm_base =
         reinterpret_cast<unsigned char*>(app) + (size_t)(in_group ? cb : 0u) * (size_t)stride * sizeof(T);
-    // row-pair: the APP values start at LDS address 0 (app == LDS origin)
+    // lane-major: the APP values start at LDS address 0 (app == LDS origin)
     const auto lane_base = [&]() {
-        if constexpr (PAIR == 2)
+        if constexpr (LM)
             return lds_origin();
         else
             return cm_base;
     }();
-    // row-pair mode keeps the APP values lane-major (lm_pitch_bytes)
-    constexpr bool LM = (PAIR == 2);
     const uint32_t Pb = LM ? lm_pitch_bytes(BGT::NSYS + mb, (int)sizeof(T)) : (uint32_t)sizeof(T);
     const uint32_t Zb = Z * Pb;
 
@@ -474,9 +474,9 @@ __device__ __forceinline__ int run_iterations(T* __restrict__ app, const cb_job&
                         fail |= row_syndrome<BG, R, T, LM>(lane_base, shr, zb, Zr, Pb);
                 }
             });
-            if constexpr (PAIR == 2) {
+            if constexpr (LM) {
                 // OR over the workgroup through the LDS word cb_flags[0]
-                // (the row-pair kernels hold no static LDS, so that their
+                // (the lane-major kernels hold no static LDS, so that their
                 // APP values start at LDS address 0)
                 if (tid == 0u) cb_flags[0] = 0u;
                 __syncthreads();
@@ -590,23 +590,20 @@ __device__ __forceinline__ int decode_group(T* __restrict__ app, const cb_job& j
 
 // One code block in row-pair mode (run_iterations PAIR = 2; blockDim.x >=
 // 2 * roundup64(Z)): LLRs into LDS, iterations, hard (and soft) outputs.
-// The APP values are lane-major (lm_pitch_bytes; LDS of Z * pitch bytes):
-// value n = column * Z + z of the code word at z * pitch + column * sizeof(T).
-// flag: one LDS word outside them (the early-termination vote).
-template <int BG, typename T, typename TIn, int MAXR>
-__device__ __forceinline__ int decode_pair(T* __restrict__ app, const cb_job& job, const cb_buffers& b,
-                                           uint32_t* flag)
+// One code block with lane-major APP values (lm_pitch_bytes; LDS of Z *
+// pitch bytes from LDS address 0): value n = column * Z + z of the code word
+// at z * pitch + column * sizeof(T).  PAIR = 2: row-pair mode (blockDim.x >=
+// 2 * roundup64(Z)); PAIR = 1: one lane group (blockDim.x >= Z).  flag: one
+// LDS word outside the values (the early-termination vote).
+template <int BG, typename T, typename TIn, int MAXR, int PAIR>
+__device__ __forceinline__ int decode_lm(T* __restrict__ app, const cb_job& job, const cb_buffers& b, uint32_t* flag)
 {
     const int Z = job.Z;
     const int ncol = bg_traits<BG>::NSYS + job.mb;
     const uint32_t pitch = lm_pitch_bytes(ncol, (int)sizeof(T));
     unsigned char* const lds = reinterpret_cast<unsigned char*>(app);
-    auto at = [&](int n) -> T& {
-        const int col = n / Z;
-        return lds_at<T>(lds, (uint32_t)(n - col * Z) * pitch + (uint32_t)col * (uint32_t)sizeof(T));
-    };
     // LLRs: thread t takes lane z = t mod Z of columns t / Z, t / Z + per, ...
-    // (per = blockDim.x / Z >= 2 column streams; coalesced over z)
+    // (per = blockDim.x / Z >= 1 column streams; coalesced over z)
     {
         const TIn* src = static_cast<const TIn*>(b.llr);
         const int per = (int)blockDim.x / Z;
@@ -624,26 +621,47 @@ __device__ __forceinline__ int decode_pair(T* __restrict__ app, const cb_job& jo
         }
         __syncthreads();
     }
-    const int iters = run_iterations<BG, T, MAXR, 2>(app, job, 1, 0, flag);
+    const int iters = run_iterations<BG, T, MAXR, PAIR, true>(app, job, 1, 0, flag);
     const int nbits = job.Kb * Z;
-    // hard decisions, 64 per wave per step by ballot (as store_hard)
+    // hard decisions, 64 per wave per step by ballot (as store_hard); each
+    // lane walks its (column, z) position instead of dividing per bit
     {
         const int nwords = (nbits + 31) >> 5;
         const int tid = (int)threadIdx.x;
         const int lane = tid & 63;
         const int nwaves = (int)((blockDim.x + 63u) >> 6);
+        const int step = nwaves * 64;
+        int n = (tid >> 6) * 64 + lane;
+        int col = n / Z, z = n - col * Z;
         for (int c = tid >> 6; c * 64 < nbits; c += nwaves) {
-            const int n = c * 64 + lane;
-            const bool bit = (n < nbits) && (at(n < nbits ? n : 0) <= (T)0);
+            const bool bit =
+                (n < nbits) && (lds_at<T>(lds, (uint32_t)z * pitch + (uint32_t)col * (uint32_t)sizeof(T)) <= (T)0);
             const uint64_t m = __ballot(bit);
             if (lane == 0) {
                 b.out[2 * c] = (uint32_t)m;
                 if (2 * c + 1 < nwords) b.out[2 * c + 1] = (uint32_t)(m >> 32);
             }
+            n += step;
+            z += step;
+            while (z >= Z) {
+                z -= Z;
+                ++col;
+            }
         }
     }
-    if (b.soft)
-        for (int n = threadIdx.x; n < nbits; n += blockDim.x) b.soft[n] = (_Float16)at(n);
+    if (b.soft) {
+        const int step = (int)blockDim.x;
+        int n = (int)threadIdx.x;
+        int col = n / Z, z = n - col * Z;
+        for (; n < nbits; n += step) {
+            b.soft[n] = (_Float16)lds_at<T>(lds, (uint32_t)z * pitch + (uint32_t)col * (uint32_t)sizeof(T));
+            z += step;
+            while (z >= Z) {
+                z -= Z;
+                ++col;
+            }
+        }
+    }
     return iters;
 }
 

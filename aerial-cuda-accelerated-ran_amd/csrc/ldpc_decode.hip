@@ -70,7 +70,7 @@ __global__ __launch_bounds__(768) void ldpc_layered_tb_pair(cuphyLDPCDecodeDesc_
     // the vote word after the lane-major APP rows (Z * pitch bytes, a multiple of 4)
     uint32_t* flag = reinterpret_cast<uint32_t*>(
         lds_raw + (size_t)job.Z * lm_pitch_bytes(bg_traits<BG>::NSYS + job.mb, (int)sizeof(T)));
-    decode_pair<BG, T, TIn, 46>(reinterpret_cast<T*>(lds_raw), job, tb_buffers<TIn>(desc, (int)blockIdx.x), flag);
+    decode_lm<BG, T, TIn, 46, 2>(reinterpret_cast<T*>(lds_raw), job, tb_buffers<TIn>(desc, (int)blockIdx.x), flag);
 }
 
 // Persistent heterogeneous decoder: each workgroup repeatedly takes the
@@ -138,14 +138,16 @@ __global__ __launch_bounds__(384) void ldpc_layered_persistent(const cuphyHipLDP
 // where the slot time is the slowest code block's decode time.  Same queue
 // protocol as ldpc_layered_persistent; a group's code blocks are decoded
 // one after another.
-template <typename T, typename TIn, int MAXR>
-__global__ __launch_bounds__(768) void ldpc_layered_persistent_pair(const cuphyHipLDPCWorkItem_t* __restrict__ items,
-                                                                    const uint32_t* __restrict__ group_first,
-                                                                    int n_groups, int lds_values,
-                                                                    uint32_t* __restrict__ queue)
+// Body of the lane-major persistent kernels (PAIR = 2: row-pair, 768
+// threads; PAIR = 1: one lane group, one code block per workgroup of at
+// least roundup64(Z) threads).  No static LDS: the APP values start at LDS
+// address 0; the queue slot and the vote word are the last 16 bytes of the
+// dynamic allocation.
+template <typename T, typename TIn, int MAXR, int PAIR>
+__device__ __forceinline__ void persistent_lm(const cuphyHipLDPCWorkItem_t* __restrict__ items,
+                                              const uint32_t* __restrict__ group_first, int n_groups, int lds_values,
+                                              uint32_t* __restrict__ queue)
 {
-    // no static LDS: the lane-major APP values start at LDS address 0; the
-    // queue slot and the vote word are the last 16 bytes of the allocation
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const uint32_t lds_bytes = (uint32_t)lds_values * (uint32_t)sizeof(T);
     int* const s_item = reinterpret_cast<int*>(lds_raw + lds_bytes - 16u);
@@ -164,12 +166,12 @@ __global__ __launch_bounds__(768) void ldpc_layered_persistent_pair(const cuphyH
             const cb_job job = item_job(w);
             // lane-major APP values: Z rows of lm_pitch_bytes, below the last 16 bytes
             const uint32_t need = (uint32_t)job.Z * lm_pitch_bytes(((w.BG == 1) ? 22 : 10) + job.mb, (int)sizeof(T));
-            if (need + 16u <= lds_bytes && 2 * ((job.Z + 63) & ~63) <= (int)blockDim.x) {
+            if (need + 16u <= lds_bytes && PAIR * ((job.Z + 63) & ~63) <= (int)blockDim.x) {
                 const cb_buffers b{w.llr, w.out, static_cast<_Float16*>(w.soft)};
                 if (w.BG == 1)
-                    decode_pair<1, T, TIn, MAXR>(app, job, b, flag);
+                    decode_lm<1, T, TIn, MAXR, PAIR>(app, job, b, flag);
                 else
-                    decode_pair<2, T, TIn, MAXR>(app, job, b, flag);
+                    decode_lm<2, T, TIn, MAXR, PAIR>(app, job, b, flag);
             }
             __syncthreads();
         }
@@ -184,6 +186,33 @@ __global__ __launch_bounds__(768) void ldpc_layered_persistent_pair(const cuphyH
     }
 }
 
+// Row-pair persistent kernel (768 threads): every code block of the queue
+// decoded alone by two lane groups that take the independent block rows of
+// each barrier segment at the same time (run_iterations PAIR = 2), with 12
+// waves on the CU's 4 SIMDs.  For batches with fewer code blocks than CUs,
+// where the slot time is the slowest code block's decode time.  Same queue
+// protocol as ldpc_layered_persistent; a group's code blocks are decoded
+// one after another.
+template <typename T, typename TIn, int MAXR>
+__global__ __launch_bounds__(768) void ldpc_layered_persistent_pair(const cuphyHipLDPCWorkItem_t* __restrict__ items,
+                                                                    const uint32_t* __restrict__ group_first,
+                                                                    int n_groups, int lds_values,
+                                                                    uint32_t* __restrict__ queue)
+{
+    persistent_lm<T, TIn, MAXR, 2>(items, group_first, n_groups, lds_values, queue);
+}
+
+// One code block per workgroup (one lane group), lane-major APP values:
+// for launches without code-block groups.  Same queue protocol.
+template <typename T, typename TIn, int MAXR>
+__global__ __launch_bounds__(384) void ldpc_layered_persistent_lm(const cuphyHipLDPCWorkItem_t* __restrict__ items,
+                                                                  const uint32_t* __restrict__ group_first,
+                                                                  int n_groups, int lds_values,
+                                                                  uint32_t* __restrict__ queue)
+{
+    persistent_lm<T, TIn, MAXR, 1>(items, group_first, n_groups, lds_values, queue);
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -194,6 +223,15 @@ void* select_persistent_pair_kernel(cuphyDataType_t llr_type, bool fp16_math)
     if (llr_type == CUPHY_R_32F && !fp16_math) return (void*)&ldpc_layered_persistent_pair<float, float, 46>;
     if (llr_type == CUPHY_R_32F) return (void*)&ldpc_layered_persistent_pair<_Float16, float, 46>;
     if (llr_type == CUPHY_R_16F) return (void*)&ldpc_layered_persistent_pair<_Float16, _Float16, 46>;
+    return nullptr;
+}
+
+template <int MAXR>
+void* select_persistent_lm_kernel(cuphyDataType_t llr_type, bool fp16_math)
+{
+    if (llr_type == CUPHY_R_32F && !fp16_math) return (void*)&ldpc_layered_persistent_lm<float, float, MAXR>;
+    if (llr_type == CUPHY_R_32F) return (void*)&ldpc_layered_persistent_lm<_Float16, float, MAXR>;
+    if (llr_type == CUPHY_R_16F) return (void*)&ldpc_layered_persistent_lm<_Float16, _Float16, MAXR>;
     return nullptr;
 }
 
@@ -264,7 +302,7 @@ static bool use_fp16_math(const cuphyLDPCDecodeConfigDesc_t& c)
     return c.llr_type == CUPHY_R_16F || c.algo == 2;
 }
 
-cuphyStatus_t check_pair_kernel(void* func);
+cuphyStatus_t check_lm_kernel(void* func);
 
 cuphyStatus_t fill_launch(const cuphyLDPCDecodeDesc_t& desc, launch_geometry& g)
 {
@@ -280,7 +318,7 @@ cuphyStatus_t fill_launch(const cuphyLDPCDecodeDesc_t& desc, launch_geometry& g)
     g.func = pair ? select_tb_pair_kernel(c.BG, c.llr_type, fp16) : select_kernel(c.BG, c.llr_type, fp16, c.num_parity_nodes);
     if (!g.func) return CUPHY_STATUS_UNSUPPORTED_CONFIG;
     if (pair) {
-        const cuphyStatus_t s = check_pair_kernel(g.func);
+        const cuphyStatus_t s = check_lm_kernel(g.func);
         if (s != CUPHY_STATUS_SUCCESS) return s;
     }
     const int nsys = (c.BG == 1) ? 22 : 10;
@@ -314,9 +352,9 @@ cuphyStatus_t ensure_lds_attr(void* func, size_t bytes)
     return CUPHY_STATUS_SUCCESS;
 }
 
-// Row-pair kernels address their APP values from LDS address 0, so they
+// Lane-major kernels address their APP values from LDS address 0, so they
 // must hold no static LDS (which would come first); checked once per kernel.
-cuphyStatus_t check_pair_kernel(void* func)
+cuphyStatus_t check_lm_kernel(void* func)
 {
     static std::mutex mu;
     static std::vector<void*> ok;
@@ -352,18 +390,22 @@ cuphyStatus_t decode_batch(const cuphyHipLDPCWorkItem_t* d_items, const uint32_t
     const bool f16 = fp16_math || llr_type == CUPHY_R_16F;
     const size_t elem = f16 ? 2 : 4;
     // a group's LDS regions are at most 384 lanes' worth of code blocks
-    if (max_values_per_group <= 0 || (size_t)max_values_per_group * elem + (pair ? 8u * 384u : 0u) > 160u * 1024u)
+    if (max_values_per_group <= 0 || (size_t)max_values_per_group * elem + (pair || !d_group_first ? 8u * 384u : 0u) > 160u * 1024u)
         return CUPHY_STATUS_INVALID_ARGUMENT;
     if (n_groups == 0) return CUPHY_STATUS_SUCCESS;
     if (llr_type != CUPHY_R_32F && llr_type != CUPHY_R_16F) return CUPHY_STATUS_UNSUPPORTED_TYPE;
-    void* func = pair                  ? select_persistent_pair_kernel(llr_type, f16)
+    // without code-block groups, one code block per workgroup: lane-major kernel
+    const bool lm = pair || !d_group_first;
+    void* func = pair                   ? select_persistent_pair_kernel(llr_type, f16)
+                 : lm                   ? (max_mb <= SMALL_MAXR ? select_persistent_lm_kernel<SMALL_MAXR>(llr_type, f16)
+                                                                : select_persistent_lm_kernel<46>(llr_type, f16))
                  : max_mb <= SMALL_MAXR ? select_persistent_kernel<SMALL_MAXR>(llr_type, f16)
                                         : select_persistent_kernel<46>(llr_type, f16);
     const unsigned block = (unsigned)block_threads;
-    // row-pair: lane-major APP rows padded to an odd number of words, at
-    // most 7 bytes per lane beyond the values (Z <= 384)
-    const size_t lds = (size_t)max_values_per_group * elem + (pair ? 8u * 384u : 0u);
-    cuphyStatus_t s = pair ? check_pair_kernel(func) : CUPHY_STATUS_SUCCESS;
+    // lane-major: APP rows padded to an odd number of words, at most 7 bytes
+    // per lane beyond the values (Z <= 384), and 16 bytes of queue slot and vote
+    const size_t lds = (size_t)max_values_per_group * elem + (lm ? 8u * 384u : 0u);
+    cuphyStatus_t s = lm ? check_lm_kernel(func) : CUPHY_STATUS_SUCCESS;
     if (s == CUPHY_STATUS_SUCCESS) s = ensure_lds_attr(func, lds);
     if (s != CUPHY_STATUS_SUCCESS) return s;
     int per_cu = 0;

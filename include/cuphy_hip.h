@@ -384,7 +384,9 @@ cuphyStatus_t CUPHYWINAPI cuphyHipLDPCBatchDecode(cuphyLDPCDecoder_t decoder, co
 
 /* Grouped form of cuphyHipLDPCBatchDecode: the device queue hands out groups
  * of items, group g being items [d_group_first[g], d_group_first[g + 1])
- * (d_group_first: num_groups + 1 device words).  The items of a group share
+ * (d_group_first: num_groups + 1 device words; null: one item per group,
+ * num_groups items, decoded with block_threads >= roundup64(Z) threads).
+ * The items of a group share
  * BG, Z, num_parity_nodes, Kb, max_iterations, flags, norm and clamp_value
  * and are decoded side by side by one workgroup of block_threads threads (a
  * multiple of 64, <= 384, >= count * Z), lanes [c*Z, (c+1)*Z) on item c, each
