@@ -217,3 +217,36 @@ def test_batch_packed_bit_exact(dec, table, et):
         got = tb.soft.cpu().numpy()
         assert np.array_equal(got.view(np.uint16), sft.view(np.uint16)), (tb.bg, tb.Z, tb.mb)
     assert all(int(w.abs().sum().item()) == 0 for w in b.workspaces)
+
+
+@pytest.mark.parametrize("et", [False, True])
+def test_batch_lane_major_equals_column_major(dec, et):
+    """One code block per group: a launch without a group list (the
+    lane-major one-code-block kernel) and the same launch with a one-item
+    group list (the column-major kernel) give identical hard and soft
+    outputs, both equal to the oracle."""
+    from aerial_amd import _lib as L
+    from aerial_amd.batch import BatchDecoder
+    print("[case] lane-major vs column-major one-CB launches et=%s fp16 seed=310" % et, flush=True)
+    n_iter = 12
+    outs = []
+    for listed in (False, True):
+        tbs, refs = make_packed(310, et, n_iter)
+        b = BatchDecoder(dec, split_min=1, pack=False, pair=False)
+        b.build(tbs, llr_type=L.CUPHY_R_16F, fp16_math=True, max_iterations=n_iter,
+                flags=L.CUPHY_LDPC_DECODE_EARLY_TERM if et else 0, soft=[tb.soft for tb in tbs])
+        for sb in b.subs:
+            assert sb.block <= 384 and sb.n_items == sb.n_groups
+            if listed:
+                sb.group_first = torch.arange(sb.n_groups + 1, dtype=torch.int32, device="cuda")
+            else:
+                assert sb.group_first is None
+        b.launch()
+        torch.cuda.synchronize()
+        for tb, (h, sft, _) in zip(tbs, refs):
+            assert np.array_equal(tb.out.cpu().numpy().view(np.uint32), h), (listed, tb.bg, tb.Z, tb.mb)
+            assert np.array_equal(tb.soft.cpu().numpy().view(np.uint16), sft.view(np.uint16)), (listed, tb.bg, tb.Z)
+        assert all(int(w.abs().sum().item()) == 0 for w in b.workspaces)
+        outs.append([(tb.out.cpu().numpy(), tb.soft.cpu().numpy()) for tb in tbs])
+    for (h0, s0), (h1, s1) in zip(*outs):
+        assert np.array_equal(h0, h1) and np.array_equal(s0.view(np.uint16), s1.view(np.uint16))
