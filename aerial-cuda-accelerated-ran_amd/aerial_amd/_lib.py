@@ -39,6 +39,9 @@ CUPHY_MAX_N_CSI1_PRMS = 4
 CUPHY_MAX_N_CSI2_REPORTS_PER_UE = 16
 MAX_WORDS_PER_CODE_BLOCK = 264
 SLOT_CB_ERROR = 0xFFFFFFFF  # CRC value of a CB/TB the fused slot kernel rejected
+# reasons a batch launch did not decode a work item (cuphyHipLDPCWorkItem_t::status)
+ITEM_BAD_BG, ITEM_BAD_Z, ITEM_BAD_MB, ITEM_BAD_KB = 0x01, 0x02, 0x04, 0x08
+ITEM_BAD_ITER, ITEM_NULL_BUFFER, ITEM_NO_FIT, ITEM_GROUP = 0x10, 0x20, 0x40, 0x80
 CUPHY_HIP_HARQ_LAST_TX = 0x100
 CUPHY_HIP_SLOT_TB_NOTIFY = 0x200
 CUPHY_HIP_SLOT_TB_DESCRAMBLE = 0x400
@@ -116,7 +119,7 @@ class LDPCWorkItem(C.Structure):
     _fields_ = [("llr", C.c_void_p), ("out", C.c_void_p), ("soft", C.c_void_p), ("BG", C.c_int16),
                 ("Z", C.c_int16), ("num_parity_nodes", C.c_int16), ("Kb", C.c_int16),
                 ("max_iterations", C.c_int16), ("flags", C.c_uint16), ("norm", Norm),
-                ("clamp_value", C.c_float), ("reserved", C.c_uint32)]
+                ("clamp_value", C.c_float), ("status", C.c_uint32)]
 
 
 class PuschSlotTb(C.Structure):
@@ -154,6 +157,11 @@ class HarqServerStats(C.Structure):
 
 class HarqTraceEntry(C.Structure):
     _fields_ = [("req", HarqRequest), ("arrival_us", C.c_double), ("next", C.c_int32), ("reserved", C.c_int32)]
+
+
+class LbrmPrm(C.Structure):
+    _fields_ = [("i_lbrm", C.c_uint8), ("maxLayers", C.c_uint8), ("maxQm", C.c_uint8), ("reserved", C.c_uint8),
+                ("n_PRB_LBRM", C.c_uint16), ("reserved2", C.c_uint16)]
 
 
 class TensorPrm(C.Structure):
@@ -214,6 +222,11 @@ SIGNATURES = {
     "cuphyHipHarqServerRelease": ([_VP, C.c_uint64], _I),
     "cuphyHipHarqServerGetStats": ([_VP, _P(HarqServerStats)], _I),
     "cuphyHipHarqServeTrace": ([_VP, _VP, _I, C.c_double, _VP, _P(_I)], _I),
+    "cuphyHipLDPCWorkItemCheck": ([_P(LDPCWorkItem), _I, _P(C.c_uint32)], _I),
+    "cuphyHipTransportBlockSize": ([_I, _I, _I, C.c_float, C.c_uint32, _I, _P(C.c_uint32), _P(C.c_uint32)], _I),
+    "cuphyHipSetPerTbParams": ([_P(PerTbParams), C.c_uint32, C.c_float, C.c_uint8, C.c_uint32, C.c_uint32,
+                                C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint8, C.c_uint8, _P(C.c_uint32), C.c_uint8,
+                                _P(LbrmPrm), _P(C.c_uint16), _P(C.c_uint16)], _I),
 }
 
 _lib = None

@@ -33,7 +33,7 @@ SMALL_MAXR = 8  # mb class boundary of the register-lean kernel variant
 # numpy view of cuphyHipLDPCWorkItem_t (48 bytes)
 ITEM_DTYPE = np.dtype([("llr", "<u8"), ("out", "<u8"), ("soft", "<u8"), ("BG", "<i2"), ("Z", "<i2"),
                        ("num_parity_nodes", "<i2"), ("Kb", "<i2"), ("max_iterations", "<i2"), ("flags", "<u2"),
-                       ("norm", "<u4"), ("clamp_value", "<f4"), ("reserved", "<u4")])
+                       ("norm", "<u4"), ("clamp_value", "<f4"), ("status", "<u4")])
 assert ITEM_DTYPE.itemsize == C.sizeof(L.LDPCWorkItem)
 
 

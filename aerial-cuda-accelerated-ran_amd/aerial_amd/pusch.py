@@ -59,6 +59,38 @@ def make_tb_params(bg, Zc, C, Kprime, Qm, Nl, G, rv, ndi=1, n_bbu_layers=None, l
     return p, mb
 
 
+def get_tb_size(num_symbols, num_prbs, num_layers, code_rate, mod_order, num_dmrs_cdm_grps_no_data1_symbols=0):
+    """TS 38.214 5.1.3.2 transport block size and code-block count of an
+    allocation (cuphyHipTransportBlockSize; pyAerial get_tb_size,
+    util.py:171-333, with the code rate as a fraction)."""
+    tbs, ncb = C.c_uint32(), C.c_uint32()
+    L.check("cuphyHipTransportBlockSize",
+            L.lib().cuphyHipTransportBlockSize(num_symbols, num_prbs, num_layers, code_rate, mod_order,
+                                               num_dmrs_cdm_grps_no_data1_symbols, C.byref(tbs), C.byref(ncb)))
+    return tbs.value, ncb.value
+
+
+def set_per_tb_params(tb_size, code_rate, qam_mod, rate_match_len, rv, ndi=1, cinit=0, user_group=0, num_layers=1,
+                      num_ue_grp_layers=None, layer_map=None, n_dmrs_cdm_grps_no_data=2, lbrm=None):
+    """PerTbParams of a PUSCH TB from its size (bits, without CRC) and code
+    rate, as pyAerial's setPerTbParams (pycuphy_ldpc_util.cpp:75-171) with
+    the PUSCH pipeline's limited-buffer rate matching (lbrm = (n_PRB_LBRM,
+    maxLayers, maxQm) or None).  Returns (params, parity nodes, info nodes Kb)."""
+    p = L.PerTbParams()
+    nl = num_layers
+    ng = nl if num_ue_grp_layers is None else num_ue_grp_layers
+    lm = (C.c_uint32 * L.MAX_N_LAYERS_PUSCH)(*(list(range(nl)) if layer_map is None else list(layer_map)))
+    lb = None
+    if lbrm is not None:
+        lb = L.LbrmPrm(i_lbrm=1, maxLayers=lbrm[1], maxQm=lbrm[2], n_PRB_LBRM=lbrm[0])
+    mb, kb = C.c_uint16(), C.c_uint16()
+    L.check("cuphyHipSetPerTbParams",
+            L.lib().cuphyHipSetPerTbParams(C.byref(p), tb_size, code_rate, qam_mod, ndi, rv, rate_match_len, cinit,
+                                           user_group, nl, ng, lm, n_dmrs_cdm_grps_no_data,
+                                           C.byref(lb) if lb is not None else None, C.byref(mb), C.byref(kb)))
+    return p, mb.value, kb.value
+
+
 def params_to_device(params):
     arr = (L.PerTbParams * len(params))(*params)
     host = np.frombuffer(bytes(arr), dtype=np.uint8).copy()

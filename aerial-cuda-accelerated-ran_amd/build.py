@@ -27,6 +27,7 @@ SOURCES = [
     "pusch_fused.hip",
     "cuphy_api.cpp",
     "harq_server.cpp",
+    "tb_params.cpp",
 ]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function",

@@ -51,7 +51,7 @@ static uint32_t rm_max_E(const PerTbParams& p)
     const uint64_t total = p.uciOnPuschFlag ? p.G : p.encodedSize;
     if (nq == 0 || C == 0) return 0;
     const uint64_t q = (total / nq) / C;
-    return (uint32_t)(nq * q + nq * ((q * nq * C < total) ? 1u : 0u));
+    return (uint32_t)(nq * q + nq * ((q * nq * C < (uint64_t)p.encodedSize) ? 1u : 0u));
 }
 struct crc_object : cuphyPuschRxCrcDecode {
     int reverse_bytes = 1;
@@ -582,7 +582,15 @@ cuphyStatus_t CUPHYWINAPI cuphyHipLDPCBatchDecodeGetWorkspaceSize(size_t* sizeIn
     return CUPHY_STATUS_SUCCESS;
 }
 
-cuphyStatus_t CUPHYWINAPI cuphyHipLDPCBatchDecode(cuphyLDPCDecoder_t decoder, const cuphyHipLDPCWorkItem_t* d_items,
+cuphyStatus_t CUPHYWINAPI cuphyHipLDPCWorkItemCheck(const cuphyHipLDPCWorkItem_t* item, int max_parity_nodes,
+                                                    uint32_t* reasons)
+{
+    if (!item || !reasons) return CUPHY_STATUS_INVALID_ARGUMENT;
+    *reasons = nrldpc::work_item_check(*item, max_parity_nodes);
+    return CUPHY_STATUS_SUCCESS;
+}
+
+cuphyStatus_t CUPHYWINAPI cuphyHipLDPCBatchDecode(cuphyLDPCDecoder_t decoder, cuphyHipLDPCWorkItem_t* d_items,
                                                   int num_items, cuphyDataType_t llr_type, int fp16_math,
                                                   int max_values_per_cb, int max_parity_nodes, void* d_workspace,
                                                   hipStream_t strm)
@@ -594,7 +602,7 @@ cuphyStatus_t CUPHYWINAPI cuphyHipLDPCBatchDecode(cuphyLDPCDecoder_t decoder, co
 }
 
 cuphyStatus_t CUPHYWINAPI cuphyHipLDPCBatchDecodeGroups(cuphyLDPCDecoder_t decoder,
-                                                        const cuphyHipLDPCWorkItem_t* d_items,
+                                                        cuphyHipLDPCWorkItem_t* d_items,
                                                         const uint32_t* d_group_first, int num_groups,
                                                         cuphyDataType_t llr_type, int fp16_math,
                                                         int max_values_per_group, int max_parity_nodes,

@@ -46,7 +46,9 @@ __device__ __forceinline__ rm_geometry rm_geometry_of(const PerTbParams& p, int 
     const int q = q1 / C;
     const int rr = C - (q1 - q * C) - 1;
     const int El = nq * q;
-    const int Eh = El + nq * ((long long)q * nq * C < (long long)total ? 1 : 0);
+    // the larger size compares with encodedSize as rate_matching.cu:345; it is used only for r > rr,
+    // i.e. when q1 is not a multiple of C, where q*nq*C < total <= encodedSize holds either way
+    const int Eh = El + nq * ((long long)q * nq * C < (long long)p.encodedSize ? 1 : 0);
     if (cb_in_tb <= rr) {
         g.E = El;
         g.start = cb_in_tb * El;

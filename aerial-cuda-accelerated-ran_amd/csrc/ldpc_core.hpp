@@ -716,6 +716,37 @@ __device__ __forceinline__ cb_buffers tb_buffers(const cuphyLDPCDecodeDesc_t& de
     return b;
 }
 
+// Reasons (CUPHY_HIP_ITEM_*) a batch launch whose kernel holds max_mb block
+// rows does not decode a work item; zi is Z's index among the lifting sizes
+// (-1: not one).  The same predicate runs on the host
+// (cuphyHipLDPCWorkItemCheck) and in the persistent kernels.
+__host__ __device__ inline uint32_t work_item_reasons(int BG, int zi, int mb, int Kb, int max_iter, bool has_llr,
+                                                      bool has_out, int max_mb)
+{
+    uint32_t r = 0;
+    if (BG != 1 && BG != 2) r |= CUPHY_HIP_ITEM_BAD_BG;
+    if (zi < 0) r |= CUPHY_HIP_ITEM_BAD_Z;
+    if (mb < 4 || mb > (BG == 2 ? 42 : 46) || mb > max_mb) r |= CUPHY_HIP_ITEM_BAD_MB;
+    if (BG == 1 ? Kb != 22 : !(Kb == 6 || Kb == 8 || Kb == 9 || Kb == 10)) r |= CUPHY_HIP_ITEM_BAD_KB;
+    if (max_iter < 0) r |= CUPHY_HIP_ITEM_BAD_ITER;
+    if (!has_llr || !has_out) r |= CUPHY_HIP_ITEM_NULL_BUFFER;
+    return r;
+}
+
+__device__ __forceinline__ uint32_t work_item_reasons(const cuphyHipLDPCWorkItem_t& w, int max_mb)
+{
+    return work_item_reasons(w.BG, z_index(w.Z), w.num_parity_nodes, w.Kb, w.max_iterations, w.llr != nullptr,
+                             w.out != nullptr, max_mb);
+}
+
+// Work items a and b share the configuration a group decodes with.
+__device__ __forceinline__ bool same_config(const cuphyHipLDPCWorkItem_t& a, const cuphyHipLDPCWorkItem_t& b)
+{
+    return a.BG == b.BG && a.Z == b.Z && a.num_parity_nodes == b.num_parity_nodes && a.Kb == b.Kb &&
+           a.max_iterations == b.max_iterations && a.flags == b.flags && a.norm.f16x2 == b.norm.f16x2 &&
+           __builtin_bit_cast(uint32_t, a.clamp_value) == __builtin_bit_cast(uint32_t, b.clamp_value);
+}
+
 // Shared configuration of a batch work item (the first of its group).
 __device__ __forceinline__ cb_job item_job(const cuphyHipLDPCWorkItem_t& w)
 {

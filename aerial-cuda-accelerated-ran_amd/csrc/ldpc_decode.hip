@@ -78,15 +78,18 @@ __global__ __launch_bounds__(768) void ldpc_layered_tb_pair(cuphyLDPCDecodeDesc_
 // it, so code blocks of any (BG, Z, mb) share one launch.  Group g is items
 // [group_first[g], group_first[g + 1]) (one item per group when group_first
 // is null): code blocks of one configuration decoded side by side, lanes
-// [c*Z, (c+1)*Z) on item c.  A group that does not fit the workgroup (count
-// * Z > blockDim.x) or the LDS (count regions > lds_values) is cut to what
-// fits; the host never builds one.  The queue state (2 words: next group,
-// finished workgroups) is returned to zero by the last workgroup to finish,
-// so the launch can be replayed from a graph.
+// [c*Z, (c+1)*Z) on item c.  Items are checked before use: an invalid first
+// item rejects its group, and the group is cut before any item that is
+// invalid, has another configuration, or does not fit the workgroup (count *
+// Z > blockDim.x) or the LDS (count regions > lds_values); every item's
+// status word says whether it was decoded (the host never builds such
+// groups).  The queue state (2 words: next group, finished workgroups) is
+// returned to zero by the last workgroup to finish, so the launch can be
+// replayed from a graph.
 constexpr int PERSISTENT_MAX_GROUP = 192;  // 384 lanes / Z = 2
 
 template <typename T, typename TIn, int MAXR>
-__global__ __launch_bounds__(384) void ldpc_layered_persistent(const cuphyHipLDPCWorkItem_t* __restrict__ items,
+__global__ __launch_bounds__(384) void ldpc_layered_persistent(cuphyHipLDPCWorkItem_t* items,
                                                                const uint32_t* __restrict__ group_first,
                                                                int n_groups, int lds_values,
                                                                uint32_t* __restrict__ queue)
@@ -104,12 +107,30 @@ __global__ __launch_bounds__(384) void ldpc_layered_persistent(const cuphyHipLDP
         __syncthreads();
         if (g >= n_groups) break;
         const int first = group_first ? (int)group_first[g] : g;
-        int count = group_first ? (int)group_first[g + 1] - first : 1;
-        const cuphyHipLDPCWorkItem_t* grp = items + first;
+        const int n_in_group = group_first ? (int)group_first[g + 1] - first : 1;
+        cuphyHipLDPCWorkItem_t* grp = items + first;
         const cb_job job = item_job(grp[0]);
-        const int nval = ((grp[0].BG == 1) ? 22 : 10) * job.Z + job.mb * job.Z;
-        count = min(count, min((int)blockDim.x / max(job.Z, 1), lds_values / group_stride(nval)));
-        count = min(count, PERSISTENT_MAX_GROUP);
+        const uint32_t bad0 = work_item_reasons(grp[0], MAXR);
+        int cap = 0, count = 0;  // items that fit; items decoded (a prefix of the group)
+        if (bad0 == 0) {
+            const int nval = ((grp[0].BG == 1) ? 22 : 10) * job.Z + job.mb * job.Z;
+            cap = min(n_in_group, min((int)blockDim.x / job.Z, lds_values / group_stride(nval)));
+            cap = min(cap, PERSISTENT_MAX_GROUP);
+            count = cap;
+            for (int c = 1; c < cap; ++c)
+                if (!same_config(grp[c], grp[0]) || grp[c].llr == nullptr || grp[c].out == nullptr) {
+                    count = c;
+                    break;
+                }
+        }
+        for (int c = threadIdx.x; c < n_in_group; c += blockDim.x) {
+            uint32_t st = 0;
+            if (c >= count) {
+                st = (c == 0) ? bad0 : work_item_reasons(grp[c], MAXR) | (same_config(grp[c], grp[0]) ? 0u : CUPHY_HIP_ITEM_GROUP);
+                if (st == 0) st = (bad0 != 0 || c < cap) ? CUPHY_HIP_ITEM_GROUP : CUPHY_HIP_ITEM_NO_FIT;
+            }
+            grp[c].status = st;
+        }
         if (count >= 1) {
             auto buf = [&](int c) {
                 return cb_buffers{grp[c].llr, grp[c].out, static_cast<_Float16*>(grp[c].soft)};
@@ -144,9 +165,8 @@ __global__ __launch_bounds__(384) void ldpc_layered_persistent(const cuphyHipLDP
 // address 0; the queue slot and the vote word are the last 16 bytes of the
 // dynamic allocation.
 template <typename T, typename TIn, int MAXR, int PAIR>
-__device__ __forceinline__ void persistent_lm(const cuphyHipLDPCWorkItem_t* __restrict__ items,
-                                              const uint32_t* __restrict__ group_first, int n_groups, int lds_values,
-                                              uint32_t* __restrict__ queue)
+__device__ __forceinline__ void persistent_lm(cuphyHipLDPCWorkItem_t* items, const uint32_t* __restrict__ group_first,
+                                              int n_groups, int lds_values, uint32_t* __restrict__ queue)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const uint32_t lds_bytes = (uint32_t)lds_values * (uint32_t)sizeof(T);
@@ -162,11 +182,16 @@ __device__ __forceinline__ void persistent_lm(const cuphyHipLDPCWorkItem_t* __re
         const int first = group_first ? (int)group_first[g] : g;
         const int count = group_first ? (int)group_first[g + 1] - first : 1;
         for (int c = 0; c < count; ++c) {
-            const cuphyHipLDPCWorkItem_t& w = items[first + c];
+            cuphyHipLDPCWorkItem_t& w = items[first + c];
             const cb_job job = item_job(w);
-            // lane-major APP values: Z rows of lm_pitch_bytes, below the last 16 bytes
-            const uint32_t need = (uint32_t)job.Z * lm_pitch_bytes(((w.BG == 1) ? 22 : 10) + job.mb, (int)sizeof(T));
-            if (need + 16u <= lds_bytes && PAIR * ((job.Z + 63) & ~63) <= (int)blockDim.x) {
+            uint32_t st = work_item_reasons(w, MAXR);
+            if (st == 0) {
+                // lane-major APP values: Z rows of lm_pitch_bytes, below the last 16 bytes
+                const uint32_t need = (uint32_t)job.Z * lm_pitch_bytes(((w.BG == 1) ? 22 : 10) + job.mb, (int)sizeof(T));
+                if (need + 16u > lds_bytes || PAIR * ((job.Z + 63) & ~63) > (int)blockDim.x) st = CUPHY_HIP_ITEM_NO_FIT;
+            }
+            if (threadIdx.x == 0) w.status = st;
+            if (st == 0) {
                 const cb_buffers b{w.llr, w.out, static_cast<_Float16*>(w.soft)};
                 if (w.BG == 1)
                     decode_lm<1, T, TIn, MAXR, PAIR>(app, job, b, flag);
@@ -194,7 +219,7 @@ __device__ __forceinline__ void persistent_lm(const cuphyHipLDPCWorkItem_t* __re
 // protocol as ldpc_layered_persistent; a group's code blocks are decoded
 // one after another.
 template <typename T, typename TIn, int MAXR>
-__global__ __launch_bounds__(768) void ldpc_layered_persistent_pair(const cuphyHipLDPCWorkItem_t* __restrict__ items,
+__global__ __launch_bounds__(768) void ldpc_layered_persistent_pair(cuphyHipLDPCWorkItem_t* items,
                                                                     const uint32_t* __restrict__ group_first,
                                                                     int n_groups, int lds_values,
                                                                     uint32_t* __restrict__ queue)
@@ -205,7 +230,7 @@ __global__ __launch_bounds__(768) void ldpc_layered_persistent_pair(const cuphyH
 // One code block per workgroup (one lane group), lane-major APP values:
 // for launches without code-block groups.  Same queue protocol.
 template <typename T, typename TIn, int MAXR>
-__global__ __launch_bounds__(384) void ldpc_layered_persistent_lm(const cuphyHipLDPCWorkItem_t* __restrict__ items,
+__global__ __launch_bounds__(384) void ldpc_layered_persistent_lm(cuphyHipLDPCWorkItem_t* items,
                                                                   const uint32_t* __restrict__ group_first,
                                                                   int n_groups, int lds_values,
                                                                   uint32_t* __restrict__ queue)
@@ -283,6 +308,13 @@ void* select_persistent_kernel(cuphyDataType_t llr_type, bool fp16_math)
 }  // namespace
 
 bool is_lifting_size(int Z) { return lifting_set_index(Z) >= 0; }
+
+uint32_t work_item_check(const cuphyHipLDPCWorkItem_t& w, int max_mb)
+{
+    // the kernel variant a launch runs holds 8 block rows when max_mb <= 8, else 46
+    return work_item_reasons(w.BG, lifting_size_index(w.Z), w.num_parity_nodes, w.Kb, w.max_iterations,
+                             w.llr != nullptr, w.out != nullptr, max_mb <= SMALL_MAXR ? SMALL_MAXR : 46);
+}
 
 bool config_supported(const cuphyLDPCDecodeConfigDesc_t& c)
 {
@@ -380,7 +412,7 @@ cuphyStatus_t decode_tb(const cuphyLDPCDecodeDesc_t& desc, hipStream_t strm)
     return CUPHY_STATUS_SUCCESS;
 }
 
-cuphyStatus_t decode_batch(const cuphyHipLDPCWorkItem_t* d_items, const uint32_t* d_group_first, int n_groups,
+cuphyStatus_t decode_batch(cuphyHipLDPCWorkItem_t* d_items, const uint32_t* d_group_first, int n_groups,
                            cuphyDataType_t llr_type, bool fp16_math, int max_values_per_group, int max_mb,
                            int block_threads, uint32_t* d_queue, int cu_count, hipStream_t strm)
 {
@@ -413,7 +445,7 @@ cuphyStatus_t decode_batch(const cuphyHipLDPCWorkItem_t* d_items, const uint32_t
         per_cu = 1;
     const int grid = std::min(n_groups, std::max(1, cu_count) * per_cu);
     int lds_values = (int)(lds / elem);
-    void* args[5] = {const_cast<cuphyHipLDPCWorkItem_t**>(&d_items), const_cast<uint32_t**>(&d_group_first),
+    void* args[5] = {&d_items, const_cast<uint32_t**>(&d_group_first),
                      &n_groups, &lds_values, &d_queue};
     if (hipLaunchKernel(func, dim3((unsigned)grid), dim3(block), args, lds, strm) != hipSuccess)
         return CUPHY_STATUS_INTERNAL_ERROR;

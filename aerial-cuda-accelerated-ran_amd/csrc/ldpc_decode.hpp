@@ -32,13 +32,17 @@ cuphyStatus_t decode_tb(const cuphyLDPCDecodeDesc_t& desc, hipStream_t strm);
 // Populate a kernel-node launch configuration (graph capture / driver launch).
 cuphyStatus_t get_launch_config(cuphyLDPCDecodeLaunchConfig_t& lc);
 
+// Reasons (CUPHY_HIP_ITEM_*) a launch holding max_mb block rows does not
+// decode the item, except those that depend on the launch's LDS and lanes.
+uint32_t work_item_check(const cuphyHipLDPCWorkItem_t& w, int max_mb);
+
 // Enqueue one persistent launch over a heterogeneous batch of work-item
 // groups (d_group_first: n_groups + 1 item offsets, or null for one item per
 // group).  max_values_per_group: LDS values of the largest group; max_mb:
 // the largest num_parity_nodes of the batch (<= 8 selects the
 // high-code-rate variant with the smaller register footprint);
 // block_threads: workgroup size (a multiple of 64, <= 384).
-cuphyStatus_t decode_batch(const cuphyHipLDPCWorkItem_t* d_items, const uint32_t* d_group_first, int n_groups,
+cuphyStatus_t decode_batch(cuphyHipLDPCWorkItem_t* d_items, const uint32_t* d_group_first, int n_groups,
                            cuphyDataType_t llr_type, bool fp16_math, int max_values_per_group, int max_mb,
                            int block_threads, uint32_t* d_queue, int cu_count, hipStream_t strm);
 

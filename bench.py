@@ -229,6 +229,23 @@ def cpu_baseline(tbs, budget_s):
                       (passes, sum(tb.ncb for tb in tbs), bits_per_pass)}
 
 
+def oracle_slot_check(tbs):
+    """Number of the slot's code blocks whose batched-form output words equal
+    the oracle's binary16 decode (10 iterations, alpha 0.8125, clamp 32) of
+    the same binary16 LLRs."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    ncores = min(16, os.cpu_count() or 1)
+    same = 0
+    for tb in tbs:
+        cfg = O.dec_cfg(tb.bg, tb.Z, tb.mb, n_iter=ITERS, alpha=0.8125, clamp=32.0, mode=O.FP16)
+        ref = O.decode_cbs(cfg, tb.llr.float().cpu().numpy()[:, :tb.num_llrs], nthreads=ncores)
+        got = tb.out.cpu().numpy().view(np.uint32)
+        same += int((got == ref).all(axis=1).sum())
+    return same
+
+
 def lds_fraction(edges, ms):
     """Algorithmic LDS bandwidth of a binary16 decode (APP read + write per
     edge update) against the nominal LDS peak."""
@@ -320,7 +337,7 @@ def main():
     kernel_ms = sum(e0.elapsed_time(e1) for step in ev for e0, e1 in step) / a.steps  # per step
     plain_gbps = info_bits / (plain_ms * 1e-3) / 1e9
 
-    # correctness spot check of the decoded bits against the transmitted ones
+    # decoded bits against the transmitted ones (channel errors left after 10 iterations)
     bit_errs, blk_errs = 0, 0
     for tb in tbs:
         bits = tb.out.view(torch.int32)
@@ -329,6 +346,7 @@ def main():
         diff = (got != tb.info)
         bit_errs += int(diff.sum().item())
         blk_errs += int(diff.any(dim=1).sum().item())
+    plain_words = [tb.out.clone() for tb in tbs]
 
     # ---- batched form: one persistent launch per slot, graph-replayed -------
     from aerial_amd.batch import BatchDecoder
@@ -364,11 +382,10 @@ def main():
     single_ms = sum(e0.elapsed_time(e1) for e0, e1 in oev) / a.steps
     del one
     batched_gbps = info_bits / (batched_ms * 1e-3) / 1e9
-    bbit_errs = 0
-    for tb in tbs:
-        bits = tb.out.view(torch.int32)
-        got = ((bits.unsqueeze(-1) >> torch.arange(32, device="cuda", dtype=torch.int32)) & 1).reshape(tb.ncb, -1)
-        bbit_errs += int((got[:, :tb.info_bits_per_cb].to(torch.uint8) != tb.info).sum().item())
+    # identical hard decisions: every output word of the batched form equals the plain form's,
+    # and both equal the oracle's binary16 decode of the same LLRs (outside any timed region)
+    batched_equals_plain = all(torch.equal(tb.out, w) for tb, w in zip(tbs, plain_words))
+    oracle_cbs = oracle_slot_check(tbs) if not a.no_cpu else None
 
     # ---- roofline of the decoder kernel (VALU-bound) ----------------------
     # dominant kernel = the batched persistent decoder: one launch per step
@@ -390,7 +407,7 @@ def main():
         config1 = config1_section(dec, stream)
     config3 = None
     if "config3" in sections:
-        config3 = config3_section(dec, stream, a.steps)
+        config3 = config3_section(dec, stream, a.steps, with_oracle=not a.no_cpu)
     throughput = None
     if "throughput" in sections:
         throughput = throughput_section(dec, stream, a.steps)
@@ -437,7 +454,7 @@ def config1_section(dec, stream):
     return {"us_per_tb": round(c1_us, 2), "gbps": round(c1_gbps, 4)}
 
 
-def config3_section(dec, stream, steps):
+def config3_section(dec, stream, steps, with_oracle=True):
     """20-cell full-load slot (2800 CBs, BG1 Z=384, R=0.93): both forms."""
     import torch
     from aerial_amd import _lib as L
@@ -465,14 +482,32 @@ def config3_section(dec, stream, steps):
         return e0.elapsed_time(e1) / n
 
     c3_plain_ms = timed(lambda: [dec.launch(d, stream) for d in c3_descs], max(2, steps // 4))
+    plain_words = [tb.out.clone() for tb in c3]
+    for tb in c3:
+        tb.out.fill_(-1)
     c3_batch_ms = timed(c3b.replay, max(2, steps // 2))
+    # identical hard decisions in both forms; the first two code blocks of every cell
+    # against the oracle's binary16 decode
+    same = all(torch.equal(tb.out, w) for tb, w in zip(c3, plain_words))
+    oracle_same = None
+    if with_oracle:
+        import numpy as np
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        ocfg = O.dec_cfg(1, 384, c3[0].mb, n_iter=ITERS, alpha=0.8125, clamp=32.0, mode=O.FP16)
+        oracle_same = 0
+        for tb in c3:
+            ref = O.decode_cbs(ocfg, tb.llr[:2].float().cpu().numpy()[:, :tb.num_llrs], nthreads=2)
+            oracle_same += int((tb.out[:2].cpu().numpy().view(np.uint32) == ref).all(axis=1).sum())
     c3_edges = sum(tb.edge_updates(ITERS) for tb in c3)
     config3 = {"code_blocks": sum(tb.ncb for tb in c3), "info_bits": c3_bits,
                "plain_ms": round(c3_plain_ms, 4), "plain_gbps": round(c3_bits / (c3_plain_ms * 1e-3) / 1e9, 3),
                "batched_ms": round(c3_batch_ms, 4), "batched_gbps": round(c3_bits / (c3_batch_ms * 1e-3) / 1e9, 3),
                "batched_valu_frac": round(c3_edges * OPS_PER_EDGE / (c3_batch_ms * 1e-3) / 1e12 / VALU_PEAK_TOPS, 4),
                "batched_lds_frac": lds_fraction(c3_edges, c3_batch_ms)["frac"],
-               "slot_budget_ms": 0.5}
+               "batched_over_plain": round(c3_plain_ms / c3_batch_ms, 3),
+               "slot_budget_ms": 0.5, "batched_equals_plain": same,
+               "oracle_equal_code_blocks": oracle_same, "oracle_checked_code_blocks": 2 * len(c3) if with_oracle else 0}
     return config3
 
 
@@ -554,8 +589,9 @@ def line_of(a, v):
                   "batched_single_launch_ms": round(v["single_ms"], 4)},
         "roofline": v["roofline"],
         "cpu_baseline": v["cpu"],
-        "check": {"plain_bit_errors": v["bit_errs"], "plain_block_errors": v["blk_errs"],
-                  "batched_bit_errors": v["bbit_errs"], "code_blocks": ncb},
+        "check": {"batched_equals_plain": v["batched_equals_plain"],
+                  "oracle_equal_code_blocks": v["oracle_cbs"], "code_blocks": ncb,
+                  "bit_errors_vs_transmitted": v["bit_errs"], "block_errors_vs_transmitted": v["blk_errs"]},
         "config1_plain_fp32": v["config1"],
         "config3_full_load_20_cells": v["config3"],
         "fused_slot_64_cells_int8": v["fused"],

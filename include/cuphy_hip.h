@@ -360,8 +360,29 @@ typedef struct {
     uint16_t flags;                  /* CUPHY_LDPC_DECODE_EARLY_TERM */
     cuphyLDPCNormalization_t norm;   /* f32, or f16x2 when the batch computes in binary16 */
     float clamp_value;
-    uint32_t reserved;
+    uint32_t status;                 /* written by the launch: 0 = decoded; otherwise CUPHY_HIP_ITEM_* reasons,
+                                        and the item's outputs are left untouched */
 } cuphyHipLDPCWorkItem_t;
+
+/* Reasons a batch launch does not decode a work item (cuphyHipLDPCWorkItem_t::status),
+ * the device-side counterpart of the argument checks of cuphy_ldpc.cpp:36-91. */
+#define CUPHY_HIP_ITEM_BAD_BG        0x01u  /* BG not 1 or 2 */
+#define CUPHY_HIP_ITEM_BAD_Z         0x02u  /* Z not one of the 51 TS 38.212 lifting sizes */
+#define CUPHY_HIP_ITEM_BAD_MB        0x04u  /* num_parity_nodes outside 4..46 (BG2: 4..42), or above 8 in a launch
+                                               whose max_parity_nodes <= 8 selects the high-code-rate kernel */
+#define CUPHY_HIP_ITEM_BAD_KB        0x08u  /* Kb not 22 (BG1) or 6, 8, 9, 10 (BG2) */
+#define CUPHY_HIP_ITEM_BAD_ITER      0x10u  /* max_iterations < 0 */
+#define CUPHY_HIP_ITEM_NULL_BUFFER   0x20u  /* llr or out is NULL */
+#define CUPHY_HIP_ITEM_NO_FIT        0x40u  /* the item's values or lanes exceed what the launch provides */
+#define CUPHY_HIP_ITEM_GROUP         0x80u  /* not decoded with its group: an earlier item of the group is invalid
+                                               or has another configuration (BG, Z, mb, Kb, iterations, flags,
+                                               normalisation, clamp) */
+
+/* Host-side check of one work item with the same predicate the batch
+ * kernels apply (every reason except NO_FIT and GROUP, which depend on the
+ * launch): *reasons = 0 for an item a launch with max_parity_nodes decodes. */
+cuphyStatus_t CUPHYWINAPI cuphyHipLDPCWorkItemCheck(const cuphyHipLDPCWorkItem_t* item, int max_parity_nodes,
+                                                    uint32_t* reasons);
 
 /* Workspace of cuphyHipLDPCBatchDecode: the device work-queue state.  The
  * caller zero-fills it once; every completed launch leaves it zeroed, so
@@ -376,8 +397,9 @@ cuphyStatus_t CUPHYWINAPI cuphyHipLDPCBatchDecodeGetWorkspaceSize(size_t* sizeIn
  * max_values_per_cb: max over items of (nsys + mb) * Z (sizes LDS);
  * max_parity_nodes: max over items of mb (<= 8 selects the register-lean
  * high-code-rate variant).  Order the items longest first for the shortest
- * makespan. */
-cuphyStatus_t CUPHYWINAPI cuphyHipLDPCBatchDecode(cuphyLDPCDecoder_t decoder, const cuphyHipLDPCWorkItem_t* d_items,
+ * makespan.  Each item's status word receives 0 or the reasons it was not
+ * decoded (CUPHY_HIP_ITEM_*); the items are otherwise only read. */
+cuphyStatus_t CUPHYWINAPI cuphyHipLDPCBatchDecode(cuphyLDPCDecoder_t decoder, cuphyHipLDPCWorkItem_t* d_items,
                                                   int num_items, cuphyDataType_t llr_type, int fp16_math,
                                                   int max_values_per_cb, int max_parity_nodes, void* d_workspace,
                                                   hipStream_t strm);
@@ -402,7 +424,7 @@ cuphyStatus_t CUPHYWINAPI cuphyHipLDPCBatchDecode(cuphyLDPCDecoder_t decoder, co
  * for batches of fewer code blocks than CUs, whose time is that of the
  * slowest code block.  Same workspace contract as cuphyHipLDPCBatchDecode. */
 cuphyStatus_t CUPHYWINAPI cuphyHipLDPCBatchDecodeGroups(cuphyLDPCDecoder_t decoder,
-                                                        const cuphyHipLDPCWorkItem_t* d_items,
+                                                        cuphyHipLDPCWorkItem_t* d_items,
                                                         const uint32_t* d_group_first, int num_groups,
                                                         cuphyDataType_t llr_type, int fp16_math,
                                                         int max_values_per_group, int max_parity_nodes,
@@ -588,6 +610,54 @@ typedef struct {
 cuphyStatus_t CUPHYWINAPI cuphyHipHarqServeTrace(cuphyHipHarqServer_t server, const cuphyHipHarqTraceEntry_t* trace,
                                                  int n, double harq_rtt_us, cuphyHipHarqCompletion_t* done,
                                                  int* n_done);
+
+/* ------------------------------------------------------------------------
+ * Per-TB parameter derivation (host only, no device call)
+ * ---------------------------------------------------------------------- */
+
+/* TS 38.214 5.1.3.2 transport block size and code-block count of an
+ * allocation: num_symbols data symbols, num_prbs PRBs, num_layers layers,
+ * code rate R (0 < R <= 1), modulation order Qm, num_dmrs_cdm_grps_no_data1_symbols
+ * DMRS symbols counted as half a data symbol.  Replaces
+ * get_TB_size_and_num_CBs (cuPHY/src/cuphy/utils.cuh:60-105) and pyAerial's
+ * get_tb_size (pyaerial/src/aerial/phy5g/ldpc/util.py:171-333, code rate as a
+ * fraction here).  INVALID_ARGUMENT on a non-positive size or rate. */
+cuphyStatus_t CUPHYWINAPI cuphyHipTransportBlockSize(int num_symbols, int num_prbs, int num_layers, float code_rate,
+                                                     uint32_t Qm, int num_dmrs_cdm_grps_no_data1_symbols,
+                                                     uint32_t* tb_size, uint32_t* num_CBs);
+
+/* Limited-buffer rate matching of a UE (TS 38.212 5.4.2.1, I_LBRM = 1):
+ * the PUSCH UE parameters i_lbrm, n_PRB_LBRM, maxLayers, maxQm
+ * (cuPHY/src/cuphy_channels/pusch_rx.cpp:8146-8164). */
+typedef struct {
+    uint8_t  i_lbrm;       /* 0: Ncb = N; 1: Ncb = min(N, Nref)            */
+    uint8_t  maxLayers;    /* 1..4                                          */
+    uint8_t  maxQm;        /* 6 or 8                                        */
+    uint8_t  reserved;
+    uint16_t n_PRB_LBRM;   /* 32..273                                       */
+    uint16_t reserved2;
+} cuphyHipLbrmPrm_t;
+
+/* PerTbParams of one PUSCH TB from its size and code rate, with the decoder's
+ * parity-node count and info-node count: base graph, CB segmentation (C, K',
+ * Zc, K, F, TS 38.212 5.2.2 / 6.2.2), N, Ncb (limited buffer when
+ * lbrm->i_lbrm), Ncb_padded, k0 of rv (Table 5.4.2.1-2), parity nodes
+ * covering min(G/C + k0, Ncb) (rounded up; down with LBRM, as the
+ * reference), nZpBitsPerCb, nDataBytes.  Same arguments and meaning as
+ * pyAerial's setPerTbParams (pyaerial/pybind11/pycuphy_ldpc_util.cpp:75-171:
+ * tbSize bits without CRC, codeRate as a fraction, rateMatchLen = G) with the
+ * PUSCH pipeline's LBRM handling (pusch_rx.cpp:8003-8236 expandBackEndParameters);
+ * lbrm may be NULL (no limited buffer).  Fields the derivation does not set
+ * are zeroed.  INVALID_ARGUMENT where the reference throws or rejects: rv > 3,
+ * Qm not in {1,2,4,6,8}, zero TB size or rate, G not a multiple of Qm*numLayers,
+ * numLayers or numUeGrpLayers outside 1..8 or below numLayers, layer map
+ * entries >= numUeGrpLayers, LBRM parameters out of range. */
+cuphyStatus_t CUPHYWINAPI cuphyHipSetPerTbParams(PerTbParams* tbParams, uint32_t tbSize, float codeRate,
+                                                 uint8_t qamMod, uint32_t ndi, uint32_t rv, uint32_t rateMatchLen,
+                                                 uint32_t cinit, uint32_t userGroupIdx, uint8_t numLayers,
+                                                 uint8_t numUeGrpLayers, const uint32_t* layerMapArray,
+                                                 uint8_t nDmrsCdmGrpsNoData, const cuphyHipLbrmPrm_t* lbrm,
+                                                 uint16_t* numParityNodes, uint16_t* Kb);
 
 #ifdef __cplusplus
 }

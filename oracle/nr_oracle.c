@@ -13,10 +13,12 @@
  * Build: see Makefile (gcc -O2 -ffp-contract=off, no fast-math: every
  * float operation is one IEEE-754 RN operation).
  */
+#define _GNU_SOURCE /* sched_getaffinity, pthread_setaffinity_np (pinned CPU-baseline threads) */
 #include "nr_oracle.h"
 
 #include <math.h>
 #include <pthread.h>
+#include <sched.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -430,6 +432,109 @@ int nr_orc_k0(int bg, int rv, int Ncb, int Zc)
     return (int)(((long long)n2[rv] * Ncb) / (50LL * Zc)) * Zc;
 }
 
+/* ======================================================================
+ * transport block size and coding parameters
+ * (5GModel/nr_matlab/pxsch/derive_coding_main.m:78-301)
+ * ==================================================================== */
+/* TS 38.214 Table 5.1.3.2-1 (the TBS_table of derive_coding_main.m:84) */
+static const int kOrcTbs[93] = {
+    24, 32, 40, 48, 56, 64, 72, 80, 88, 96, 104, 112, 120, 128, 136, 144, 152, 160, 168, 176, 184, 192, 208, 224,
+    240, 256, 272, 288, 304, 320, 336, 352, 368, 384, 408, 432, 456, 480, 504, 528, 552, 576, 608, 640, 672, 704,
+    736, 768, 808, 848, 888, 928, 984, 1032, 1064, 1128, 1160, 1192, 1224, 1256, 1288, 1320, 1352, 1416, 1480,
+    1544, 1608, 1672, 1736, 1800, 1864, 1928, 2024, 2088, 2152, 2216, 2280, 2408, 2472, 2536, 2600, 2664, 2728,
+    2792, 2856, 2976, 3104, 3240, 3368, 3496, 3624, 3752, 3824};
+
+int nr_orc_tbs_table(int i) { return (i >= 0 && i < 93) ? kOrcTbs[i] : -1; }
+
+/* derive_coding_main.m:86-125 in binary64: N_info = N_RE R Qm nl with
+ * N_RE = min(156, re_per_prb) n_prb.  For R <= 1/4 the code-block count
+ * uses N'_info (TS 38.214 5.1.3.2; MATLAB's line :111 writes N_info and a
+ * strict <, which the standard does not). */
+int nr_orc_tb_size(double re_per_prb, int n_prb, double R, int Qm, int nl)
+{
+    const double Nre = fmin(156.0, re_per_prb) * n_prb;
+    const double Ninfo = Nre * R * Qm * nl;
+    if (Ninfo <= 3824.0) {
+        const double n = fmax(3.0, floor(log2(Ninfo)) - 6.0);
+        const double Np = fmax(24.0, pow(2.0, n) * floor(Ninfo / pow(2.0, n)));
+        for (int i = 0; i < 93; ++i)
+            if (kOrcTbs[i] >= Np) return kOrcTbs[i];
+        return -1;
+    }
+    const double n = floor(log2(Ninfo - 24.0)) - 5.0;
+    const double Np = fmax(3840.0, pow(2.0, n) * round((Ninfo - 24.0) / pow(2.0, n)));
+    double C = 1.0;
+    if (R <= 0.25) C = ceil((Np + 24.0) / 3816.0);
+    else if (Np > 8424.0) C = ceil((Np + 24.0) / 8424.0);
+    return (int)(8.0 * C * ceil((Np + 24.0) / (8.0 * C)) - 24.0);
+}
+
+/* derive_coding_main.m:127-301: base graph, TB CRC, segmentation (C, K'),
+ * K_b, lifting size Zc (smallest with Zc K_b >= K'), K, F, N and the
+ * limited-buffer N_ref (0 without LBRM). */
+int nr_orc_derive_coding(int tbs, double R, int i_lbrm, int n_prb_lbrm, int max_layers, int max_qm,
+                         nr_orc_coding* o)
+{
+    static const int Zs[51] = {2, 4, 8, 16, 32, 64, 128, 256, 3, 6, 12, 24, 48, 96, 192, 384, 5, 10,
+                               20, 40, 80, 160, 320, 7, 14, 28, 56, 112, 224, 9, 18, 36, 72, 144, 288, 11,
+                               22, 44, 88, 176, 352, 13, 26, 52, 104, 208, 15, 30, 60, 120, 240};
+    memset(o, 0, sizeof(*o));
+    o->tbs = tbs;
+    o->bg = (tbs <= 292 || (tbs <= 3824 && R <= 0.67) || R <= 0.25) ? 2 : 1;       /* :138-143 */
+    o->crc_len = tbs > 3824 ? 24 : 16;                                              /* :145-154 */
+    o->B = tbs + o->crc_len;
+    const int Kcb = o->bg == 1 ? 8448 : 3840;                                       /* :163-167 */
+    if (o->B <= Kcb) {
+        o->C = 1;
+        o->Kp = o->B;
+    } else {
+        o->C = (int)ceil((double)o->B / (Kcb - 24));
+        if ((o->B + 24 * o->C) % o->C) return -1;                                  /* K' = B'/C exact */
+        o->Kp = (o->B + 24 * o->C) / o->C;
+    }
+    if (o->bg == 1) o->Kb = 22;                                                     /* :182-198 */
+    else o->Kb = o->B > 640 ? 10 : (o->B > 560 ? 9 : (o->B > 192 ? 8 : 6));
+    int best = -1;                                                                  /* :207-211 */
+    for (int i = 0; i < 51; ++i) {
+        const int d = Zs[i] * o->Kb - o->Kp;
+        if (d >= 0 && (best < 0 || d < Zs[best] * o->Kb - o->Kp)) best = i;
+    }
+    if (best < 0) return -1;
+    o->Zc = Zs[best];
+    o->ils = best < 8 ? 0 : best < 16 ? 1 : best < 23 ? 2 : best < 29 ? 3 : best < 35 ? 4 : best < 41 ? 5
+                                                                                        : best < 46 ? 6 : 7;
+    o->K = o->Zc * (o->bg == 1 ? 22 : 10);                                          /* :263-269 */
+    o->F = o->K - o->Kp;
+    o->N = o->Zc * (o->bg == 1 ? 66 : 50);
+    if (i_lbrm) {                                                                   /* :276-289 */
+        const double ninfo = 156.0 * n_prb_lbrm * max_qm * (948.0 / 1024.0) * (max_layers < 4 ? max_layers : 4);
+        const double n = floor(log2(ninfo - 24.0)) - 5.0;
+        const double Np = fmax(3840.0, pow(2.0, n) * round((ninfo - 24.0) / pow(2.0, n)));
+        const double Cl = ceil((Np + 24.0) / 8424.0);
+        const double tbs_lbrm = 8.0 * Cl * ceil((Np + 24.0) / (8.0 * Cl)) - 24.0;
+        o->tbs_lbrm = (int)tbs_lbrm;
+        o->Nref = (int)floor(tbs_lbrm / (o->C * (2.0 / 3.0)));
+    }
+    return 0;
+}
+
+/* derate_match.m:122-129: parity nodes covering the received LLRs,
+ * nV_parity = clamp(ceil((min(floor(G/C) + k0, N) - (K - F - 2Zc)) / Zc), 4, 46/42),
+ * N = 66 Zc (BG1) or 50 Zc (BG2). */
+int nr_orc_num_parity_nodes(int bg, int Zc, int K, int F, int G, int C, int k0)
+{
+    const long long N = (bg == 1 ? 66LL : 50LL) * Zc;
+    long long n = (long long)(G / C) + k0;
+    if (n > N) n = N;
+    const long long par = n - (K - F - 2LL * Zc);
+    long long mb = (par + Zc - 1) / Zc; /* ceil for par > 0 */
+    if (par <= 0) mb = 0;
+    const int maxr = bg == 1 ? 46 : 42;
+    if (mb < 4) mb = 4;
+    if (mb > maxr) mb = maxr;
+    return (int)mb;
+}
+
 /* derate_match.m:54-63 */
 int nr_orc_rm_E(const nr_orc_rm_cfg* c, int r)
 {
@@ -460,6 +565,19 @@ void nr_orc_gold(unsigned int cinit, int n0, int nbits, unsigned char* bits)
     free(x2);
 }
 
+/* Bit selection (derate_match.m:152-170, TS 38.212 5.4.2.1): circular-buffer
+ * positions of the E selected bits, starting at k0 and skipping the filler
+ * bits [Kd, Kd + F). */
+void nr_orc_bit_select(int E, int Kd, int F, int k0, int Ncb, int* pos)
+{
+    int k = 0, j = 0;
+    while (k < E) {
+        const int idx = (k0 + j) % Ncb;
+        if (idx < Kd || idx >= Kd + F) pos[k++] = idx;
+        ++j;
+    }
+}
+
 static void derate_match_impl(const nr_orc_rm_cfg* c, const float* llr_demap, float* harq, int fp16)
 {
     const int twoZ = 2 * c->Zc;
@@ -480,10 +598,11 @@ static void derate_match_impl(const nr_orc_rm_cfg* c, const float* llr_demap, fl
             for (int n = 0; n < c->Ncb_padded; ++n) out[n] = 0.0f;
         /* bit selection after de-interleaving (derate_match.m:138-170):
          * the k-th selected LLR is e_k = LLR_c[(k % EQ)*Qm + k / EQ] */
-        int k = 0, j = 0;
-        while (k < E) {
-            const int idx = (c->k0 + j) % c->Ncb;
-            if (idx < fill_lo || idx >= fill_hi) {
+        int* pos = (int*)malloc((size_t)E * sizeof(int));
+        nr_orc_bit_select(E, fill_lo, fill_hi - fill_lo, c->k0, c->Ncb, pos);
+        for (int k = 0; k < E; ++k) {
+            const int idx = pos[k];
+            {
                 const int kIdx = k / EQ, jIdx = k % EQ;
                 const size_t n = start + (size_t)jIdx * c->Qm + kIdx;
                 float v = llr_demap[n];
@@ -505,10 +624,9 @@ static void derate_match_impl(const nr_orc_rm_cfg* c, const float* llr_demap, fl
                     if (s < -LLR_MAX_ABS) s = -LLR_MAX_ABS;
                     *o = s;
                 }
-                ++k;
             }
-            ++j;
         }
+        free(pos);
         for (int n = fill_lo; n < fill_hi; ++n) out[twoZ + n] = LLR_MAX_ABS; /* :179 */
         for (int n = 0; n < twoZ; ++n) out[n] = 0.0f;                        /* punctured */
         start += (size_t)E;
@@ -583,29 +701,53 @@ typedef struct {
     size_t llr_stride;
     uint32_t* hard;
     size_t hard_stride;
-    int ncb, tid, nthreads;
+    int* iters;
+    int ncb, tid, nthreads, pin;
 } par_job_t;
 
 static void* par_worker(void* p)
 {
     par_job_t* j = (par_job_t*)p;
-    for (int i = j->tid; i < j->ncb; i += j->nthreads)
-        nr_orc_decode_cb(j->cfg, j->llr + (size_t)i * j->llr_stride,
-                         j->hard ? j->hard + (size_t)i * j->hard_stride : NULL, NULL);
+    if (j->pin) { /* thread t on logical CPU t of the ones this process may use */
+        cpu_set_t allowed, one;
+        if (sched_getaffinity(0, sizeof(allowed), &allowed) == 0) {
+            int n = 0;
+            for (int c = 0; c < CPU_SETSIZE; ++c) {
+                if (!CPU_ISSET(c, &allowed)) continue;
+                if (n++ == j->tid) {
+                    CPU_ZERO(&one);
+                    CPU_SET(c, &one);
+                    pthread_setaffinity_np(pthread_self(), sizeof(one), &one);
+                    break;
+                }
+            }
+        }
+    }
+    for (int i = j->tid; i < j->ncb; i += j->nthreads) {
+        const int it = nr_orc_decode_cb(j->cfg, j->llr + (size_t)i * j->llr_stride,
+                                        j->hard ? j->hard + (size_t)i * j->hard_stride : NULL, NULL);
+        if (j->iters) j->iters[i] = it;
+    }
     return NULL;
 }
 
-int nr_orc_decode_cbs(const nr_orc_dec_cfg* cfg, const float* llr, size_t llr_stride, int ncb,
-                      uint32_t* hard, size_t hard_stride, int nthreads)
+int nr_orc_decode_cbs_it(const nr_orc_dec_cfg* cfg, const float* llr, size_t llr_stride, int ncb,
+                         uint32_t* hard, size_t hard_stride, int* iters, int nthreads, int pin)
 {
     if (nthreads < 1) nthreads = 1;
     if (nthreads > 256) nthreads = 256;
     pthread_t th[256];
     par_job_t jobs[256];
     for (int t = 0; t < nthreads; ++t) {
-        jobs[t] = (par_job_t){cfg, llr, llr_stride, hard, hard_stride, ncb, t, nthreads};
+        jobs[t] = (par_job_t){cfg, llr, llr_stride, hard, hard_stride, iters, ncb, t, nthreads, pin};
         if (pthread_create(&th[t], NULL, par_worker, &jobs[t]) != 0) return -1;
     }
     for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
     return 0;
+}
+
+int nr_orc_decode_cbs(const nr_orc_dec_cfg* cfg, const float* llr, size_t llr_stride, int ncb,
+                      uint32_t* hard, size_t hard_stride, int nthreads)
+{
+    return nr_orc_decode_cbs_it(cfg, llr, llr_stride, ncb, hard, hard_stride, NULL, nthreads, 0);
 }

@@ -97,6 +97,11 @@ int nr_orc_decode_cb(const nr_orc_dec_cfg* cfg, const float* llr, uint32_t* hard
 int nr_orc_decode_cbs(const nr_orc_dec_cfg* cfg, const float* llr, size_t llr_stride, int ncb,
                       uint32_t* hard, size_t hard_stride, int nthreads);
 
+/* the same with the iterations run per code block (iters may be NULL);
+ * pin != 0: thread t runs on the t-th CPU this process may use */
+int nr_orc_decode_cbs_it(const nr_orc_dec_cfg* cfg, const float* llr, size_t llr_stride, int ncb,
+                         uint32_t* hard, size_t hard_stride, int* iters, int nthreads, int pin);
+
 /* one check-node update (compute_cC2V.m + update_C2V.m) on d inputs t[] in
  * the given mode; c2v[j] = sign * (j == argmin ? min2 : min1), unnormalised */
 void nr_orc_check_node(int d, const float* t, float* c2v, int mode);
@@ -130,6 +135,25 @@ int  nr_orc_rm_E(const nr_orc_rm_cfg* c, int r);
 void nr_orc_derate_match(const nr_orc_rm_cfg* c, const float* llr_demap, float* harq);
 /* same, HARQ buffer held in binary16 (every stored value rounded RN) */
 void nr_orc_derate_match_fp16(const nr_orc_rm_cfg* c, const float* llr_demap, float* harq);
+/* ---- TB size and coding parameters (derive_coding_main.m:78-301) ------- */
+typedef struct {
+    int tbs, bg, crc_len, B, C, Kp, Kb, Zc, ils, K, F, N;
+    int tbs_lbrm, Nref;      /* limited buffer: TBS_LBRM and N_ref (0 without) */
+} nr_orc_coding;
+/* entry i of TS 38.214 Table 5.1.3.2-1 (-1 outside 0..92) */
+int nr_orc_tbs_table(int i);
+/* TBS of an allocation (N_RE = min(156, re_per_prb) * n_prb, rate R) */
+int nr_orc_tb_size(double re_per_prb, int n_prb, double R, int Qm, int nl);
+/* coding parameters of a TB of tbs payload bits; returns 0, or -1 if no
+ * lifting size fits */
+int nr_orc_derive_coding(int tbs, double R, int i_lbrm, int n_prb_lbrm, int max_layers, int max_qm,
+                         nr_orc_coding* out);
+
+/* circular-buffer positions of the E selected bits from k0, filler bits
+ * [Kd, Kd + F) skipped (derate_match.m:152-170) */
+void nr_orc_bit_select(int E, int Kd, int F, int k0, int Ncb, int* pos);
+/* parity nodes the decoder uses for a transmission (derate_match.m:122-129) */
+int  nr_orc_num_parity_nodes(int bg, int Zc, int K, int F, int G, int C, int k0);
 /* TS 38.212 5.4.2.1 k0 */
 int  nr_orc_k0(int bg, int rv, int Ncb, int Zc);
 

@@ -27,6 +27,11 @@ class DecCfg(C.Structure):
                 ("alpha", C.c_float), ("clamp", C.c_float)]
 
 
+class Coding(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("tbs", "bg", "crc_len", "B", "C", "Kp", "Kb", "Zc", "ils", "K", "F", "N",
+                                       "tbs_lbrm", "Nref")]
+
+
 class RmCfg(C.Structure):
     _fields_ = [("bg", C.c_int), ("Zc", C.c_int), ("C", C.c_int), ("Qm", C.c_int), ("Nl", C.c_int),
                 ("G", C.c_int), ("K", C.c_int), ("F", C.c_int), ("Ncb", C.c_int),
@@ -57,6 +62,14 @@ def lib():
         L.nr_orc_decode_cb.argtypes = [P(DecCfg), C.c_void_p, C.c_void_p, C.c_void_p]
         L.nr_orc_decode_cbs.argtypes = [P(DecCfg), C.c_void_p, C.c_size_t, C.c_int, C.c_void_p,
                                         C.c_size_t, C.c_int]
+        L.nr_orc_decode_cbs_it.argtypes = [P(DecCfg), C.c_void_p, C.c_size_t, C.c_int, C.c_void_p,
+                                           C.c_size_t, C.c_void_p, C.c_int, C.c_int]
+        L.nr_orc_num_parity_nodes.argtypes = [C.c_int] * 7
+        L.nr_orc_bit_select.argtypes = [C.c_int] * 5 + [C.c_void_p]
+        L.nr_orc_bit_select.restype = None
+        L.nr_orc_tbs_table.argtypes = [C.c_int]
+        L.nr_orc_tb_size.argtypes = [C.c_double, C.c_int, C.c_double, C.c_int, C.c_int]
+        L.nr_orc_derive_coding.argtypes = [C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, P(Coding)]
         L.nr_orc_check_node.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_int]
         L.nr_orc_round_half.argtypes = [C.c_float]
         L.nr_orc_round_half.restype = C.c_float
@@ -169,6 +182,50 @@ def decode_cbs(cfg, llr2d, nthreads=1, hard_stride=None):
     rc = lib().nr_orc_decode_cbs(C.byref(cfg), _p(llr2d), llr2d.shape[1], ncb, _p(hard), hs, nthreads)
     assert rc == 0
     return hard
+
+
+def decode_cbs_it(cfg, llr2d, nthreads=1, hard_stride=None, pin=False):
+    """Decode every row of llr2d on nthreads host threads (pinned to CPUs
+    when pin); returns (hard words (ncb, hard_stride), iterations (ncb,))."""
+    llr2d = np.ascontiguousarray(llr2d, np.float32)
+    ncb = llr2d.shape[0]
+    nw = (cfg.kb_out * cfg.Z + 31) // 32
+    hs = nw if hard_stride is None else hard_stride
+    hard = np.zeros((ncb, hs), np.uint32)
+    iters = np.zeros(ncb, np.int32)
+    rc = lib().nr_orc_decode_cbs_it(C.byref(cfg), _p(llr2d), llr2d.shape[1], ncb, _p(hard), hs, _p(iters),
+                                    nthreads, int(pin))
+    assert rc == 0
+    return hard, iters
+
+
+def tbs_table():
+    return [lib().nr_orc_tbs_table(i) for i in range(93)]
+
+
+def tb_size(re_per_prb, n_prb, R, Qm, nl):
+    """TS 38.214 5.1.3.2 TBS (derive_coding_main.m:78-125)."""
+    return lib().nr_orc_tb_size(float(re_per_prb), n_prb, float(R), Qm, nl)
+
+
+def derive_coding(tbs, R, i_lbrm=0, n_prb_lbrm=273, max_layers=4, max_qm=8):
+    """Coding parameters of a TB (derive_coding_main.m:127-301) as a dict."""
+    o = Coding()
+    rc = lib().nr_orc_derive_coding(tbs, float(R), i_lbrm, n_prb_lbrm, max_layers, max_qm, C.byref(o))
+    assert rc == 0, (tbs, R)
+    return {n: getattr(o, n) for n, _ in o._fields_}
+
+
+def bit_select(E, Kd, F, k0, Ncb):
+    """Circular-buffer positions of the E selected bits (filler [Kd, Kd+F) skipped)."""
+    pos = np.zeros(E, np.int32)
+    lib().nr_orc_bit_select(E, Kd, F, k0, Ncb, _p(pos))
+    return pos
+
+
+def num_parity_nodes(bg, Zc, K, F, G, C_, k0):
+    """Parity nodes of a transmission (derate_match.m:122-129)."""
+    return lib().nr_orc_num_parity_nodes(bg, Zc, K, F, G, C_, k0)
 
 
 def check_node(t, mode=FP32):

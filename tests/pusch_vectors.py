@@ -4,6 +4,9 @@
 k0 skipping the filler bits, then bit interleaving, TS 38.212 5.4.2) and
 BPSK-AWGN channel LLRs per transmitted bit, laid out like the equalizer
 output the de-rate-matcher reads (QAM_STRIDE = 8 slots per symbol)."""
+import json
+import os
+
 import numpy as np
 
 import oracle as O
@@ -103,3 +106,17 @@ def quantize_int8(llr, steps_per_unit=4):
     (numpy rounding, half to even); the value seen downstream is q / steps."""
     q = np.clip(np.round(np.asarray(llr, np.float64) * steps_per_unit), -127, 127).astype(np.int8)
     return q
+
+
+_NORM = None
+
+
+def table_alpha(bg, mb):
+    """The reference's min-sum normalisation for (BG, parity nodes), rounded
+    to binary16 (fixture golden/min_sum_norm.json, extracted from
+    ldpc.cpp:139-235 and LDPC_decode.m:101-208)."""
+    global _NORM
+    if _NORM is None:
+        _NORM = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                            "min_sum_norm.json")))
+    return float(np.float16(_NORM["BG%d" % bg][mb]))

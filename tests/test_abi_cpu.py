@@ -180,3 +180,80 @@ def test_harq_server_argument_validation(L):
     assert lib.cuphyHipHarqServerPoll(None, None, 0, C.byref(n)) == INV
     assert lib.cuphyHipHarqServerRelease(None, 1) == INV
     assert lib.cuphyHipHarqServeTrace(None, None, 0, 0.0, None, C.byref(n)) == INV
+
+
+def test_set_normalization_is_the_reference_table(L):
+    """SetNormalization gives, for every (BG, parity nodes), the reference's
+    g_min_sum_norm_BG{1,2}_Z384 entry (ldpc.cpp:139-235 = LDPC_decode.m:101-208,
+    fixture extracted from both by tests/golden/extract_norm_table.py):
+    binary32 as is, binary16 rounded to nearest even into both halves."""
+    import json
+    import numpy as np
+    table = json.load(open(os.path.join(ROOT, "tests", "golden", "min_sum_norm.json")))
+    lib = L.lib()
+    # the decoder object reads only the context's device fields; SetNormalization is host only,
+    # so a zero-filled context stands in for a device context here
+    ctx = (C.c_uint8 * 64)()
+    dec = C.c_void_p()
+    assert lib.cuphyCreateLDPCDecoder(C.cast(ctx, C.c_void_p), C.byref(dec), 0) == L.CUPHY_STATUS_SUCCESS
+    try:
+        for bg in (1, 2):
+            vals = table["BG%d" % bg]
+            for mb in range(0, len(vals) + 2):
+                for llr_type in (L.CUPHY_R_32F, L.CUPHY_R_16F):
+                    cfg = L.LDPCDecodeConfigDesc()
+                    cfg.llr_type, cfg.BG, cfg.num_parity_nodes, cfg.Z, cfg.Kb = llr_type, bg, mb, 384, 22 if bg == 1 else 10
+                    st = lib.cuphyErrorCorrectionLDPCDecodeSetNormalization(dec, C.byref(cfg))
+                    if mb < 4 or mb >= len(vals):
+                        assert st == L.CUPHY_STATUS_INVALID_ARGUMENT, (bg, mb)
+                        continue
+                    assert st == L.CUPHY_STATUS_SUCCESS
+                    if llr_type == L.CUPHY_R_32F:
+                        assert cfg.norm.f32 == np.float32(vals[mb]), (bg, mb)
+                    else:
+                        h = int(np.array([vals[mb]], np.float32).astype(np.float16).view(np.uint16)[0])
+                        assert cfg.norm.f16x2 == (h | (h << 16)), (bg, mb)
+    finally:
+        lib.cuphyDestroyLDPCDecoder(dec)
+
+
+def test_work_item_check(L):
+    """The work-item predicate the persistent kernels apply, on the host:
+    every valid (BG, lifting Z, mb, Kb) passes; each kind of bad field is
+    named, alone and combined."""
+    lib = L.lib()
+    lifting = sorted({a << j for a in (2, 3, 5, 7, 9, 11, 13, 15) for j in range(8) if (a << j) <= 384})
+
+    def check(max_mb=46, **kw):
+        w = L.LDPCWorkItem(llr=16, out=32, soft=None, BG=1, Z=384, num_parity_nodes=46, Kb=22, max_iterations=10,
+                           flags=0, clamp_value=32.0, status=0)
+        for k, v in kw.items():
+            setattr(w, k, v)
+        r = C.c_uint32(0xDEAD)
+        assert lib.cuphyHipLDPCWorkItemCheck(C.byref(w), max_mb, C.byref(r)) == L.CUPHY_STATUS_SUCCESS
+        return r.value
+
+    for bg in (1, 2):
+        for Z in lifting:
+            for mb in range(4, (46 if bg == 1 else 42) + 1):
+                for kb in ((22,) if bg == 1 else (6, 8, 9, 10)):
+                    assert check(BG=bg, Z=Z, num_parity_nodes=mb, Kb=kb) == 0, (bg, Z, mb, kb)
+    for Z in [z for z in range(-2, 400) if z not in lifting] + [1000, -32768]:
+        assert check(Z=Z) == L.ITEM_BAD_Z, Z
+    for bg in (0, 3, -1, 7):
+        assert check(BG=bg) & L.ITEM_BAD_BG, bg
+    for mb in (0, 3, 47, -5):
+        assert check(num_parity_nodes=mb) == L.ITEM_BAD_MB, mb
+    assert check(BG=2, Kb=10, num_parity_nodes=43) == L.ITEM_BAD_MB
+    assert check(num_parity_nodes=9, max_mb=8) == L.ITEM_BAD_MB      # beyond the high-code-rate kernel's rows
+    assert check(num_parity_nodes=8, max_mb=8) == 0
+    assert check(num_parity_nodes=46, max_mb=9) == 0                 # max_mb > 8 runs the 46-row kernel
+    for bg, kb in ((1, 10), (1, 21), (2, 22), (2, 7), (2, 5), (2, 11)):
+        assert check(BG=bg, Kb=kb, num_parity_nodes=10) == L.ITEM_BAD_KB, (bg, kb)
+    assert check(max_iterations=-1) == L.ITEM_BAD_ITER
+    assert check(max_iterations=0) == 0
+    assert check(llr=None) == L.ITEM_NULL_BUFFER and check(out=None) == L.ITEM_NULL_BUFFER
+    assert check(BG=3, Z=5 * 7, num_parity_nodes=2, max_iterations=-2, out=None) == \
+        L.ITEM_BAD_BG | L.ITEM_BAD_Z | L.ITEM_BAD_MB | L.ITEM_BAD_KB | L.ITEM_BAD_ITER | L.ITEM_NULL_BUFFER
+    r = C.c_uint32()
+    assert lib.cuphyHipLDPCWorkItemCheck(None, 46, C.byref(r)) == L.CUPHY_STATUS_INVALID_ARGUMENT

@@ -250,3 +250,169 @@ def test_batch_lane_major_equals_column_major(dec, et):
         outs.append([(tb.out.cpu().numpy(), tb.soft.cpu().numpy()) for tb in tbs])
     for (h0, s0), (h1, s1) in zip(*outs):
         assert np.array_equal(h0, h1) and np.array_equal(s0.view(np.uint16), s1.view(np.uint16))
+
+
+def test_full_load_slot_slice_matches_oracle(dec):
+    """Two cells of the 20-cell full-load slot (280 CBs, BG1 Zc=384, R=0.93
+    -> mb=4, consistent-Gaussian LLRs with mu from the cell's SNR, binary16,
+    10 iterations), generated and batch-decoded exactly as the benchmark's
+    full-load section does (same seed, same first cells, graph replay of the
+    tuned throughput-regime launch): every code block equals the oracle's
+    binary16 decode bit for bit, and the plain per-TB form gives the same
+    words."""
+    from aerial_amd import _lib as L
+    from aerial_amd import workloads as W
+    from aerial_amd.batch import BatchDecoder
+    from aerial_amd.decoder import LdpcDecodeConfig
+    print("[case] full-load slot slice: 2 cells x 140 CBs BG1 Zc=384 mb=4 fp16 10 iterations seed=3", flush=True)
+    c3 = W.full_load_slot(seed=3, n_cells=2, cbs_per_cell=140)
+    assert all((tb.bg, tb.Z, tb.mb, tb.ncb) == (1, 384, 4, 140) for tb in c3)
+    b = BatchDecoder(dec).build(c3, llr_type=L.CUPHY_R_16F, fp16_math=True, max_iterations=10, alpha=0.8125)
+    assert sum(tb.ncb for tb in c3) >= b.split_min  # throughput regime, as the 2,800-CB slot
+    b.capture()
+    for tb in c3:
+        tb.out.fill_(-1)
+    torch.cuda.synchronize()
+    b.replay()
+    torch.cuda.synchronize()
+    batched = [tb.out.clone() for tb in c3]
+    assert all(int(w.abs().sum().item()) == 0 for w in b.workspaces)
+    for tb in c3:
+        tb.out.fill_(-1)
+        cfg = LdpcDecodeConfig(1, 384, 4, llr_type=L.CUPHY_R_16F, max_iterations=10, alpha=0.8125, clamp_value=32.0)
+        dec.decode_tb(cfg, [tb.llr], [tb.out])
+    torch.cuda.synchronize()
+    ocfg = O.dec_cfg(1, 384, 4, n_iter=10, alpha=0.8125, clamp=32.0, mode=O.FP16)
+    for k, (tb, got) in enumerate(zip(c3, batched)):
+        assert torch.equal(tb.out, got), "plain and batched words differ in cell %d" % k
+        llr = tb.llr.float().cpu().numpy()[:, :tb.num_llrs]
+        ref = O.decode_cbs(ocfg, llr, nthreads=8)
+        assert np.array_equal(got.cpu().numpy().view(np.uint32), ref), "cell %d differs from the oracle" % k
+
+
+def _launch_items(dec, arr, first, max_values, max_mb, block, llr_type):
+    """One cuphyHipLDPCBatchDecodeGroups launch over a host item array;
+    returns the items as the device left them (status words)."""
+    import ctypes as C
+    from aerial_amd import _lib as L
+    from aerial_amd.batch import ITEM_DTYPE
+    items = torch.from_numpy(arr.view(np.uint8).copy()).cuda()
+    gf = torch.from_numpy(np.asarray(first, np.uint32).view(np.int32).copy()).cuda() if first is not None else None
+    ws = torch.zeros(64, dtype=torch.int32, device="cuda")
+    n = (len(first) - 1) if first is not None else len(arr)
+    L.check("cuphyHipLDPCBatchDecodeGroups", L.lib().cuphyHipLDPCBatchDecodeGroups(
+        dec.dec, C.c_void_p(items.data_ptr()), C.c_void_p(gf.data_ptr() if gf is not None else None), n, llr_type,
+        1, max_values, max_mb, block, C.c_void_p(ws.data_ptr()), None))
+    torch.cuda.synchronize()
+    assert int(ws.abs().sum().item()) == 0
+    return items.cpu().numpy().view(ITEM_DTYPE)
+
+
+@pytest.mark.parametrize("mode", ["grouped", "lane_major", "row_pair"])
+def test_batch_rejects_invalid_items(dec, mode):
+    """Invalid work items (BG not 1/2, Z not a lifting size, mb or Kb out of
+    range, negative iterations, null buffers, another configuration inside a
+    group, more rows than the launch's kernel holds) are not decoded: their
+    outputs stay untouched and their status word names the reason, while the
+    valid items of the same launch equal the oracle bit for bit."""
+    from aerial_amd import _lib as L
+    from aerial_amd.batch import ITEM_DTYPE, _half_bits, group_values
+    print("[case] batch with invalid items, %s" % mode, flush=True)
+    mix = [(1, 64, 10, 4), (2, 104, 6, 3), (1, 36, 20, 4)]
+    tbs, refs = make_tbs("fp16", seed=61, mix=mix)
+    h = _half_bits(0.8125)
+    sink = torch.full((64, 512), -1, dtype=torch.int32, device="cuda")  # outputs of the invalid items
+
+    def item(tb, c, out=None, **kw):
+        a = np.zeros(1, ITEM_DTYPE)
+        a["llr"] = tb.llr.data_ptr() + c * tb.llr.stride(0) * 2
+        a["out"] = (tb.out.data_ptr() + c * tb.out.stride(0) * 4) if out is None else out
+        a["BG"], a["Z"], a["num_parity_nodes"], a["Kb"] = tb.bg, tb.Z, tb.mb, 22 if tb.bg == 1 else 10
+        a["max_iterations"], a["norm"], a["clamp_value"], a["status"] = 7, h | (h << 16), 32.0, 0xABCD
+        for k, v in kw.items():
+            a[k] = v
+        return a
+
+    sink_row = [0]
+
+    def bad(tb, **kw):
+        r = sink_row[0]
+        sink_row[0] += 1
+        out = kw.pop("out", sink.data_ptr() + r * 512 * 4)
+        return item(tb, 0, out=out, **kw)
+
+    A, B, Cc = tbs
+    BAD = [(dict(BG=3), L.ITEM_BAD_BG | L.ITEM_BAD_KB), (dict(Z=17), L.ITEM_BAD_Z), (dict(Z=0), L.ITEM_BAD_Z),
+           (dict(Z=500), L.ITEM_BAD_Z), (dict(num_parity_nodes=47), L.ITEM_BAD_MB),
+           (dict(num_parity_nodes=3), L.ITEM_BAD_MB), (dict(Kb=21), L.ITEM_BAD_KB),
+           (dict(max_iterations=-1), L.ITEM_BAD_ITER), (dict(llr=0), L.ITEM_NULL_BUFFER)]
+    rows, expect, first = [], [], [0]
+    if mode == "grouped":
+        # group 1: A's four code blocks with a different mb third: decoded up to it
+        rows += [item(A, 0), item(A, 1), bad(A, num_parity_nodes=11), item(A, 3)]
+        expect += [0, 0, L.ITEM_GROUP, L.ITEM_GROUP]
+        first.append(len(rows))
+        # group 2: an invalid first item rejects the whole group
+        rows += [bad(B, Kb=7), bad(B), bad(B)]
+        expect += [L.ITEM_BAD_KB, L.ITEM_GROUP, L.ITEM_GROUP]
+        first.append(len(rows))
+        # group 3: a null output buffer cuts the group
+        rows += [item(Cc, 0), item(Cc, 1), bad(Cc, out=0), bad(Cc)]
+        expect += [0, 0, L.ITEM_NULL_BUFFER, L.ITEM_GROUP]
+        first.append(len(rows))
+        decoded = {(0, 0), (0, 1), (2, 0), (2, 1)}
+        block, grp = 384, first
+    else:
+        for t, tb in enumerate(tbs):
+            for c in range(tb.ncb):
+                rows.append(item(tb, c))
+                expect.append(0)
+        decoded = {(t, c) for t, tb in enumerate(tbs) for c in range(tb.ncb)}
+        block, grp = (768, list(range(len(rows) + len(BAD) + 1))) if mode == "row_pair" else (384, None)
+    for kw, why in BAD:
+        rows.append(bad(A, **kw))
+        expect.append(why)
+    if mode == "grouped":
+        for kw, why in BAD:
+            first.append(first[-1] + 1)
+        grp = first
+    elif mode == "row_pair":
+        grp = list(range(len(rows) + 1))
+    arr = np.concatenate(rows)
+    max_values = max(group_values(tb.bg, tb.Z, tb.mb, tb.ncb) for tb in tbs)
+    got = _launch_items(dec, arr, grp, max_values, max(tb.mb for tb in tbs), block, L.CUPHY_R_16F)
+    assert list(got["status"]) == expect, (list(got["status"]), expect)
+    assert int((sink != -1).sum().item()) == 0, "an invalid item's output was written"
+    for t, (tb, ref) in enumerate(zip(tbs, refs)):
+        out = tb.out.cpu().numpy().view(np.uint32)
+        for c in range(tb.ncb):
+            if (t, c) in decoded:
+                assert np.array_equal(out[c], ref[c]), (mode, t, c)
+            else:
+                assert (out[c] == 0xFFFFFFFF).all(), (mode, t, c)
+
+
+def test_batch_rejects_rows_beyond_high_rate_kernel(dec):
+    """A launch whose max_parity_nodes (<= 8) selects the high-code-rate
+    kernel rejects an item with more parity rows instead of decoding part of
+    them; its other items decode as the oracle."""
+    from aerial_amd import _lib as L
+    from aerial_amd.batch import ITEM_DTYPE, _half_bits, group_values
+    print("[case] batch: mb 10 item in an mb <= 8 launch", flush=True)
+    tbs, refs = make_tbs("fp16", seed=62, mix=[(1, 96, 6, 2), (2, 64, 10, 1)])
+    h = _half_bits(0.8125)
+    arr = np.zeros(3, ITEM_DTYPE)
+    for i, (t, c) in enumerate([(0, 0), (0, 1), (1, 0)]):
+        tb = tbs[t]
+        arr["llr"][i] = tb.llr.data_ptr() + c * tb.llr.stride(0) * 2
+        arr["out"][i] = tb.out.data_ptr() + c * tb.out.stride(0) * 4
+        arr["BG"][i], arr["Z"][i], arr["num_parity_nodes"][i], arr["Kb"][i] = tb.bg, tb.Z, tb.mb, 22 if tb.bg == 1 else 10
+        arr["max_iterations"][i], arr["norm"][i], arr["clamp_value"][i] = 7, h | (h << 16), 32.0
+    for grp in (None, [0, 1, 2, 3]):
+        for tb in tbs:
+            tb.out.fill_(-1)
+        got = _launch_items(dec, arr, grp, max(group_values(tb.bg, tb.Z, tb.mb, 1) for tb in tbs), 8, 384,
+                            L.CUPHY_R_16F)
+        assert list(got["status"]) == [0, 0, L.ITEM_BAD_MB]
+        assert np.array_equal(tbs[0].out.cpu().numpy().view(np.uint32), refs[0])
+        assert (tbs[1].out.cpu().numpy().view(np.uint32) == 0xFFFFFFFF).all()

@@ -51,14 +51,15 @@ def qam_stride_layout(llr_demap, Qm, Nl, n_bbu, layer_map, fill=123.0):
 DESCR_CASES = RM_CASES + [(1, 384, 50, 8448, 8, 1, 50 * 12000, 1)]
 
 
-@pytest.mark.parametrize("descr", [False, True])
+# (case, descrambling): every RM case without and with descrambling, the long TB with it only
+RM_RUNS = [(c, d) for d in (False, True) for c in range(len(DESCR_CASES)) if d or c < len(RM_CASES)]
+
+
 @pytest.mark.parametrize("fp", [0, 1, 2, 3])
-@pytest.mark.parametrize("case", range(len(DESCR_CASES)))
+@pytest.mark.parametrize("case,descr", RM_RUNS)
 def test_derate_match_two_transmissions(case, fp, descr):
     _need_gpu()
     from aerial_amd.pusch import DeRateMatch, make_tb_params
-    if case >= len(RM_CASES) and not descr:
-        pytest.skip("the long TB is a descrambling case")
     bg, Zc, C, Kp, Qm, Nl, G, rv = DESCR_CASES[case]
     in16, out16 = fp in (1, 3), fp in (2, 3)
     cinit = (0x1234567 * (case + 1) + 0x9E37) & 0x7FFFFFFF if descr else 0
@@ -69,7 +70,7 @@ def test_derate_match_two_transmissions(case, fp, descr):
     rm = DeRateMatch(fp_config=fp, descrambling=descr)
     p1, _ = make_tb_params(bg, Zc, C, Kp, Qm, Nl, G, rv, ndi=1, n_bbu_layers=n_bbu, layer_map=lmap)
     ocfg = O.rm_cfg(bg=bg, Zc=Zc, C=C, Qm=Qm, Nl=Nl, G=G, K=p1.K, F=p1.F, Ncb=p1.Ncb,
-                    Ncb_padded=p1.Ncb_padded, k0=p1.k0, ndi=1, descramble=int(descr), cinit=cinit)
+                    Ncb_padded=p1.Ncb_padded, k0=0, ndi=1, descramble=int(descr), cinit=cinit)
     n_cmp = 2 * Zc + p1.Ncb
     ref = np.zeros(C * p1.Ncb_padded, np.float32)
     out_t = torch.float16 if out16 else torch.float32
@@ -80,7 +81,8 @@ def test_derate_match_two_transmissions(case, fp, descr):
         llr = tb_llrs(G, 1000 * case + tx)
         if in16:
             llr = llr.astype(np.float16).astype(np.float32)
-        ocfg.ndi, ocfg.k0 = ndi, p.k0
+        ocfg.ndi, ocfg.k0 = ndi, O.k0(bg, rv_, p.Ncb, Zc)
+        assert p.k0 == ocfg.k0
         O.derate_match(ocfg, llr, ref, fp16=out16)
         buf = qam_stride_layout(llr, Qm, Nl, n_bbu, lmap)
         x = torch.from_numpy(buf).cuda()
@@ -123,6 +125,93 @@ def test_derate_match_multi_ue_first_cb_offset():
         n = 2 * p.Zc + p.Ncb
         got = o.float().cpu().numpy().reshape(C + first, -1)[first:, :n]
         assert np.array_equal(got, ref.reshape(C + first, -1)[first:, :n])
+    rm.close()
+
+
+def _modulo_cases():
+    """Distinct rows of the reference's de-rate-match table
+    (test_derate_match_modulo.cu:122-228: E, K - 2Zc, F, k0, Ncb, Zc of its
+    PUSCH test vectors and two multi-loop cases; fixture in tests/golden)."""
+    import json
+    import os
+    rows = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                       "derate_modulo_cases.json")))
+    seen, out = set(), []
+    for r in rows:
+        key = (r["E"], r["K"], r["F"], r["k0"], r["Ncb"], r["Zc"])
+        if key not in seen:
+            seen.add(key)
+            out.append(r)
+    return out
+
+
+def test_derate_match_reference_cases():
+    """Every distinct (E, K, F, k0, Ncb, Zc) of the reference's bit-selection
+    table as a one-CB TB (incl. the limited-buffer Ncb = 17915 of TVnr_7321,
+    k0 inside the filler hole and E up to 5 Ncb), two transmissions
+    (overwrite, then combine at the same k0): the binary32 HARQ buffer equals
+    the oracle's, whose bit selection the CPU tests pin to the reference's
+    textbook selection on the same rows."""
+    _need_gpu()
+    from aerial_amd.pusch import DeRateMatch, make_tb_params
+    cases = _modulo_cases()
+    print("[case] derate: %d reference (E, K, F, k0, Ncb, Zc) rows, fp32" % len(cases), flush=True)
+    rm = DeRateMatch(fp_config=0)
+    for i, r in enumerate(cases):
+        Zc, F, E = r["Zc"], r["F"], r["E"]
+        K = r["K"] + 2 * Zc
+        bg = 1 if K == 22 * Zc else 2
+        Qm = next(q for q in (8, 6, 4, 2, 1) if E % q == 0)
+        N = (66 if bg == 1 else 50) * Zc
+        p, _ = make_tb_params(bg, Zc, 1, K - F, Qm, 1, E, 0, ndi=1, Nref=r["Ncb"] if r["Ncb"] < N else 0)
+        assert p.Ncb == r["Ncb"] and p.K == K and p.F == F
+        p.k0 = r["k0"]
+        ocfg = O.rm_cfg(bg=bg, Zc=Zc, C=1, Qm=Qm, Nl=1, G=E, K=K, F=F, Ncb=r["Ncb"], Ncb_padded=p.Ncb_padded,
+                        k0=r["k0"], ndi=1)
+        ref = np.zeros(p.Ncb_padded, np.float32)
+        harq = torch.full((p.Ncb_padded,), 9.0, dtype=torch.float32, device="cuda")
+        for tx in range(2):
+            llr = tb_llrs(E, 7000 + 10 * i + tx)
+            p.ndi = ocfg.ndi = 1 - tx
+            O.derate_match(ocfg, llr, ref)
+            rm.run([p], [torch.from_numpy(qam_stride_layout(llr, Qm, 1, 1, [0])).cuda()], [harq])
+            torch.cuda.synchronize()
+            n = 2 * Zc + r["Ncb"]
+            got = harq.cpu().numpy()[:n]
+            assert np.array_equal(got.view(np.uint32), ref[:n].view(np.uint32)), (r, tx)
+    rm.close()
+
+
+def test_derate_match_uci_on_pusch():
+    """uciOnPuschFlag: the UL-SCH LLRs are G values in transmission order
+    (no QAM_STRIDE layout) and the E split follows G, with encodedSize (all
+    bits of the allocation) larger than G; two transmissions against the
+    oracle."""
+    _need_gpu()
+    from aerial_amd.pusch import DeRateMatch, make_tb_params
+    print("[case] derate uci-on-pusch BG1 Zc=208 C=3 Qm=4 Nl=2", flush=True)
+    bg, Zc, C, Kp, Qm, Nl = 1, 208, 3, 4400, 4, 2
+    G = 8 * 1879  # 1879 symbol pairs over 3 CBs: E = 5008, 5008, 5016
+    rm = DeRateMatch(fp_config=0)
+    ref = None
+    for tx, (ndi, rv) in enumerate([(1, 0), (0, 2)]):
+        p, _ = make_tb_params(bg, Zc, C, Kp, Qm, Nl, G, rv, ndi=ndi)
+        p.uciOnPuschFlag = 1
+        p.encodedSize = G + 8 * 137  # UCI bits of the allocation beyond the UL-SCH's G
+        if ref is None:
+            ref = np.zeros(C * p.Ncb_padded, np.float32)
+            harq = torch.full((C * p.Ncb_padded,), 5.0, dtype=torch.float32, device="cuda")
+        ocfg = O.rm_cfg(bg=bg, Zc=Zc, C=C, Qm=Qm, Nl=Nl, G=G, K=p.K, F=p.F, Ncb=p.Ncb, Ncb_padded=p.Ncb_padded,
+                        k0=O.k0(bg, rv, p.Ncb, Zc), ndi=ndi)
+        llr = tb_llrs(G, 4100 + tx)
+        O.derate_match(ocfg, llr, ref)
+        x = torch.from_numpy(llr).cuda()
+        p.d_schAndCsi2LLRs = x.data_ptr()  # the UL-SCH LLRs of a UCI-on-PUSCH TB come from PerTbParams
+        rm.run([p], [x], [harq])
+        torch.cuda.synchronize()
+        n = 2 * Zc + p.Ncb
+        got = harq.cpu().numpy().reshape(C, -1)[:, :n]
+        assert np.array_equal(got.view(np.uint32), ref.reshape(C, -1)[:, :n].view(np.uint32)), tx
     rm.close()
 
 

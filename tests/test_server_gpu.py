@@ -1,10 +1,16 @@
 """HARQ serving engine: continuous batching with device-resident HARQ
 buffers.  Each TB's results in a mix of overlapping arrivals (concurrent
 launches, retransmissions combined across launches) are bit-identical to
-decoding that TB alone, transmission by transmission, with the fused slot
-decoder; admission by HBM budget holds the pool to its size."""
+the oracle's chain over the same transmissions (binary16 de-rate-matching
+with HARQ combining at rv 0, 2, 3, 1 -> layered decoding with syndrome early
+termination -> CB/TB CRC) and to decoding that TB alone, transmission by
+transmission, with the fused slot decoder; admission by HBM budget holds the
+pool to its size."""
 import numpy as np
 import pytest
+
+import oracle as O
+import pusch_vectors as P
 
 pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
@@ -65,6 +71,60 @@ def check_against_alone(dec, templates, done, meta, outs, arrivals, max_iteratio
         assert done[last].tb_crc == int(ref[3].cpu().numpy().view(np.uint32)[0]), (a, ti)
 
 
+def oracle_chain(tb, n_tx, max_iterations):
+    """The oracle over a TB's first n_tx transmissions: binary16 HARQ
+    combining (derate_match.m) of each transmission's LLRs at rv 0, 2, 3, 1,
+    then every CB decoded in binary16 with syndrome early termination at the
+    parity nodes covering what was received so far and the reference's
+    normalisation for them, then CB/TB CRC and TB bytes.  k0 and the parity
+    nodes come from the oracle.  Returns (words (C, 264), iterations,
+    cb_crc, tb_bytes, tb_crc) of the last transmission."""
+    from aerial_amd.server import RV_SEQUENCE
+    bg, Zc, C, Kp, Qm, Nl = tb.bg, tb.Zc, tb.C, tb.Kp, tb.Qm, tb.Nl
+    K = P.nsys(bg) * Zc
+    N = (66 if bg == 1 else 50) * Zc
+    npad = (N + 2 * Zc + 7) // 8 * 8
+    harq = np.zeros(C * npad, np.float32)
+    mb = 4
+    for k in range(n_tx):
+        rv = RV_SEQUENCE[k]
+        k0 = O.k0(bg, rv, N, Zc)
+        m = O.num_parity_nodes(bg, Zc, K, K - Kp, tb.G, C, k0)
+        mb = m if k == 0 else max(mb, m)
+        cfg = O.rm_cfg(bg=bg, Zc=Zc, C=C, Qm=Qm, Nl=Nl, G=tb.G, K=K, F=K - Kp, Ncb=N, Ncb_padded=npad, k0=k0,
+                       ndi=int(k == 0))
+        llr = tb.llrs[rv].view(-1, P.QAM_STRIDE)[:, :Qm].reshape(-1).float().cpu().numpy()
+        O.derate_match(cfg, llr, harq, fp16=True)
+    dcfg = O.dec_cfg(bg, Zc, mb, n_iter=max_iterations, alpha=P.table_alpha(bg, mb), clamp=32.0, mode=O.FP16,
+                     early_term=True)
+    h = harq.reshape(C, npad)[:, :(P.nsys(bg) + mb) * Zc]
+    words, iters = O.decode_cbs_it(dcfg, h, nthreads=8, hard_stride=O.WORDS_PER_CB)
+    cb_crc, tb_bytes, tb_crc = O.crc_desegment(words.reshape(-1), C, Kp)
+    return words, iters, cb_crc, tb_bytes, tb_crc
+
+
+def check_against_oracle(templates, done, meta, outs, arrivals, max_iterations):
+    """Every TB's final outputs (decoded words, iterations per CB, CB CRCs,
+    TB bytes, the completion's TB CRC) equal the oracle's chain over the
+    transmissions the TB received."""
+    from aerial_amd import _lib as L
+    n_tx = {}
+    for i, (a, k, ti) in enumerate(meta):
+        if done[i].status == L.CUPHY_STATUS_SUCCESS:
+            n_tx[a] = max(n_tx.get(a, 0), k + 1)
+    assert len(n_tx) == len(arrivals)
+    for a, (_, _, ti) in enumerate(arrivals):
+        tb = templates.tbs[ti]
+        words, iters, cb_crc, tb_bytes, tb_crc = oracle_chain(tb, n_tx[a], max_iterations)
+        got = outs[a]
+        nw = (tb.Kp + 31) // 32
+        assert np.array_equal(got[0].cpu().numpy().view(np.uint32)[:, :nw], words[:, :nw]), (a, ti)
+        assert np.array_equal(got[4].cpu().numpy().astype(np.int64), iters.astype(np.int64)), (a, ti)
+        assert np.array_equal(got[1].cpu().numpy().view(np.uint32), cb_crc), (a, ti)
+        assert np.array_equal(got[2].cpu().numpy()[:len(tb_bytes)], tb_bytes), (a, ti)
+        assert done[4 * a + n_tx[a] - 1].tb_crc == tb_crc, (a, ti)
+
+
 @pytest.mark.parametrize("resident", [False, True])
 def test_results_equal_alone_and_in_the_mix(dec, templates, resident):
     from aerial_amd.server import HarqServer, build_trace, poisson_arrivals
@@ -79,6 +139,7 @@ def test_results_equal_alone_and_in_the_mix(dec, templates, resident):
     st = srv.stats()
     assert st["pending"] == 0 and st["pool_bytes_in_use"] == 0, st
     assert st["launches"] >= (1 if resident else 2)
+    check_against_oracle(templates, done, meta, outs, arrivals, 8)
     check_against_alone(dec, templates, done, meta, outs, arrivals, 8)
     # some TBs needed retransmissions, and overlapping launches happened
     ks = [k for i, (a, k, ti) in enumerate(meta) if done[i].status == 0]
@@ -139,6 +200,7 @@ def test_admission_by_hbm_budget(dec, templates, resident):
     assert st["pool_bytes_peak"] <= pool, st
     assert st["admission_waits"] > 0, st
     assert st["pool_bytes_in_use"] == 0
+    check_against_oracle(templates, done, meta, outs, arrivals, 8)
     check_against_alone(dec, templates, done, meta, outs, arrivals, 8)
     srv.close()
 

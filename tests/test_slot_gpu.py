@@ -64,7 +64,11 @@ class Case:
         from aerial_amd.pusch import make_tb_params
         bg, Zc, C, Kp, Qm, Nl, nbbu, E, snr = self.spec
         p, mb = make_tb_params(bg, Zc, C, Kp, Qm, Nl, self.G, rv, ndi=ndi, n_bbu_layers=nbbu, layer_map=self.lmap)
-        self.mb = mb if ndi else max(self.mb, mb)
+        # k0 and parity nodes as the oracle derives them (derate_match.m:86-129); the host helper must agree
+        k0 = O.k0(bg, rv, p.Ncb, Zc)
+        mb_o = O.num_parity_nodes(bg, Zc, p.K, p.F, self.G, C, k0)
+        assert (p.k0, mb) == (k0, mb_o), (self.spec, rv)
+        self.mb = mb_o if ndi else max(self.mb, mb_o)
         llr = P.bpsk_llrs(self.tb.tx_bits(self.G, Qm, Nl, rv), snr, seed=9000 + 10 * self.k + tx)
         if self.descramble:  # the channel carries the scrambled bits b(n) ^ c(n)
             llr = np.where(O.gold(self.cinit, 0, self.G) != 0, -llr, llr).astype(np.float32)
@@ -83,7 +87,7 @@ class Case:
                 seen = llr.astype(np.float16).astype(np.float32)
             scale = 1.0
         ocfg = O.rm_cfg(bg=bg, Zc=Zc, C=C, Qm=Qm, Nl=Nl, G=self.G, K=p.K, F=p.F, Ncb=p.Ncb,
-                        Ncb_padded=p.Ncb_padded, k0=p.k0, ndi=ndi, descramble=int(self.descramble),
+                        Ncb_padded=p.Ncb_padded, k0=k0, ndi=ndi, descramble=int(self.descramble),
                         cinit=self.cinit)
         O.derate_match(ocfg, seen, self.ref_harq, fp16=True)
         p.cinit = self.cinit
@@ -124,7 +128,8 @@ def run_slot(dec, dtype, early_term, specs=SLOT, descramble=False):
         assert all(int(w.abs().sum().item()) == 0 for w in sd.workspaces), "TB words not returned to zero"
         for c, tb in zip(cases, tbs):
             bg, Zc, C, Kp = c.spec[:4]
-            alpha = half_to_float(sd._alpha_bits(tb))
+            alpha = P.table_alpha(bg, c.mb)  # the reference's table, as SetNormalization must give it
+            assert half_to_float(sd._alpha_bits(tb)) == alpha
             words, iters, cb_crc, tb_bytes, tb_crc = c.oracle_decode(alpha, early_term)
             n = 2 * Zc + c.p.Ncb
             got_h = c.harq.float().cpu().numpy().reshape(C, -1)[:, :n]
@@ -145,8 +150,12 @@ def test_slot_matches_oracle(dec, dtype):
     assert len(cases) == len(SLOT)
 
 
-def test_slot_early_termination_matches_oracle(dec):
-    run_slot(dec, "fp16", early_term=True)
+@pytest.mark.parametrize("dtype", ["fp16", "int8"])
+def test_slot_early_termination_matches_oracle(dec, dtype):
+    """Syndrome early termination with identical stopping: iterations per CB,
+    bits, CRCs and HARQ buffers equal the oracle's, for binary16 and for the
+    int8 LLRs of the fused receive-side slot."""
+    run_slot(dec, dtype, early_term=True)
 
 
 @pytest.mark.parametrize("dtype", ["fp16", "int8", "fp32"])
