@@ -215,6 +215,7 @@ SIGNATURES = {
     "cuphyHipLDPCBatchDecodeGroups": ([_VP, _VP, _VP, _I, _I, _I, _I, _I, _I, _VP, _VP], _I),
     "cuphyHipPuschSlotDecodeGetWorkspaceSize": ([_I, _P(C.c_size_t)], _I),
     "cuphyHipPuschSlotDecode": ([_VP, _VP, _I, _VP, _I, _I, _I, _I, _I, _VP, _VP], _I),
+    "cuphyHipPuschSlotDecodeNoAlloc": ([_VP, _VP, _I, _VP, _I, _I, _I, _I, _I, _I, _VP], _I),
     "cuphyHipHarqServerCreate": ([_VP, _P(HarqServerConfig), _P(_VP)], _I),
     "cuphyHipHarqServerDestroy": ([_VP], _I),
     "cuphyHipHarqServerSubmit": ([_VP, _P(HarqRequest)], _I),

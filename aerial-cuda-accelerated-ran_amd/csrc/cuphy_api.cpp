@@ -290,6 +290,9 @@ cuphyStatus_t CUPHYWINAPI cuphyCreateLDPCDecoder(cuphyContext_t context, cuphyLD
     if (!d) return CUPHY_STATUS_ALLOC_FAILED;
     d->device = context->device;
     d->cu_count = context->cu_count;
+    // set-up of the allocation-free slot entry point on this decoder's GPU; a
+    // failure here is retried by that entry point's first call
+    (void)nrldpc::noalloc_scratch_init(d->device);
     *pdecoder = d;
     return CUPHY_STATUS_SUCCESS;
 }
@@ -630,6 +633,16 @@ cuphyStatus_t CUPHYWINAPI cuphyHipPuschSlotDecode(cuphyLDPCDecoder_t decoder, co
     if (!decoder) return CUPHY_STATUS_INVALID_ARGUMENT;
     return nrldpc::slot_decode(d_tbs, num_tbs, d_cb_items, num_cb_items, llr_type, max_lds_values, max_parity_nodes,
                                max_cbs_per_item, d_workspace, decoder->cu_count, strm);
+}
+
+cuphyStatus_t CUPHYWINAPI cuphyHipPuschSlotDecodeNoAlloc(cuphyLDPCDecoder_t decoder, const cuphyHipPuschSlotTb_t* d_tbs,
+                                                         int num_tbs, const uint32_t* d_cb_items, int num_cb_items,
+                                                         int num_cbs, cuphyDataType_t llr_type, int max_lds_values,
+                                                         int max_parity_nodes, int max_cbs_per_item, hipStream_t strm)
+{
+    if (!decoder) return CUPHY_STATUS_INVALID_ARGUMENT;
+    return nrldpc::slot_decode_noalloc(d_tbs, num_tbs, d_cb_items, num_cb_items, num_cbs, llr_type, max_lds_values,
+                                       max_parity_nodes, max_cbs_per_item, decoder->cu_count, strm);
 }
 
 }  // extern "C"

@@ -40,6 +40,7 @@
 #include "pusch_fused.hpp"
 
 #include <algorithm>
+#include <mutex>
 
 namespace nrldpc {
 
@@ -646,6 +647,19 @@ __global__ __launch_bounds__(384) void pusch_serve_resident(const serve_ring_hdr
     }
 }
 
+// Scratch of cuphyHipPuschSlotDecodeNoAlloc: the queue state and two words
+// per CB of the launch (the workspace of cuphyHipPuschSlotDecode), for
+// callers that cannot allocate device memory.  Each GPU the library runs
+// on holds one copy until the process ends.
+static __device__ __align__(256) uint32_t scratch[FUSED_WS_QUEUE_WORDS + 2 * NOALLOC_MAX_CBS];
+
+// Zero n words (a kernel rather than a memset, so that a caller may capture
+// the call in a graph).
+__global__ __launch_bounds__(256) void zero_words(uint32_t* __restrict__ p, int n)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
+}
+
 template <bool GROUPED>
 void* fused_kernel_of(cuphyDataType_t llr_type, bool small)
 {
@@ -711,6 +725,56 @@ cuphyStatus_t serve_resident_launch(const serve_ring_hdr* ring, const uint32_t* 
     if (hipLaunchKernel(func, dim3((unsigned)grid), dim3(FUSED_BLOCK), args, lds, strm) != hipSuccess)
         return CUPHY_STATUS_INTERNAL_ERROR;
     return CUPHY_STATUS_SUCCESS;
+}
+
+namespace {
+constexpr int MAX_SCRATCH_DEVICES = 64;
+std::mutex g_scratch_mutex;
+uint32_t* g_scratch_ptr[MAX_SCRATCH_DEVICES] = {};  // device address of `scratch` on each GPU
+}  // namespace
+
+cuphyStatus_t noalloc_scratch_init(int device)
+{
+    if (device < 0 || device >= MAX_SCRATCH_DEVICES) return CUPHY_STATUS_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[device]) return CUPHY_STATUS_SUCCESS;
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess) return CUPHY_STATUS_INTERNAL_ERROR;
+    if (cur != device && hipSetDevice(device) != hipSuccess) return CUPHY_STATUS_INTERNAL_ERROR;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (cur != device) (void)hipSetDevice(cur);  // restoring the caller's device; the lookup's result decides
+    if (e != hipSuccess || !ptr) return CUPHY_STATUS_INTERNAL_ERROR;
+    g_scratch_ptr[device] = static_cast<uint32_t*>(ptr);
+    return CUPHY_STATUS_SUCCESS;
+}
+
+cuphyStatus_t slot_decode_noalloc(const cuphyHipPuschSlotTb_t* d_tbs, int n_tbs, const uint32_t* d_items, int n_items,
+                                  int n_cbs, cuphyDataType_t llr_type, int max_lds_values, int max_parity_nodes,
+                                  int max_cbs_per_item, int cu_count, hipStream_t strm)
+{
+    if (n_cbs < 0 || n_cbs > NOALLOC_MAX_CBS) return CUPHY_STATUS_INVALID_ARGUMENT;
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_SCRATCH_DEVICES) return CUPHY_STATUS_INTERNAL_ERROR;
+    uint32_t* ws = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mutex);
+        ws = g_scratch_ptr[dev];
+    }
+    if (!ws) {  // first call on a GPU the decoder was not created on
+        const cuphyStatus_t s = noalloc_scratch_init(dev);
+        if (s != CUPHY_STATUS_SUCCESS) return s;
+        std::lock_guard<std::mutex> lock(g_scratch_mutex);
+        ws = g_scratch_ptr[dev];
+    }
+    // the queue state and the words of every TB start at zero
+    const int words = FUSED_WS_QUEUE_WORDS + 2 * n_cbs;
+    void* zargs[2] = {&ws, const_cast<int*>(&words)};
+    if (hipLaunchKernel((void*)&zero_words, dim3((unsigned)((words + 255) / 256)), dim3(256), zargs, 0, strm) !=
+        hipSuccess)
+        return CUPHY_STATUS_INTERNAL_ERROR;
+    return slot_decode(d_tbs, n_tbs, d_items, n_items, llr_type, max_lds_values, max_parity_nodes, max_cbs_per_item,
+                       ws, cu_count, strm);
 }
 
 cuphyStatus_t slot_decode(const cuphyHipPuschSlotTb_t* d_tbs, int n_tbs, const uint32_t* d_items, int n_items,

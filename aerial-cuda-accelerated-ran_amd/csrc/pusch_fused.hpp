@@ -13,6 +13,9 @@ namespace nrldpc {
 // are used) followed by two words per CB slot, of which each TB uses the
 // pair at its first slot: TB-CRC accumulator and count of finished CBs.
 constexpr int FUSED_WS_QUEUE_WORDS = 64;
+// Largest num_cbs of cuphyHipPuschSlotDecodeNoAlloc (its scratch array
+// holds FUSED_WS_QUEUE_WORDS + 2 * NOALLOC_MAX_CBS words, 128.25 KB).
+constexpr int NOALLOC_MAX_CBS = 16384;
 
 // Work items of the fused kernels: (TB << 16) | ((count - 1) << 12) | CB,
 // the group of `count` consecutive CBs CB .. CB + count - 1 of the TB,
@@ -93,6 +96,17 @@ cuphyStatus_t serve_resident_launch(const serve_ring_hdr* ring, const uint32_t* 
                                     const cuphyHipPuschSlotTb_t* jobs, int n_slots, serve_state* st, uint32_t* tbw,
                                     uint32_t* abandoned, int grid, cuphyDataType_t llr_type, uint64_t idle_ticks,
                                     uint64_t life_ticks, hipStream_t strm);
+
+// Look up the device address of the allocation-free entry point's scratch
+// array on `device` (once per device; the device is made current for the
+// lookup).
+cuphyStatus_t noalloc_scratch_init(int device);
+
+// slot_decode with the library's scratch array of the current device as the
+// workspace, cleared first (n_cbs words pairs plus the queue state).
+cuphyStatus_t slot_decode_noalloc(const cuphyHipPuschSlotTb_t* d_tbs, int n_tbs, const uint32_t* d_items, int n_items,
+                                  int n_cbs, cuphyDataType_t llr_type, int max_lds_values, int max_parity_nodes,
+                                  int max_cbs_per_item, int cu_count, hipStream_t strm);
 
 // Enqueue one persistent fused launch (de-rate-match + HARQ combine +
 // decode + CB/TB CRC) over the CB items of a slot.

@@ -508,6 +508,22 @@ cuphyStatus_t CUPHYWINAPI cuphyHipPuschSlotDecode(cuphyLDPCDecoder_t decoder, co
                                                   cuphyDataType_t llr_type, int max_lds_values, int max_parity_nodes,
                                                   int max_cbs_per_item, void* d_workspace, hipStream_t strm);
 
+/* Allocation-free form of cuphyHipPuschSlotDecode.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * queue state and TB-CRC words live in a scratch array of the library of
+ * 64 + 2 * 16384 words on each GPU (held until the process ends), cleared by
+ * every call; num_cbs is what cuphyHipPuschSlotDecodeGetWorkspaceSize would
+ * be given (share_offset + num_CBs of every TB <= num_cbs), at most 16384,
+ * else CUPHY_STATUS_INVALID_ARGUMENT.  Calls on one GPU share the array:
+ * calls that overlap in time on one GPU race (serialize them on one stream).
+ * No hipMalloc, hipFree or allocating library call.  Other arguments and
+ * results as cuphyHipPuschSlotDecode. */
+cuphyStatus_t CUPHYWINAPI cuphyHipPuschSlotDecodeNoAlloc(cuphyLDPCDecoder_t decoder, const cuphyHipPuschSlotTb_t* d_tbs,
+                                                         int num_tbs, const uint32_t* d_cb_items, int num_cb_items,
+                                                         int num_cbs, cuphyDataType_t llr_type, int max_lds_values,
+                                                         int max_parity_nodes, int max_cbs_per_item, hipStream_t strm);
+
 /* ------------------------------------------------------------------------
  * HARQ serving engine (no cuphy.h counterpart: cuPHY is driven slot by slot
  * by its caller).  Transmissions of transport blocks are submitted as they
