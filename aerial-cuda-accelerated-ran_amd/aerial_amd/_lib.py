@@ -133,11 +133,12 @@ class PuschSlotTb(C.Structure):
 class HarqServerConfig(C.Structure):
     _fields_ = [("harq_pool_bytes", C.c_size_t), ("max_inflight", C.c_int), ("max_tbs_per_launch", C.c_int),
                 ("max_cbs_per_launch", C.c_int), ("llr_type", C.c_int), ("flags", C.c_uint32),
-                ("harq_timeout_us", C.c_double)]
+                ("harq_timeout_us", C.c_double), ("resident_idle_us", C.c_double)]
 
 
 class HarqRequest(C.Structure):
-    _fields_ = [("harq_id", C.c_uint64), ("tag", C.c_uint64), ("prm", PerTbParams), ("llr", C.c_void_p),
+    _fields_ = [("cell_id", C.c_uint32), ("rnti", C.c_uint16), ("harq_pid", C.c_uint8), ("reserved0", C.c_uint8),
+                ("tag", C.c_uint64), ("prm", PerTbParams), ("llr", C.c_void_p),
                 ("cb_bits", C.c_void_p), ("cb_crc", C.c_void_p), ("tb_bytes", C.c_void_p),
                 ("cb_iterations", C.c_void_p), ("num_parity_nodes", C.c_int16), ("max_iterations", C.c_int16),
                 ("flags", C.c_uint16), ("reserved", C.c_uint16), ("llr_scale", C.c_float), ("norm", Norm),
@@ -145,14 +146,16 @@ class HarqRequest(C.Structure):
 
 
 class HarqCompletion(C.Structure):
-    _fields_ = [("tag", C.c_uint64), ("harq_id", C.c_uint64), ("tb_crc", C.c_uint32), ("status", C.c_int32),
+    _fields_ = [("tag", C.c_uint64), ("cell_id", C.c_uint32), ("rnti", C.c_uint16), ("harq_pid", C.c_uint8),
+                ("reserved0", C.c_uint8), ("tb_crc", C.c_uint32), ("status", C.c_int32),
                 ("t_submit_us", C.c_double), ("t_launch_us", C.c_double), ("t_done_us", C.c_double)]
 
 
 class HarqServerStats(C.Structure):
     _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("launches", C.c_uint64),
                 ("pool_bytes_in_use", C.c_uint64), ("pool_bytes_peak", C.c_uint64), ("pending", C.c_uint64),
-                ("admission_waits", C.c_uint64), ("evictions", C.c_uint64)]
+                ("admission_waits", C.c_uint64), ("evictions", C.c_uint64), ("pool_bytes_cached", C.c_uint64),
+                ("pool_reclaims", C.c_uint64)]
 
 
 class HarqTraceEntry(C.Structure):
@@ -220,7 +223,7 @@ SIGNATURES = {
     "cuphyHipHarqServerDestroy": ([_VP], _I),
     "cuphyHipHarqServerSubmit": ([_VP, _P(HarqRequest)], _I),
     "cuphyHipHarqServerPoll": ([_VP, _VP, _I, _P(_I)], _I),
-    "cuphyHipHarqServerRelease": ([_VP, C.c_uint64], _I),
+    "cuphyHipHarqServerRelease": ([_VP, C.c_uint32, C.c_uint16, C.c_uint8], _I),
     "cuphyHipHarqServerGetStats": ([_VP, _P(HarqServerStats)], _I),
     "cuphyHipHarqServeTrace": ([_VP, _VP, _I, C.c_double, _VP, _P(_I)], _I),
     "cuphyHipLDPCWorkItemCheck": ([_P(LDPCWorkItem), _I, _P(C.c_uint32)], _I),

@@ -32,22 +32,34 @@ def norm_f16(decoder, bg, Z, mb):
     return cfg.norm.f16x2 & 0xFFFF
 
 
+def harq_key(process):
+    """(cell, rnti, HARQ process id) of a process given as that tuple or as
+    an int n (process id n mod 16, RNTI n / 16 mod 65536, cell n / 2^20)."""
+    if isinstance(process, tuple):
+        cell, rnti, pid = process
+        return int(cell), int(rnti), int(pid)
+    n = int(process)
+    return n >> 20, (n >> 4) & 0xFFFF, n & 0xF
+
+
 class HarqServer:
     """Owner of a cuphyHipHarqServer_t."""
 
     def __init__(self, decoder, pool_bytes=2 << 30, max_inflight=3, max_tbs=1024, max_cbs=8192,
-                 llr_type=L.CUPHY_R_16F, resident=False, harq_timeout_us=0.0):
+                 llr_type=L.CUPHY_R_16F, resident=False, harq_timeout_us=0.0, resident_idle_us=0.0):
         """resident: one resident launch takes code blocks as they are
         admitted (max_tbs / max_cbs then bound the TBs / code blocks in
         flight); otherwise admitted transmissions are batched into up to
         max_inflight launches.  harq_timeout_us > 0: a HARQ buffer not used
-        for that long is freed (a later retransmission of it is rejected)."""
+        for that long is freed (a later retransmission of it is rejected).
+        resident_idle_us > 0: the resident launch leaves that long after its
+        last work (default 20 ms)."""
         self.dec = decoder
         self._lib = L.lib()
         cfg = L.HarqServerConfig(harq_pool_bytes=pool_bytes, max_inflight=max_inflight, max_tbs_per_launch=max_tbs,
                                  max_cbs_per_launch=max_cbs, llr_type=llr_type,
                                  flags=L.CUPHY_HIP_HARQ_SERVER_RESIDENT if resident else 0,
-                                 harq_timeout_us=harq_timeout_us)
+                                 harq_timeout_us=harq_timeout_us, resident_idle_us=resident_idle_us)
         self.h = C.c_void_p()
         L.check("cuphyHipHarqServerCreate", self._lib.cuphyHipHarqServerCreate(decoder.dec, C.byref(cfg),
                                                                                C.byref(self.h)))
@@ -64,14 +76,16 @@ class HarqServer:
         except Exception:
             pass
 
-    def request(self, harq_id, params, llr, outputs, mb, max_iterations=10, early_term=True, llr_scale=1.0,
+    def request(self, process, params, llr, outputs, mb, max_iterations=10, early_term=True, llr_scale=1.0,
                 clamp_value=32.0, last=False, tag=0, descramble=False):
-        """A cuphyHipHarqRequest_t; outputs = (cb_bits, cb_crc, tb_bytes,
-        tb_crc, cb_iterations) device tensors as SlotTb.alloc_outputs makes
-        them (the engine reports the TB CRC in the completion; tb_crc and
-        cb_iterations may be None); descramble: with params.cinit."""
+        """A cuphyHipHarqRequest_t; process: the HARQ process, (cell, rnti,
+        process id) or an int (harq_key); outputs = (cb_bits, cb_crc,
+        tb_bytes, tb_crc, cb_iterations) device tensors as
+        SlotTb.alloc_outputs makes them (the engine reports the TB CRC in the
+        completion; tb_crc and cb_iterations may be None); descramble: with
+        params.cinit."""
         r = L.HarqRequest()
-        r.harq_id = harq_id
+        r.cell_id, r.rnti, r.harq_pid = harq_key(process)
         r.tag = tag
         r.prm = params
         r.llr = llr.data_ptr()
@@ -103,8 +117,8 @@ class HarqServer:
         L.check("cuphyHipHarqServerPoll", self._lib.cuphyHipHarqServerPoll(self.h, out, max_out, C.byref(n)))
         return [out[i] for i in range(n.value)]
 
-    def release(self, harq_id):
-        L.check("cuphyHipHarqServerRelease", self._lib.cuphyHipHarqServerRelease(self.h, harq_id))
+    def release(self, process):
+        L.check("cuphyHipHarqServerRelease", self._lib.cuphyHipHarqServerRelease(self.h, *harq_key(process)))
 
     def stats(self):
         s = L.HarqServerStats()
@@ -200,8 +214,9 @@ def build_trace(server, templates, arrivals, max_iterations=10, outputs=None):
             i = 4 * a + k
             p, _ = tb.params(rv, 1 if k == 0 else 0)
             e = entries[i]
-            e.req = server.request(a, p, tb.llrs[rv], o, templates.mbs[ti][k], max_iterations=max_iterations,
-                                   early_term=True, last=(k == 3), tag=i)
+            # the arrival's HARQ process: its cell, an RNTI and process id unique within the trace
+            e.req = server.request((cell, 1 + (a >> 4) % 65519, a & 15), p, tb.llrs[rv], o, templates.mbs[ti][k],
+                                   max_iterations=max_iterations, early_term=True, last=(k == 3), tag=i)
             e.arrival_us = t if k == 0 else -1.0
             e.next = i + 1 if k < 3 else -1
             meta.append((a, k, ti))

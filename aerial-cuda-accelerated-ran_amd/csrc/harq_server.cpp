@@ -5,13 +5,14 @@
 // The reference has no such layer (cuPHY is driven slot by slot by
 // cuPHY-CP); its per-slot pipeline (pusch_rx.cpp) is what one launch here
 // corresponds to.  Design:
-//   * one device pool of harq_pool_bytes; a first-fit allocator with
-//     coalescing hands each HARQ process a buffer of (first CB + C) *
-//     Ncb_padded binary16 values when its first transmission (ndi = 1) is
+//   * one device pool of harq_pool_bytes (harq_pool.hpp: size classes over a
+//     coalescing first-fit arena, HARQ processes by (cell, RNTI, process id)
+//     in a bucket table) hands each HARQ process a buffer of (first CB + C)
+//     * Ncb_padded binary16 values when its first transmission (ndi = 1) is
 //     admitted; the buffer is freed when a transmission passes the TB CRC,
-//     carries CUPHY_HIP_HARQ_LAST_TX, or on Release.  A new TB whose buffer
-//     does not fit waits (admission by HBM budget); retransmissions always
-//     have theirs.
+//     carries CUPHY_HIP_HARQ_LAST_TX, on Release, or after harq_timeout_us
+//     unused.  A new TB whose buffer does not fit waits (admission by HBM
+//     budget); retransmissions always have theirs.
 //   * max_inflight launch slots, each with its own stream, event, pinned
 //     staging area and device descriptor area.  Poll() retires finished
 //     slots, then fills every free slot with the admitted requests in
@@ -25,9 +26,11 @@
 //   * resident mode (CUPHY_HIP_HARQ_SERVER_RESIDENT): instead of batched
 //     launches, one resident launch (pusch_serve_resident) takes code
 //     blocks from a ring in host-mapped memory as admitted TBs are
-//     published; TBs occupy descriptor slots until their CRC appears.  Its
-//     workgroups leave after 20 ms without work (handing their tickets to
-//     the next launch) and the engine starts it again when work is open.
+//     published; TBs occupy descriptor slots until their CRC appears.  The
+//     launch leaves after 20 ms without work (its workgroups hand their
+//     tickets to the next launch) and the engine starts it again when work
+//     is open; no item is published into a ring slot whose item an open TB
+//     may not have had read yet.
 //   * per-TB completion: each TB's CRC is written by the kernel straight
 //     into the slot's host-mapped pinned array (CUPHY_HIP_SLOT_TB_NOTIFY:
 //     after all of the TB's outputs are in device memory) as soon as the
@@ -43,11 +46,11 @@
 #include <deque>
 #include <map>
 #include <new>
-#include <unordered_map>
 #include <vector>
 
 #include "api_objects.hpp"
 #include "cuphy_hip.h"
+#include "harq_pool.hpp"
 #include "nr_bg_tables.h"
 #include "pusch_fused.hpp"
 
@@ -55,68 +58,13 @@ namespace {
 
 using clk = std::chrono::steady_clock;
 
-constexpr size_t POOL_ALIGN = 256;
 constexpr uint32_t CB_REJECTED = 0xFFFFFFFFu;
 constexpr uint32_t CRC_PENDING = 0xFFFFFFFEu;  // TB CRC slot before the kernel writes it (CRCs are < 2^24)
 constexpr int SMALL_MB = 8;
 
-size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+using nrharq::round_up;
 
-// First-fit allocator over [0, size) with a coalescing free list.
-class pool_allocator {
-public:
-    void init(size_t size)
-    {
-        free_.clear();
-        if (size) free_[0] = size;
-        in_use_ = peak_ = 0;
-    }
-    bool alloc(size_t bytes, size_t& off)
-    {
-        bytes = round_up(bytes, POOL_ALIGN);
-        for (auto it = free_.begin(); it != free_.end(); ++it) {
-            if (it->second < bytes) continue;
-            off = it->first;
-            const size_t rest = it->second - bytes;
-            free_.erase(it);
-            if (rest) free_[off + bytes] = rest;
-            in_use_ += bytes;
-            peak_ = std::max(peak_, in_use_);
-            return true;
-        }
-        return false;
-    }
-    void release(size_t off, size_t bytes)
-    {
-        bytes = round_up(bytes, POOL_ALIGN);
-        in_use_ -= bytes;
-        auto it = free_.emplace(off, bytes).first;
-        auto next = std::next(it);
-        if (next != free_.end() && it->first + it->second == next->first) {
-            it->second += next->second;
-            free_.erase(next);
-        }
-        if (it != free_.begin()) {
-            auto prev = std::prev(it);
-            if (prev->first + prev->second == it->first) {
-                prev->second += it->second;
-                free_.erase(it);
-            }
-        }
-    }
-    size_t in_use() const { return in_use_; }
-    size_t peak() const { return peak_; }
-
-private:
-    std::map<size_t, size_t> free_;
-    size_t in_use_ = 0, peak_ = 0;
-};
-
-struct harq_proc {
-    size_t off = 0, bytes = 0;
-    bool busy = false;
-    double t_last = 0.0;  // last admission or completion of a transmission using it
-};
+nrharq::key key_of(const cuphyHipHarqRequest_t& r) { return nrharq::key{r.cell_id, r.rnti, r.harq_pid}; }
 
 struct queued {
     cuphyHipHarqRequest_t req;
@@ -161,8 +109,8 @@ struct cuphyHipHarqServer {
     int cu_count = 1;
     cuphyHipHarqServerConfig_t cfg{};
     unsigned char* d_pool = nullptr;
-    pool_allocator pool;
-    std::unordered_map<uint64_t, harq_proc> procs;
+    nrharq::pool pool;                 // size classes over the device pool's byte range
+    nrharq::process_table procs;       // HARQ processes by (cell, RNTI, process id)
     std::deque<queued> pending;
     std::vector<launch_slot> slots;
     std::deque<cuphyHipHarqCompletion_t> ready;
@@ -196,6 +144,7 @@ struct cuphyHipHarqServer {
     std::vector<queued> slot_req;
     std::vector<uint8_t> slot_open;
     std::vector<double> slot_t_publish;
+    std::vector<uint32_t> slot_first_item;  // ring index of the slot's first item
     int n_open_slots = 0;
 
     double now_us() const { return std::chrono::duration<double, std::micro>(clk::now() - t0).count(); }
@@ -214,7 +163,10 @@ struct cuphyHipHarqServer {
     {
         cuphyHipHarqCompletion_t c;
         c.tag = q.req.tag;
-        c.harq_id = q.req.harq_id;
+        c.cell_id = q.req.cell_id;
+        c.rnti = q.req.rnti;
+        c.harq_pid = q.req.harq_pid;
+        c.reserved0 = 0;
         c.tb_crc = crc;
         c.status = status;
         c.t_submit_us = q.t_submit;
@@ -231,22 +183,40 @@ struct cuphyHipHarqServer {
         const double t = now_us();
         if (t - t_last_evict < 1000.0) return;
         t_last_evict = t;
-        for (auto it = procs.begin(); it != procs.end();) {
-            if (!it->second.busy && t - it->second.t_last > cfg.harq_timeout_us) {
-                pool.release(it->second.off, it->second.bytes);
-                it = procs.erase(it);
-                ++evictions;
-            } else {
-                ++it;
-            }
-        }
+        evictions += nrharq::evict_idle(procs, pool, t, cfg.harq_timeout_us);
     }
-    void free_proc(uint64_t id)
+    void free_proc(const nrharq::key& k)
     {
-        auto it = procs.find(id);
-        if (it == procs.end()) return;
-        pool.release(it->second.off, it->second.bytes);
-        procs.erase(it);
+        nrharq::process* pr = procs.find(k);
+        if (!pr) return;
+        pool.release(pr->off, pr->bytes);
+        procs.erase(k);
+    }
+    // The process of a request, with a buffer for a new TB (ndi = 1): an
+    // existing one if it is large enough, else a new one from the pool.
+    // Returns null (and sets *wait) when the pool has no room now.
+    nrharq::process* buffer_for(const queued& q, bool& wait)
+    {
+        wait = false;
+        const nrharq::key k = key_of(q.req);
+        nrharq::process* pr = procs.find(k);
+        if (!q.req.prm.ndi) return pr;
+        const size_t need = harq_bytes(q.req.prm);
+        if (pr && pr->bytes < need) {
+            free_proc(k);
+            pr = nullptr;
+        }
+        if (pr) return pr;
+        size_t off = 0, got = 0;
+        if (!pool.alloc(need, off, got)) {
+            wait = true;
+            return nullptr;
+        }
+        nrharq::process np;
+        np.off = off;
+        np.bytes = got;
+        np.need = need;
+        return procs.insert(k, np);
     }
     cuphyStatus_t retire();
     cuphyStatus_t admit_and_launch();
@@ -307,8 +277,10 @@ void cuphyHipHarqServer::destroy()
 // previous launch.
 cuphyStatus_t cuphyHipHarqServer::start_resident()
 {
+    // the launch's number marks it when its workgroups leave (collective idle exit)
+    const uint32_t epoch = (uint32_t)(launches + 1);
     if (nrldpc::serve_resident_launch(dv_ring, dv_items, ring_size - 1, dv_jobs, (int)slot_open.size(), d_state, d_tbw,
-                                      d_abandoned, r_grid, cfg.llr_type, idle_ticks, life_ticks, r_stream) !=
+                                      d_abandoned, r_grid, cfg.llr_type, idle_ticks, life_ticks, epoch, r_stream) !=
             CUPHY_STATUS_SUCCESS ||
         hipEventRecord(r_done, r_stream) != hipSuccess)
         return CUPHY_STATUS_INTERNAL_ERROR;
@@ -338,14 +310,13 @@ cuphyStatus_t cuphyHipHarqServer::retire_resident()
             continue;
         }
         const queued& r = slot_req[k];
-        auto it = procs.find(r.req.harq_id);
-        if (it != procs.end()) {
-            it->second.busy = false;
-            it->second.t_last = t;
+        if (nrharq::process* pr = procs.find(key_of(r.req))) {
+            pr->busy = false;
+            pr->t_last = t;
         }
         // the kernel released every output of the TB (HARQ buffer included)
         // before its CRC: the buffer may go to another TB now
-        if (crc == 0 || crc == CB_REJECTED || (r.req.flags & CUPHY_HIP_HARQ_LAST_TX)) free_proc(r.req.harq_id);
+        if (crc == 0 || crc == CB_REJECTED || (r.req.flags & CUPHY_HIP_HARQ_LAST_TX)) free_proc(key_of(r.req));
         complete(r, crc, crc == CB_REJECTED ? CUPHY_STATUS_UNSUPPORTED_CONFIG : CUPHY_STATUS_SUCCESS,
                  slot_t_publish[k], t);
         items_open -= r.req.prm.num_CBs;
@@ -365,46 +336,50 @@ cuphyStatus_t cuphyHipHarqServer::admit_resident()
     std::deque<queued> rest;
     bool new_blocked = false;
     const uint32_t first = published;
+    // ring back-pressure: every item not yet read belongs to an open TB, so
+    // no item at or after the oldest open TB's first item may be overwritten
+    uint32_t oldest = published;
+    for (int k = 0; k < (int)slot_open.size() && n_open_slots > 0; ++k)
+        if (slot_open[k] && (int32_t)(slot_first_item[k] - oldest) < 0) oldest = slot_first_item[k];
     while (!pending.empty()) {
         queued q = pending.front();
         pending.pop_front();
         const PerTbParams& p = q.req.prm;
         const uint32_t C = p.num_CBs;
-        if (free_tb_slots.empty() || items_open + C > ring_size ||
-            items_open + C > (uint32_t)cfg.max_cbs_per_launch) {
+        // groups of consecutive CBs decoded side by side in one workgroup
+        const uint32_t P = (uint32_t)nrldpc::fused_group_size(
+            p, q.req.num_parity_nodes, (q.req.flags & CUPHY_HIP_SLOT_TB_DESCRAMBLE) != 0, nrldpc::SERVE_LDS_VALUES);
+        const uint32_t n_items = (C + P - 1) / P;
+        if (free_tb_slots.empty() || items_open + C > ring_size || items_open + C > (uint32_t)cfg.max_cbs_per_launch ||
+            published + n_items - oldest > ring_size) {
             rest.push_back(q);
             continue;
         }
-        auto it = procs.find(q.req.harq_id);
-        if (it != procs.end() && it->second.busy) {  // previous transmission still running
+        {
+            const nrharq::process* cur = procs.find(key_of(q.req));
+            if (cur && cur->busy) {  // previous transmission still running
+                rest.push_back(q);
+                continue;
+            }
+            if (p.ndi && new_blocked && !(cur && cur->bytes >= harq_bytes(p))) {  // behind a waiting new TB
+                rest.push_back(q);
+                continue;
+            }
+        }
+        bool wait = false;
+        nrharq::process* pr = buffer_for(q, wait);
+        if (wait) {
+            ++admission_waits;
+            new_blocked = true;
             rest.push_back(q);
             continue;
         }
-        if (p.ndi) {
-            const size_t need = harq_bytes(p);
-            if (it != procs.end() && it->second.bytes < need) {
-                free_proc(q.req.harq_id);
-                it = procs.end();
-            }
-            if (it == procs.end()) {
-                size_t off = 0;
-                if (new_blocked || !pool.alloc(need, off)) {
-                    if (!new_blocked) ++admission_waits;
-                    new_blocked = true;
-                    rest.push_back(q);
-                    continue;
-                }
-                harq_proc hp;
-                hp.off = off;
-                hp.bytes = need;
-                it = procs.emplace(q.req.harq_id, hp).first;
-            }
-        } else if (it == procs.end()) {
+        if (!pr) {  // retransmission without a HARQ buffer
             complete(q, CB_REJECTED, CUPHY_STATUS_INVALID_ARGUMENT, now_us(), now_us());
             continue;
         }
-        it->second.busy = true;
-        it->second.t_last = now_us();
+        pr->busy = true;
+        pr->t_last = now_us();
         const int k = free_tb_slots.back();
         free_tb_slots.pop_back();
         const cuphyHipHarqRequest_t& r = q.req;
@@ -412,7 +387,7 @@ cuphyStatus_t cuphyHipHarqServer::admit_resident()
         cuphyHipPuschSlotTb_t& j = h_jobs[k];
         j.prm = dv_params + k;
         j.llr = r.llr;
-        j.harq = d_pool + it->second.off;
+        j.harq = d_pool + pr->off;
         j.cb_bits = r.cb_bits;
         j.cb_crc = r.cb_crc;
         j.tb_bytes = r.tb_bytes;
@@ -428,13 +403,12 @@ cuphyStatus_t cuphyHipHarqServer::admit_resident()
         j.clamp_value = r.clamp_value;
         j.share_offset = (uint32_t)k;  // accumulator pair of slot k
         h_tbcrc[k] = CRC_PENDING;
-        // groups of consecutive CBs decoded side by side in one workgroup
-        const uint32_t P = (uint32_t)nrldpc::fused_group_size(
-            r.prm, r.num_parity_nodes, (r.flags & CUPHY_HIP_SLOT_TB_DESCRAMBLE) != 0, nrldpc::SERVE_LDS_VALUES);
-        uint32_t n_items = 0;
+        slot_first_item[k] = published;
+        if (n_open_slots == 0) oldest = published;
+        uint32_t n = 0;
         for (uint32_t c = 0; c < C; c += P)
-            h_items[(published + n_items++) & (ring_size - 1)] = nrldpc::make_item(k, (int)c, (int)std::min(P, C - c));
-        published += n_items;
+            h_items[(published + n++) & (ring_size - 1)] = nrldpc::make_item(k, (int)c, (int)std::min(P, C - c));
+        published += n;
         items_open += C;
         slot_req[k] = q;
         slot_open[k] = 1;
@@ -488,13 +462,12 @@ cuphyStatus_t cuphyHipHarqServer::retire()
         for (size_t i = 0; i < s.reqs.size(); ++i) {
             const queued& r = s.reqs[i];
             const uint32_t crc = s.h_crc[i];
-            auto it = procs.find(r.req.harq_id);
-            if (it != procs.end()) {
-                it->second.busy = false;
-                it->second.t_last = now_us();
+            if (nrharq::process* pr = procs.find(key_of(r.req))) {
+                pr->busy = false;
+                pr->t_last = now_us();
             }
             if (crc == 0 || crc == CB_REJECTED || crc == CRC_PENDING || (r.req.flags & CUPHY_HIP_HARQ_LAST_TX))
-                free_proc(r.req.harq_id);
+                free_proc(key_of(r.req));
         }
         s.reqs.clear();
         s.taken.clear();
@@ -521,36 +494,31 @@ cuphyStatus_t cuphyHipHarqServer::admit_and_launch()
                 rest.push_back(q);
                 continue;
             }
-            auto it = procs.find(q.req.harq_id);
-            if (it != procs.end() && it->second.busy) {  // previous transmission still running
+            {
+                const nrharq::process* cur = procs.find(key_of(q.req));
+                if (cur && cur->busy) {  // previous transmission still running
+                    rest.push_back(q);
+                    continue;
+                }
+                if (p.ndi && new_blocked && !(cur && cur->bytes >= harq_bytes(p))) {  // later new TBs wait
+                    rest.push_back(q);
+                    continue;
+                }
+            }
+            bool wait = false;
+            nrharq::process* pr = buffer_for(q, wait);
+            if (wait) {
+                ++admission_waits;
+                new_blocked = true;  // later new TBs wait behind this one
                 rest.push_back(q);
                 continue;
             }
-            if (p.ndi) {
-                const size_t need = harq_bytes(p);
-                if (it != procs.end() && it->second.bytes < need) {
-                    free_proc(q.req.harq_id);
-                    it = procs.end();
-                }
-                if (it == procs.end()) {
-                    size_t off = 0;
-                    if (new_blocked || !pool.alloc(need, off)) {
-                        if (!new_blocked) ++admission_waits;
-                        new_blocked = true;  // later new TBs wait behind this one
-                        rest.push_back(q);
-                        continue;
-                    }
-                    harq_proc hp;
-                    hp.off = off;
-                    hp.bytes = need;
-                    it = procs.emplace(q.req.harq_id, hp).first;
-                }
-            } else if (it == procs.end()) {  // retransmission without a HARQ buffer
+            if (!pr) {  // retransmission without a HARQ buffer
                 complete(q, CB_REJECTED, CUPHY_STATUS_INVALID_ARGUMENT, now_us(), now_us());
                 continue;
             }
-            it->second.busy = true;
-            it->second.t_last = now_us();
+            pr->busy = true;
+            pr->t_last = now_us();
             s.reqs.push_back(q);
             cbs += C;
         }
@@ -582,7 +550,7 @@ cuphyStatus_t cuphyHipHarqServer::launch(launch_slot& s)
         cuphyHipPuschSlotTb_t& j = h_jobs[t];
         j.prm = d_params + t;
         j.llr = r.llr;
-        j.harq = d_pool + procs[r.harq_id].off;
+        j.harq = d_pool + procs.find(key_of(r))->off;
         j.cb_bits = r.cb_bits;
         j.cb_crc = r.cb_crc;
         j.tb_bytes = r.tb_bytes;
@@ -667,7 +635,8 @@ cuphyStatus_t CUPHYWINAPI cuphyHipHarqServerCreate(cuphyLDPCDecoder_t decoder, c
     if (!decoder || !cfg || !server) return CUPHY_STATUS_INVALID_ARGUMENT;
     if (cfg->max_inflight < 1 || cfg->max_inflight > 8 || cfg->max_tbs_per_launch < 1 ||
         cfg->max_tbs_per_launch > 65536 || cfg->max_cbs_per_launch < 1 || cfg->harq_pool_bytes == 0 ||
-        (cfg->flags & ~CUPHY_HIP_HARQ_SERVER_RESIDENT) != 0 || !(cfg->harq_timeout_us >= 0.0))
+        (cfg->flags & ~CUPHY_HIP_HARQ_SERVER_RESIDENT) != 0 || !(cfg->harq_timeout_us >= 0.0) ||
+        !(cfg->resident_idle_us >= 0.0))
         return CUPHY_STATUS_INVALID_ARGUMENT;
     if ((cfg->flags & CUPHY_HIP_HARQ_SERVER_RESIDENT) && cfg->max_tbs_per_launch > 65535)
         return CUPHY_STATUS_INVALID_ARGUMENT;  // descriptor slot in 16 bits of an item
@@ -712,13 +681,15 @@ cuphyStatus_t CUPHYWINAPI cuphyHipHarqServerCreate(cuphyLDPCDecoder_t decoder, c
              hipEventCreateWithFlags(&s->r_done, hipEventDisableTiming) == hipSuccess &&
              hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, s->device) == hipSuccess && khz > 0 &&
              hipDeviceSynchronize() == hipSuccess;
-        // a resident workgroup leaves after 20 ms without work, and never
-        // runs longer than 60 s
-        s->idle_ticks = (uint64_t)khz * 20u;
+        // a resident launch leaves resident_idle_us (default 20 ms) after its
+        // last work, and never runs longer than 60 s
+        const double idle_us = cfg->resident_idle_us > 0.0 ? cfg->resident_idle_us : 20000.0;
+        s->idle_ticks = (uint64_t)((double)khz * idle_us / 1000.0);
         s->life_ticks = (uint64_t)khz * 60000u;
         s->slot_req.resize(ns);
         s->slot_open.assign(ns, 0);
         s->slot_t_publish.assign(ns, -1.0);
+        s->slot_first_item.assign(ns, 0u);
         for (int k = ns - 1; k >= 0; --k) s->free_tb_slots.push_back(k);
     }
     s->slots.resize(s->resident ? 0 : cfg->max_inflight);
@@ -795,13 +766,15 @@ cuphyStatus_t CUPHYWINAPI cuphyHipHarqServerPoll(cuphyHipHarqServer_t server, cu
     return st;
 }
 
-cuphyStatus_t CUPHYWINAPI cuphyHipHarqServerRelease(cuphyHipHarqServer_t server, uint64_t harq_id)
+cuphyStatus_t CUPHYWINAPI cuphyHipHarqServerRelease(cuphyHipHarqServer_t server, uint32_t cell_id, uint16_t rnti,
+                                                    uint8_t harq_pid)
 {
     if (!server) return CUPHY_STATUS_INVALID_ARGUMENT;
-    auto it = server->procs.find(harq_id);
-    if (it == server->procs.end()) return CUPHY_STATUS_INVALID_ARGUMENT;
-    if (it->second.busy) return CUPHY_STATUS_NOT_SUPPORTED;  // a launch is using it
-    server->free_proc(harq_id);
+    const nrharq::key k{cell_id, rnti, harq_pid};
+    const nrharq::process* pr = server->procs.find(k);
+    if (!pr) return CUPHY_STATUS_INVALID_ARGUMENT;
+    if (pr->busy) return CUPHY_STATUS_NOT_SUPPORTED;  // a launch is using it
+    server->free_proc(k);
     return CUPHY_STATUS_SUCCESS;
 }
 
@@ -816,6 +789,8 @@ cuphyStatus_t CUPHYWINAPI cuphyHipHarqServerGetStats(cuphyHipHarqServer_t server
     stats->pending = server->pending.size();
     stats->admission_waits = server->admission_waits;
     stats->evictions = server->evictions;
+    stats->pool_bytes_cached = server->pool.cached();
+    stats->pool_reclaims = server->pool.reclaims();
     return CUPHY_STATUS_SUCCESS;
 }
 
@@ -830,7 +805,9 @@ cuphyStatus_t CUPHYWINAPI cuphyHipHarqServeTrace(cuphyHipHarqServer_t server, co
     for (int i = 0; i < n; ++i) {
         std::memset(&done[i], 0, sizeof(done[i]));
         done[i].tag = (uint64_t)i;
-        done[i].harq_id = trace[i].req.harq_id;
+        done[i].cell_id = trace[i].req.cell_id;
+        done[i].rnti = trace[i].req.rnti;
+        done[i].harq_pid = trace[i].req.harq_pid;
         done[i].status = CUPHY_STATUS_NOT_SUPPORTED;
     }
     // release schedule: (time, entry)

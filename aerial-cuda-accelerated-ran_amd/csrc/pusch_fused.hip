@@ -556,8 +556,14 @@ void pusch_slot_fused(const cuphyHipPuschSlotTb_t* __restrict__ tbs, int n_tbs,
 // leaves after idle_ticks of the wall clock without work, when the host
 // sets `stop`, or after life_ticks in all, and then leaves its ticket in
 // abandoned[workgroup]; the same workgroup of the next launch (same grid)
-// starts with it, so no item is lost or taken twice.  The host launches the
-// kernel again whenever work is open and none is running.  Each CB starts
+// starts with it, so no item is lost or taken twice.  Leaving is collective:
+// the first workgroup to leave marks the launch (st->leave = epoch, the
+// launch's number), and every other workgroup then leaves at its next wait,
+// parking its ticket the same way, without taking a new one -- so no
+// published item waits behind a parked ticket while the launch goes on
+// serving later ones; the launch ends within one code block's time.  The
+// host launches the kernel again whenever work is open and none is
+// running.  Each CB starts
 // with an agent-scope acquire (inputs and HARQ buffers written since may
 // sit behind stale lines of this XCD's L2) and ends with the TB-CRC
 // protocol of complete_cb in notify mode.
@@ -565,13 +571,18 @@ void pusch_slot_fused(const cuphyHipPuschSlotTb_t* __restrict__ tbs, int n_tbs,
 constexpr uint32_t NO_TICKET = 0xFFFFFFFFu;
 
 __device__ __forceinline__ int serve_next(const serve_ring_hdr* ring, serve_state* st, uint32_t* abandoned,
-                                          bool first, uint64_t t_start, uint64_t idle_ticks, uint64_t life_ticks)
+                                          bool first, uint64_t t_start, uint64_t idle_ticks, uint64_t life_ticks,
+                                          uint32_t epoch)
 {
     uint32_t i = NO_TICKET;
     if (first) {
         i = __hip_atomic_exchange(&abandoned[blockIdx.x], NO_TICKET, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    if (i == NO_TICKET) i = __hip_atomic_fetch_add(&st->head, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (i == NO_TICKET) {
+        // the launch is leaving: take no new ticket
+        if (__hip_atomic_load(&st->leave, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch) return -1;
+        i = __hip_atomic_fetch_add(&st->head, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
     const uint64_t t_wait = wall_clock64();
     for (;;) {
         const uint32_t t = __hip_atomic_load(&st->tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -587,9 +598,11 @@ __device__ __forceinline__ int serve_next(const serve_ring_hdr* ring, serve_stat
             if ((int32_t)(ht - i) > 0) return (int)(i & 0x7FFFFFFFu);
         }
         const uint64_t now = wall_clock64();
-        if (__hip_atomic_load(&st->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) || now - t_wait > idle_ticks ||
-            now - t_start > life_ticks) {
+        if (__hip_atomic_load(&st->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
+            __hip_atomic_load(&st->leave, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch ||
+            now - t_wait > idle_ticks || now - t_start > life_ticks) {
             __hip_atomic_store(&abandoned[blockIdx.x], i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&st->leave, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             return -1;
         }
         __builtin_amdgcn_s_sleep(32);
@@ -602,7 +615,7 @@ __global__ __launch_bounds__(384) void pusch_serve_resident(const serve_ring_hdr
                                                             const cuphyHipPuschSlotTb_t* jobs, int n_slots,
                                                             serve_state* st, uint32_t* abandoned,
                                                             uint32_t* __restrict__ tbw, int lds_values,
-                                                            uint64_t idle_ticks, uint64_t life_ticks)
+                                                            uint64_t idle_ticks, uint64_t life_ticks, uint32_t epoch)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     __shared__ fused_lds sh;
@@ -615,7 +628,7 @@ __global__ __launch_bounds__(384) void pusch_serve_resident(const serve_ring_hdr
     const uint64_t t_start = wall_clock64();
     bool first = true;
     for (;;) {
-        if (tid == 0) sh.item = serve_next(ring, st, abandoned, first, t_start, idle_ticks, life_ticks);
+        if (tid == 0) sh.item = serve_next(ring, st, abandoned, first, t_start, idle_ticks, life_ticks, epoch);
         first = false;
         __syncthreads();
         const int i = sh.item;
@@ -706,7 +719,7 @@ int serve_resident_grid(int cu_count)
 cuphyStatus_t serve_resident_launch(const serve_ring_hdr* ring, const uint32_t* ring_items, uint32_t ring_mask,
                                     const cuphyHipPuschSlotTb_t* jobs, int n_slots, serve_state* st, uint32_t* tbw,
                                     uint32_t* abandoned, int grid, cuphyDataType_t llr_type, uint64_t idle_ticks,
-                                    uint64_t life_ticks, hipStream_t strm)
+                                    uint64_t life_ticks, uint32_t epoch, hipStream_t strm)
 {
     void* func = nullptr;
     switch (llr_type) {
@@ -719,9 +732,9 @@ cuphyStatus_t serve_resident_launch(const serve_ring_hdr* ring, const uint32_t* 
     const size_t lds = ((size_t)lds_values * 2 + 15) / 16 * 16 + 64u;
     cuphyStatus_t s = ensure_lds_attr(func, lds);
     if (s != CUPHY_STATUS_SUCCESS) return s;
-    void* args[11] = {const_cast<serve_ring_hdr**>(&ring), const_cast<uint32_t**>(&ring_items), &ring_mask,
+    void* args[12] = {const_cast<serve_ring_hdr**>(&ring), const_cast<uint32_t**>(&ring_items), &ring_mask,
                       const_cast<cuphyHipPuschSlotTb_t**>(&jobs), &n_slots, &st, &abandoned, &tbw, &lds_values,
-                      &idle_ticks, &life_ticks};
+                      &idle_ticks, &life_ticks, &epoch};
     if (hipLaunchKernel(func, dim3((unsigned)grid), dim3(FUSED_BLOCK), args, lds, strm) != hipSuccess)
         return CUPHY_STATUS_INTERNAL_ERROR;
     return CUPHY_STATUS_SUCCESS;

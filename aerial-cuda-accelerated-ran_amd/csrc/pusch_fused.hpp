@@ -85,7 +85,8 @@ struct serve_state {       // one 256-byte line per word: tickets do not contend
     uint32_t pad1[63];
     uint32_t token;        // a workgroup is reading the host tail
     uint32_t stop;         // host stop seen
-    uint32_t pad2[62];
+    uint32_t leave;        // = the launch's epoch once one of its workgroups left: the others follow
+    uint32_t pad2[61];
 };
 
 // Workgroups of a resident launch (the same for every launch of a server:
@@ -95,7 +96,7 @@ int serve_resident_grid(int cu_count);
 cuphyStatus_t serve_resident_launch(const serve_ring_hdr* ring, const uint32_t* ring_items, uint32_t ring_mask,
                                     const cuphyHipPuschSlotTb_t* jobs, int n_slots, serve_state* st, uint32_t* tbw,
                                     uint32_t* abandoned, int grid, cuphyDataType_t llr_type, uint64_t idle_ticks,
-                                    uint64_t life_ticks, hipStream_t strm);
+                                    uint64_t life_ticks, uint32_t epoch, hipStream_t strm);
 
 // Look up the device address of the allocation-free entry point's scratch
 // array on `device` (once per device; the device is made current for the

@@ -552,10 +552,15 @@ typedef struct {
     uint32_t flags;                  /* 0 (batched launches) or CUPHY_HIP_HARQ_SERVER_RESIDENT */
     double harq_timeout_us;          /* > 0: free a HARQ buffer unused this long (the reference's
                                         HarqPoolManager timeout, harq_pool.hpp); 0: never */
+    double resident_idle_us;         /* resident mode: the launch leaves this long after its last work
+                                        (0: 20000) and is started again when work arrives */
 } cuphyHipHarqServerConfig_t;
 
 typedef struct {
-    uint64_t harq_id;                /* HARQ process; one TB at a time */
+    uint32_t cell_id;                /* HARQ process (cell, RNTI, process id); one TB at a time, as the */
+    uint16_t rnti;                   /* reference's HarqPoolManager keys its buffers (harq_pool.hpp:389-545) */
+    uint8_t harq_pid;
+    uint8_t reserved0;
     uint64_t tag;                    /* returned in the completion */
     PerTbParams prm;                 /* this transmission's parameters (ndi, rv/k0, G, ...) */
     const void* llr;                 /* device: this transmission's LLRs, rate-match input layout */
@@ -577,7 +582,10 @@ typedef struct {
 
 typedef struct {
     uint64_t tag;
-    uint64_t harq_id;
+    uint32_t cell_id;                /* the request's HARQ process */
+    uint16_t rnti;
+    uint8_t harq_pid;
+    uint8_t reserved0;
     uint32_t tb_crc;                 /* TB CRC remainder, 0 = pass */
     int32_t status;                  /* CUPHY_STATUS_SUCCESS, or why the request was rejected */
     double t_submit_us;              /* host steady clock, microseconds since server creation */
@@ -591,6 +599,8 @@ typedef struct {
     uint64_t pending;                /* submitted, not yet launched */
     uint64_t admission_waits;        /* polls at which a new TB waited for pool space */
     uint64_t evictions;              /* HARQ buffers freed by harq_timeout_us */
+    uint64_t pool_bytes_cached;      /* freed buffers kept in their size class for reuse */
+    uint64_t pool_reclaims;          /* times the cached buffers went back to the pool to make room */
 } cuphyHipHarqServerStats_t;
 
 cuphyStatus_t CUPHYWINAPI cuphyHipHarqServerCreate(cuphyLDPCDecoder_t decoder, const cuphyHipHarqServerConfig_t* cfg,
@@ -606,7 +616,8 @@ cuphyStatus_t CUPHYWINAPI cuphyHipHarqServerSubmit(cuphyHipHarqServer_t server, 
 cuphyStatus_t CUPHYWINAPI cuphyHipHarqServerPoll(cuphyHipHarqServer_t server, cuphyHipHarqCompletion_t* out,
                                                  int max_out, int* n_out);
 /* Free a HARQ process's buffer (e.g. the MAC gave up on the TB). */
-cuphyStatus_t CUPHYWINAPI cuphyHipHarqServerRelease(cuphyHipHarqServer_t server, uint64_t harq_id);
+cuphyStatus_t CUPHYWINAPI cuphyHipHarqServerRelease(cuphyHipHarqServer_t server, uint32_t cell_id, uint16_t rnti,
+                                                    uint8_t harq_pid);
 cuphyStatus_t CUPHYWINAPI cuphyHipHarqServerGetStats(cuphyHipHarqServer_t server, cuphyHipHarqServerStats_t* stats);
 
 /* Load-test driver: run a trace of transmissions in real time through the

@@ -178,7 +178,7 @@ def test_harq_server_argument_validation(L):
     assert lib.cuphyHipHarqServerSubmit(None, None) == INV
     n = C.c_int()
     assert lib.cuphyHipHarqServerPoll(None, None, 0, C.byref(n)) == INV
-    assert lib.cuphyHipHarqServerRelease(None, 1) == INV
+    assert lib.cuphyHipHarqServerRelease(None, 0, 1, 2) == INV
     assert lib.cuphyHipHarqServeTrace(None, None, 0, 0.0, None, C.byref(n)) == INV
 
 

@@ -147,6 +147,30 @@ def test_results_equal_alone_and_in_the_mix(dec, templates, resident):
     srv.close()
 
 
+def test_resident_frequent_idle_exits(dec, templates):
+    """A resident launch that leaves 100 us after its last work, fed TBs
+    with gaps of that order: the launch leaves and restarts many times, its
+    workgroups handing their unfilled tickets to the next launch, while TBs
+    keep arriving.  Every transmission completes exactly once and every TB
+    equals the oracle's chain and its decode alone."""
+    from aerial_amd.server import HarqServer, build_trace, poisson_arrivals
+    print("[case] server: resident launch with 100 us idle exit, 40 TBs over 4 ms", flush=True)
+    srv = HarqServer(dec, pool_bytes=1 << 30, max_tbs=64, max_cbs=1024, resident=True, resident_idle_us=100.0)
+    arrivals = poisson_arrivals(n_cells=4, rate_per_s=10000.0, duration_us=6000.0, n_templates=len(templates.tbs),
+                                seed=9)[:40]
+    assert len(arrivals) == 40
+    entries, outs, meta = build_trace(srv, templates, arrivals, max_iterations=8)
+    done = srv.serve_trace(entries, harq_rtt_us=250.0)
+    st = srv.stats()
+    srv.close()
+    assert st["launches"] >= 4, st
+    assert st["completed"] == st["submitted"] and st["pending"] == 0 and st["pool_bytes_in_use"] == 0, st
+    released = [i for i in range(len(meta)) if done[i].status == 0]
+    assert len(set(done[i].tag for i in released)) == len(released)
+    check_against_oracle(templates, done, meta, outs, arrivals, 8)
+    check_against_alone(dec, templates, done, meta, outs, arrivals, 8)
+
+
 @pytest.mark.parametrize("resident", [False, True])
 def test_tb_completes_before_its_launch_ends(dec, resident):
     # a one-CB TB and a 24-CB low-SNR TB in the same launch: the small TB's
