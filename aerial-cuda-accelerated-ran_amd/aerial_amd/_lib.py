@@ -167,6 +167,10 @@ class LbrmPrm(C.Structure):
                 ("n_PRB_LBRM", C.c_uint16), ("reserved2", C.c_uint16)]
 
 
+class LDPCEncodeLaunchConfig(C.Structure):
+    _fields_ = [("m_kernelNodeParams", HipKernelNodeParams), ("m_desc", C.c_void_p), ("m_kernelArgs", C.c_void_p * 1)]
+
+
 class TensorPrm(C.Structure):
     _fields_ = [("desc", C.c_void_p), ("pAddr", C.c_void_p)]
 
@@ -227,6 +231,10 @@ SIGNATURES = {
     "cuphyHipHarqServerGetStats": ([_VP, _P(HarqServerStats)], _I),
     "cuphyHipHarqServeTrace": ([_VP, _VP, _I, C.c_double, _VP, _P(_I)], _I),
     "cuphyHipLDPCWorkItemCheck": ([_P(LDPCWorkItem), _I, _P(C.c_uint32)], _I),
+    "cuphyLDPCEncodeGetDescrInfo": ([_P(C.c_size_t), _P(C.c_size_t), C.c_uint16, _P(C.c_size_t)], _I),
+    "cuphySetupLDPCEncode": ([_P(LDPCEncodeLaunchConfig), _VP, _VP, _VP, _VP, _I, _I, C.c_uint8, _I, _I, C.c_uint8,
+                              _I, _VP, _VP, _VP, _VP, _VP, _VP, C.c_uint8, _VP], _I),
+    "cuphyHipDlRateMatch": ([_VP, _I, _VP, _I, _I, C.c_uint8, _VP, _VP], _I),
     "cuphyHipTransportBlockSize": ([_I, _I, _I, C.c_float, C.c_uint32, _I, _P(C.c_uint32), _P(C.c_uint32)], _I),
     "cuphyHipSetPerTbParams": ([_P(PerTbParams), C.c_uint32, C.c_float, C.c_uint8, C.c_uint32, C.c_uint32,
                                 C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint8, C.c_uint8, _P(C.c_uint32), C.c_uint8,

@@ -23,6 +23,7 @@ SOURCES = [
     "ldpc_decode.hip",
     "ldpc_encode.hip",
     "derate_match.hip",
+    "dl_rate_match.hip",
     "crc_decode.hip",
     "pusch_fused.hip",
     "cuphy_api.cpp",

@@ -14,6 +14,7 @@
 #include "ldpc_decode.hpp"
 #include "ldpc_encode.hpp"
 #include "derate_match.hpp"
+#include "dl_rate_match.hpp"
 #include "crc_decode.hpp"
 #include "pusch_fused.hpp"
 
@@ -378,6 +379,94 @@ cuphyStatus_t CUPHYWINAPI cuphyErrorCorrectionLDPCTransportBlockDecode(cuphyLDPC
     if (decodeDesc->num_tbs == 0) return CUPHY_STATUS_SUCCESS;
     if (!nrldpc::config_supported(c)) return CUPHY_STATUS_UNSUPPORTED_CONFIG;
     return nrldpc::decode_tb(*decodeDesc, strm);
+}
+
+// ---------------------------------------------------------------------------
+// LDPC encoder API (cuphy.h:3781-3850; setup as error_correction/
+// ldpc_encode.cu:394-620): bit tensors [K bits x C] in, [N bits x C] out.
+// ---------------------------------------------------------------------------
+cuphyStatus_t CUPHYWINAPI cuphyLDPCEncodeGetDescrInfo(size_t* pDescrSizeBytes, size_t* pDescrAlignBytes,
+                                                      uint16_t maxUes, size_t* pWorkspaceBytes)
+{
+    if (!pDescrSizeBytes || !pDescrAlignBytes || !pWorkspaceBytes) return CUPHY_STATUS_INVALID_ARGUMENT;
+    *pDescrSizeBytes = sizeof(nrldpc::enc_descr);
+    *pDescrAlignBytes = alignof(nrldpc::enc_descr);
+    *pWorkspaceBytes = 2u * (size_t)maxUes * sizeof(nrldpc::enc_tensor);  // inputs, then outputs
+    return CUPHY_STATUS_SUCCESS;
+}
+
+cuphyStatus_t CUPHYWINAPI cuphySetupLDPCEncode(cuphyLDPCEncodeLaunchConfig_t cfg, cuphyTensorDescriptor_t inDesc,
+                                               void* inAddr, cuphyTensorDescriptor_t outDesc, void* outAddr, int BG,
+                                               int Z, uint8_t puncture, int maxParityNodes, int max_rv,
+                                               uint8_t batching, int batched_TBs, void** inBatchedAddr,
+                                               void** outBatchedAddr, void* h_workspace, void* d_workspace,
+                                               void* cpu_desc, void* gpu_desc, uint8_t enable_desc_async_copy,
+                                               hipStream_t strm)
+{
+    // ldpc_encode.cu:416-470
+    if (!cfg || !inDesc || !outDesc || !cpu_desc || max_rv < 0 || max_rv > 3) return CUPHY_STATUS_INVALID_ARGUMENT;
+    if (batching && (batched_TBs < 1 || batched_TBs > 65535 || !inBatchedAddr || !outBatchedAddr))
+        return CUPHY_STATUS_INVALID_ARGUMENT;
+    if (!h_workspace || !d_workspace || (enable_desc_async_copy && !gpu_desc)) return CUPHY_STATUS_INVALID_ARGUMENT;
+    if (inDesc->rank > 2 || outDesc->rank > 2) return CUPHY_STATUS_UNSUPPORTED_RANK;
+    if (inDesc->type != CUPHY_BIT || outDesc->type != CUPHY_BIT) return CUPHY_STATUS_UNSUPPORTED_TYPE;
+    if (BG < 1 || BG > 2 || !nrldpc::is_lifting_size(Z)) return CUPHY_STATUS_UNSUPPORTED_CONFIG;
+    // a subset of the parity rows only for rv 0 (ldpc_encode.cu:455-458)
+    const int all_rows = BG == 1 ? 46 : 42;
+    const bool fewer = max_rv == 0 && maxParityNodes != 0 && maxParityNodes < all_rows;
+    const int num_rows = fewer ? maxParityNodes : all_rows;
+    if (num_rows + (puncture ? 2 : 0) < 4) return CUPHY_STATUS_INVALID_ARGUMENT;  // :466-469
+    const int nsys = BG == 1 ? 22 : 10;
+    const int out_bits = (nsys + num_rows) * Z - (puncture ? 2 * Z : 0);
+    auto cols = [](const cuphyTensorDescriptor& t) { return t.rank > 1 ? t.dims[1] : 1; };
+    auto stride_words = [](const cuphyTensorDescriptor& t) {
+        return t.rank > 1 ? t.strides[1] / 32 : (t.dims[0] + 31) / 32;
+    };
+    if (inDesc->dims[0] < nsys * Z || outDesc->dims[0] < out_bits) return CUPHY_STATUS_INVALID_ARGUMENT;
+    if ((inDesc->rank > 1 && inDesc->strides[1] % 32) || (outDesc->rank > 1 && outDesc->strides[1] % 32))
+        return CUPHY_STATUS_UNSUPPORTED_ALIGNMENT;
+    const int n_tbs = batching ? batched_TBs : 1;
+    nrldpc::enc_tensor* h_in = static_cast<nrldpc::enc_tensor*>(h_workspace);
+    nrldpc::enc_tensor* h_out = h_in + n_tbs;
+    for (int t = 0; t < n_tbs; ++t) {
+        void* ia = batching ? inBatchedAddr[t] : inAddr;
+        void* oa = batching ? outBatchedAddr[t] : outAddr;
+        if (!ia || !oa) return CUPHY_STATUS_INVALID_ARGUMENT;
+        h_in[t] = nrldpc::enc_tensor{static_cast<uint32_t*>(ia), stride_words(*inDesc), cols(*inDesc)};
+        h_out[t] = nrldpc::enc_tensor{static_cast<uint32_t*>(oa), stride_words(*outDesc), cols(*outDesc)};
+    }
+    nrldpc::enc_descr& d = *static_cast<nrldpc::enc_descr*>(cpu_desc);
+    d.input = static_cast<const nrldpc::enc_tensor*>(d_workspace);
+    d.output = static_cast<const nrldpc::enc_tensor*>(d_workspace) + n_tbs;
+    d.num_TBs = n_tbs;
+    d.BG = (int16_t)BG;
+    d.Kb = (int16_t)nsys;  // the information columns the kernel reads (ldpc_encode.cu:483)
+    d.Z = (int16_t)Z;
+    d.num_rows = (int16_t)num_rows;
+    d.puncture = puncture ? 1 : 0;
+    if (enable_desc_async_copy &&
+        (hipMemcpyAsync(d_workspace, h_workspace, 2u * n_tbs * sizeof(nrldpc::enc_tensor), hipMemcpyHostToDevice,
+                        strm) != hipSuccess ||
+         hipMemcpyAsync(gpu_desc, cpu_desc, sizeof(nrldpc::enc_descr), hipMemcpyHostToDevice, strm) != hipSuccess))
+        return CUPHY_STATUS_MEMCPY_ERROR;
+    cfg->m_desc = gpu_desc;
+    // the descriptor is the kernel's by-value argument (ldpc_encode.cu:544-546)
+    cfg->m_kernelArgs[0] = cpu_desc;
+    const int max_cw = cols(*inDesc);
+    set_node(cfg->m_kernelNodeParams, nrldpc::encode_bits_kernel(), dim3((unsigned)std::max(1, max_cw), (unsigned)n_tbs, 1),
+             dim3(256), &cfg->m_kernelArgs[0]);
+    cfg->m_kernelNodeParams.sharedMemBytes = (unsigned)((nsys + std::max(num_rows, 4) + 4) * Z);
+    return CUPHY_STATUS_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
+// DL rate matching of encoded code blocks (library extension)
+// ---------------------------------------------------------------------------
+cuphyStatus_t CUPHYWINAPI cuphyHipDlRateMatch(const PerTbParams* d_tb_prms, int num_tbs, const uint32_t* const* d_cw,
+                                              int max_cbs, int emax, uint8_t scrambling, uint32_t* d_out,
+                                              hipStream_t strm)
+{
+    return nrldpc::dl_rate_match(d_tb_prms, num_tbs, d_cw, max_cbs, emax, scrambling != 0, d_out, strm);
 }
 
 // cuphy_ldpc.cpp:116-155: the kernels need no workspace.

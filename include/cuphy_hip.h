@@ -639,6 +639,58 @@ cuphyStatus_t CUPHYWINAPI cuphyHipHarqServeTrace(cuphyHipHarqServer_t server, co
                                                  int* n_done);
 
 /* ------------------------------------------------------------------------
+ * LDPC encoder (cuphy.h:3781-3850): bit tensors in, bit tensors out
+ * ---------------------------------------------------------------------- */
+
+/* Launch configuration of the encoder (cuphy.h:3781-3790; the kernel node
+ * parameters are hipKernelNodeParams here). */
+struct cuphyLDPCEncodeLaunchConfig {
+    hipKernelNodeParams m_kernelNodeParams;
+    void* m_desc;
+    void* m_kernelArgs[1];
+};
+typedef struct cuphyLDPCEncodeLaunchConfig* cuphyLDPCEncodeLaunchConfig_t;
+
+/* Descriptor size / alignment and workspace bytes for maxUes TBs
+ * (cuphy.h:3799-3802). */
+cuphyStatus_t CUPHYWINAPI cuphyLDPCEncodeGetDescrInfo(size_t* pDescrSizeBytes, size_t* pDescrAlignBytes,
+                                                      uint16_t maxUes, size_t* pWorkspaceBytes);
+
+/* Set up an encode launch (cuphy.h:3828-3847): inDesc CUPHY_BIT [K bits x C]
+ * (K >= Kb * Z, Kb = 22 / 10, filler bits as the caller set them),
+ * outDesc CUPHY_BIT [N bits x C], N >= (Kb + num_rows) * Z less 2 Z when
+ * puncture; column strides multiples of 32 bits.  num_rows = maxParityNodes
+ * when max_rv == 0 and 0 < maxParityNodes < 46 / 42, else every parity
+ * row.  batching: inBatchedAddr / outBatchedAddr give batched_TBs TBs of
+ * the descriptors' shape.  h_workspace / d_workspace hold the TBs' tensor
+ * addresses (copied with the descriptor when enable_desc_async_copy);
+ * launch m_kernelNodeParams on a stream or put it in a graph.  Errors as
+ * the reference (ldpc_encode.cu:416-470). */
+cuphyStatus_t CUPHYWINAPI cuphySetupLDPCEncode(cuphyLDPCEncodeLaunchConfig_t ldpcEncodeLaunchConfig,
+                                               cuphyTensorDescriptor_t inDesc, void* inAddr,
+                                               cuphyTensorDescriptor_t outDesc, void* outAddr, int BG, int Z,
+                                               uint8_t puncture, int maxParityNodes, int max_rv, uint8_t batching,
+                                               int batched_TBs, void** inBatchedAddr, void** outBatchedAddr,
+                                               void* h_workspace, void* d_workspace, void* cpu_desc, void* gpu_desc,
+                                               uint8_t enable_desc_async_copy, hipStream_t strm);
+
+/* DL rate matching of encoded code blocks: for each TB t (d_tb_prms[t]:
+ * bg, Zc, num_CBs, N = encoded bits per CB, Ncb, K, F, k0, Qm, Nl,
+ * encodedSize = G, cinit) and CB r, bit selection from k0 around Ncb
+ * skipping the filler bits, bit interleaving, and, when scrambling, XOR
+ * with the TB's Gold sequence; CB r's E_r bits at bit (t * max_cbs + r) *
+ * emax of d_out, the rest of its emax-bit slot zero.  d_cw: device array of
+ * num_tbs device pointers, CB r of TB t at word r * ceil(N / 32) (the
+ * encoder's punctured output).  The reference's cuphySetupDlRateMatching
+ * (cuphy.h:3726-3752) with layer mapping and modulation disabled produces
+ * this layout; that entry point also takes the PDSCH DMRS / UE-group
+ * structures and is not reproduced.  emax: a multiple of 32, at most
+ * 393,216; a CB whose E_r exceeds it (or N > 25,344) is left zero. */
+cuphyStatus_t CUPHYWINAPI cuphyHipDlRateMatch(const PerTbParams* d_tb_prms, int num_tbs, const uint32_t* const* d_cw,
+                                              int max_cbs, int emax, uint8_t scrambling, uint32_t* d_out,
+                                              hipStream_t strm);
+
+/* ------------------------------------------------------------------------
  * Per-TB parameter derivation (host only, no device call)
  * ---------------------------------------------------------------------- */
 

@@ -257,3 +257,49 @@ def test_work_item_check(L):
         L.ITEM_BAD_BG | L.ITEM_BAD_Z | L.ITEM_BAD_MB | L.ITEM_BAD_KB | L.ITEM_BAD_ITER | L.ITEM_NULL_BUFFER
     r = C.c_uint32()
     assert lib.cuphyHipLDPCWorkItemCheck(None, 46, C.byref(r)) == L.CUPHY_STATUS_INVALID_ARGUMENT
+
+
+def test_encoder_setup_validation(L):
+    """cuphyLDPCEncodeGetDescrInfo sizes and cuphySetupLDPCEncode's argument
+    checks (ldpc_encode.cu:416-470); setup without the descriptor copy makes
+    no device call."""
+    lib = L.lib()
+    dsz, dal, wsz = C.c_size_t(), C.c_size_t(), C.c_size_t()
+    assert lib.cuphyLDPCEncodeGetDescrInfo(C.byref(dsz), C.byref(dal), 10, C.byref(wsz)) == 0
+    assert dsz.value > 0 and dal.value in (4, 8, 16) and wsz.value == 10 * 2 * 16
+    assert lib.cuphyLDPCEncodeGetDescrInfo(None, C.byref(dal), 10, C.byref(wsz)) == L.CUPHY_STATUS_INVALID_ARGUMENT
+
+    def tensor(t, nbits, ncw):
+        h = C.c_void_p()
+        assert lib.cuphyCreateTensorDescriptor(C.byref(h)) == 0
+        dims = (C.c_int * 2)(nbits, ncw)
+        strides = (C.c_int * 2)(1, (nbits + 31) // 32 * 32)
+        assert lib.cuphySetTensorDescriptor(h, t, 2, dims, strides, 0) == 0
+        return h
+
+    tin, tout = tensor(L.CUPHY_BIT, 22 * 384, 4), tensor(L.CUPHY_BIT, 66 * 384, 4)
+    tf = tensor(L.CUPHY_R_32F, 22 * 384, 4)
+    cfg = L.LDPCEncodeLaunchConfig()
+    cpu = (C.c_uint8 * dsz.value)()
+    hws = (C.c_uint8 * wsz.value)()
+    dws = C.c_void_p(0x1000)  # not dereferenced without the copy
+    buf = C.c_void_p(0x2000)
+
+    def setup(i=tin, o=tout, bg=1, Z=384, punct=1, rows=0, rv=0, batching=0, n=1, h=hws, d=dws):
+        return lib.cuphySetupLDPCEncode(C.byref(cfg), i, buf, o, buf, bg, Z, punct, rows, rv, batching, n, None, None,
+                                        C.cast(h, C.c_void_p) if h is not None else None, d,
+                                        C.cast(cpu, C.c_void_p), None, 0, None)
+
+    assert setup() == L.CUPHY_STATUS_SUCCESS
+    g = cfg.m_kernelNodeParams
+    assert (g.gridDim.x, g.gridDim.y, g.blockDim.x) == (4, 1, 256) and cfg.m_kernelArgs[0] is not None
+    assert setup(rv=4) == L.CUPHY_STATUS_INVALID_ARGUMENT and setup(rv=-1) == L.CUPHY_STATUS_INVALID_ARGUMENT
+    assert setup(batching=1, n=2) == L.CUPHY_STATUS_INVALID_ARGUMENT        # no batched address arrays
+    assert setup(h=None) == L.CUPHY_STATUS_INVALID_ARGUMENT
+    assert setup(i=tf) == L.CUPHY_STATUS_UNSUPPORTED_TYPE
+    assert setup(bg=3) == L.CUPHY_STATUS_UNSUPPORTED_CONFIG and setup(Z=17) == L.CUPHY_STATUS_UNSUPPORTED_CONFIG
+    assert setup(punct=0, rows=3) == L.CUPHY_STATUS_INVALID_ARGUMENT        # fewer than 4 rows in LDS
+    assert setup(punct=1, rows=2) == L.CUPHY_STATUS_SUCCESS
+    assert setup(punct=0, rows=0) == L.CUPHY_STATUS_INVALID_ARGUMENT       # 68 Z bits do not fit the output
+    for h in (tin, tout, tf):
+        lib.cuphyDestroyTensorDescriptor(h)
