@@ -9,6 +9,7 @@
 #include <cstring>
 #include <new>
 
+#include "trace.hpp"
 #include "cuphy_hip.h"
 #include "api_objects.hpp"
 #include "ldpc_decode.hpp"
@@ -315,6 +316,7 @@ cuphyStatus_t CUPHYWINAPI cuphyErrorCorrectionLDPCDecode(cuphyLDPCDecoder_t deco
                                                          void* softOutputsAddr, const cuphyLDPCDecodeConfigDesc_t* config,
                                                          hipStream_t strm)
 {
+    CUPHY_HIP_RANGE("cuphyErrorCorrectionLDPCDecode");
     // cuphy_ldpc.cpp:36-55
     if (!decoder || !tensorDescDst || !dstAddr || !tensorDescLLR || !LLRAddr || !config ||
         config->max_iterations < 0 || config->BG < 1 || config->BG > 2 ||
@@ -365,6 +367,7 @@ cuphyStatus_t CUPHYWINAPI cuphyErrorCorrectionLDPCTransportBlockDecode(cuphyLDPC
                                                                        const cuphyLDPCDecodeDesc_t* decodeDesc,
                                                                        hipStream_t strm)
 {
+    CUPHY_HIP_RANGE("cuphyErrorCorrectionLDPCTransportBlockDecode");
     if (!decoder || !decodeDesc) return CUPHY_STATUS_INVALID_ARGUMENT;  // cuphy_ldpc.cpp:105-109
     const cuphyLDPCDecodeConfigDesc_t& c = decodeDesc->config;
     if (c.max_iterations < 0 || c.BG < 1 || c.BG > 2 || c.Z < 2 || c.Z > 384 || c.num_parity_nodes < 4 ||
@@ -403,6 +406,7 @@ cuphyStatus_t CUPHYWINAPI cuphySetupLDPCEncode(cuphyLDPCEncodeLaunchConfig_t cfg
                                                void* cpu_desc, void* gpu_desc, uint8_t enable_desc_async_copy,
                                                hipStream_t strm)
 {
+    CUPHY_HIP_RANGE("cuphySetupLDPCEncode");
     // ldpc_encode.cu:416-470
     if (!cfg || !inDesc || !outDesc || !cpu_desc || max_rv < 0 || max_rv > 3) return CUPHY_STATUS_INVALID_ARGUMENT;
     if (batching && (batched_TBs < 1 || batched_TBs > 65535 || !inBatchedAddr || !outBatchedAddr))
@@ -466,6 +470,7 @@ cuphyStatus_t CUPHYWINAPI cuphyHipDlRateMatch(const PerTbParams* d_tb_prms, int 
                                               int max_cbs, int emax, uint8_t scrambling, uint32_t* d_out,
                                               hipStream_t strm)
 {
+    CUPHY_HIP_RANGE("cuphyHipDlRateMatch");
     return nrldpc::dl_rate_match(d_tb_prms, num_tbs, d_cw, max_cbs, emax, scrambling != 0, d_out, strm);
 }
 
@@ -506,6 +511,7 @@ cuphyStatus_t CUPHYWINAPI cuphyErrorCorrectionLDPCDecodeSetNormalization(cuphyLD
 cuphyStatus_t CUPHYWINAPI cuphyErrorCorrectionLDPCDecodeGetLaunchDescriptor(cuphyLDPCDecoder_t decoder,
                                                                             cuphyLDPCDecodeLaunchConfig_t* launchConfig)
 {
+    CUPHY_HIP_RANGE("cuphyErrorCorrectionLDPCDecodeGetLaunchDescriptor");
     if (!decoder || !launchConfig) return CUPHY_STATUS_INVALID_ARGUMENT;
     return nrldpc::get_launch_config(*launchConfig);
 }
@@ -542,6 +548,7 @@ cuphyStatus_t CUPHYWINAPI cuphySetupPuschRxRateMatch(cuphyPuschRxRateMatchHndl_t
                                                      void* pGpuDesc, uint8_t enableCpuToGpuDescrAsyncCpy,
                                                      cuphyPuschRxRateMatchLaunchCfg_t* pLaunchCfg, hipStream_t strm)
 {
+    CUPHY_HIP_RANGE("cuphySetupPuschRxRateMatch");
     if (!puschRxRateMatchHndl || !pTbPrmsCpu || !pTbPrmsGpu || !pTPrmRmIn || !pTPrmCdm1RmIn || !ppRmOut || !pCpuDesc ||
         !pGpuDesc || !pLaunchCfg || !pSchUserIdxsCpu || nSchUes > MAX_N_TBS_SUPPORTED)
         return CUPHY_STATUS_INVALID_ARGUMENT;
@@ -615,6 +622,7 @@ cuphyStatus_t CUPHYWINAPI cuphySetupPuschRxCrcDecode(cuphyPuschRxCrcDecodeHndl_t
                                                      cuphyPuschRxCrcDecodeLaunchCfg_t* pCbCrcLaunchCfg,
                                                      cuphyPuschRxCrcDecodeLaunchCfg_t* pTbCrcLaunchCfg, hipStream_t strm)
 {
+    CUPHY_HIP_RANGE("cuphySetupPuschRxCrcDecode");
     if (!puschRxCrcDecodeHndl || !pOutputCBCRCs || !pOutputTBs || !pInputCodeBlocks || !pOutputTBCRCs || !pTbPrmsCpu ||
         !pTbPrmsGpu || !pCpuDesc || !pGpuDesc || !pCbCrcLaunchCfg || !pTbCrcLaunchCfg || !pSchUserIdxsCpu ||
         nSchUes > MAX_N_TBS_SUPPORTED)
@@ -664,6 +672,7 @@ cuphyStatus_t CUPHYWINAPI cuphyHipLDPCEncodeCodeBlocks(int BG, int Z, const uint
                                                        int num_code_blocks, uint8_t* cw, int cw_stride,
                                                        hipStream_t strm)
 {
+    CUPHY_HIP_RANGE("cuphyHipLDPCEncodeCodeBlocks");
     return nrldpc::encode_cbs(BG, Z, info, info_stride, num_code_blocks, cw, cw_stride, strm);
 }
 
@@ -687,6 +696,7 @@ cuphyStatus_t CUPHYWINAPI cuphyHipLDPCBatchDecode(cuphyLDPCDecoder_t decoder, cu
                                                   int max_values_per_cb, int max_parity_nodes, void* d_workspace,
                                                   hipStream_t strm)
 {
+    CUPHY_HIP_RANGE("cuphyHipLDPCBatchDecode");
     if (!decoder || !d_workspace || max_parity_nodes < 4 || max_parity_nodes > 46) return CUPHY_STATUS_INVALID_ARGUMENT;
     if (max_values_per_cb <= 0 || max_values_per_cb > 68 * 384) return CUPHY_STATUS_INVALID_ARGUMENT;
     return nrldpc::decode_batch(d_items, nullptr, num_items, llr_type, fp16_math != 0, max_values_per_cb,
@@ -700,6 +710,7 @@ cuphyStatus_t CUPHYWINAPI cuphyHipLDPCBatchDecodeGroups(cuphyLDPCDecoder_t decod
                                                         int max_values_per_group, int max_parity_nodes,
                                                         int block_threads, void* d_workspace, hipStream_t strm)
 {
+    CUPHY_HIP_RANGE("cuphyHipLDPCBatchDecodeGroups");
     if (!decoder || !d_workspace || max_parity_nodes < 4 || max_parity_nodes > 46)
         return CUPHY_STATUS_INVALID_ARGUMENT;
     return nrldpc::decode_batch(d_items, d_group_first, num_groups, llr_type, fp16_math != 0, max_values_per_group,
@@ -719,6 +730,7 @@ cuphyStatus_t CUPHYWINAPI cuphyHipPuschSlotDecode(cuphyLDPCDecoder_t decoder, co
                                                   cuphyDataType_t llr_type, int max_lds_values, int max_parity_nodes,
                                                   int max_cbs_per_item, void* d_workspace, hipStream_t strm)
 {
+    CUPHY_HIP_RANGE("cuphyHipPuschSlotDecode");
     if (!decoder) return CUPHY_STATUS_INVALID_ARGUMENT;
     return nrldpc::slot_decode(d_tbs, num_tbs, d_cb_items, num_cb_items, llr_type, max_lds_values, max_parity_nodes,
                                max_cbs_per_item, d_workspace, decoder->cu_count, strm);
@@ -729,6 +741,7 @@ cuphyStatus_t CUPHYWINAPI cuphyHipPuschSlotDecodeNoAlloc(cuphyLDPCDecoder_t deco
                                                          int num_cbs, cuphyDataType_t llr_type, int max_lds_values,
                                                          int max_parity_nodes, int max_cbs_per_item, hipStream_t strm)
 {
+    CUPHY_HIP_RANGE("cuphyHipPuschSlotDecodeNoAlloc");
     if (!decoder) return CUPHY_STATUS_INVALID_ARGUMENT;
     return nrldpc::slot_decode_noalloc(d_tbs, num_tbs, d_cb_items, num_cb_items, num_cbs, llr_type, max_lds_values,
                                        max_parity_nodes, max_cbs_per_item, decoder->cu_count, strm);

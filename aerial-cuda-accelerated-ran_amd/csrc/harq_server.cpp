@@ -48,6 +48,7 @@
 #include <new>
 #include <vector>
 
+#include "trace.hpp"
 #include "api_objects.hpp"
 #include "cuphy_hip.h"
 #include "harq_pool.hpp"
@@ -277,6 +278,7 @@ void cuphyHipHarqServer::destroy()
 // previous launch.
 cuphyStatus_t cuphyHipHarqServer::start_resident()
 {
+    CUPHY_HIP_RANGE("harq.start_resident");
     // the launch's number marks it when its workgroups leave (collective idle exit)
     const uint32_t epoch = (uint32_t)(launches + 1);
     if (nrldpc::serve_resident_launch(dv_ring, dv_items, ring_size - 1, dv_jobs, (int)slot_open.size(), d_state, d_tbw,
@@ -532,6 +534,7 @@ cuphyStatus_t cuphyHipHarqServer::admit_and_launch()
 
 cuphyStatus_t cuphyHipHarqServer::launch(launch_slot& s)
 {
+    CUPHY_HIP_RANGE("harq.launch");
     const int n = (int)s.reqs.size();
     PerTbParams* h_params = reinterpret_cast<PerTbParams*>(s.h_stage + params_off());
     cuphyHipPuschSlotTb_t* h_jobs = reinterpret_cast<cuphyHipPuschSlotTb_t*>(s.h_stage + jobs_off());
@@ -726,6 +729,7 @@ cuphyStatus_t CUPHYWINAPI cuphyHipHarqServerDestroy(cuphyHipHarqServer_t server)
 
 cuphyStatus_t CUPHYWINAPI cuphyHipHarqServerSubmit(cuphyHipHarqServer_t server, const cuphyHipHarqRequest_t* req)
 {
+    CUPHY_HIP_RANGE("cuphyHipHarqServerSubmit");
     if (!server || !req) return CUPHY_STATUS_INVALID_ARGUMENT;
     const PerTbParams& p = req->prm;
     const int bg = (int)p.bg;
@@ -798,6 +802,7 @@ cuphyStatus_t CUPHYWINAPI cuphyHipHarqServeTrace(cuphyHipHarqServer_t server, co
                                                  int n, double harq_rtt_us, cuphyHipHarqCompletion_t* done,
                                                  int* n_done)
 {
+    CUPHY_HIP_RANGE("cuphyHipHarqServeTrace");
     if (!server || n < 0 || (n > 0 && (!trace || !done)) || !n_done || harq_rtt_us < 0.0)
         return CUPHY_STATUS_INVALID_ARGUMENT;
     for (int i = 0; i < n; ++i)

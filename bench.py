@@ -12,7 +12,8 @@ A step decodes the whole slot.  Forms:
   batched -- the whole slot in one persistent launch (when available)
 value = decoded info bits of the slot / step time, in Gbps, for the best
 form; every form's number is reported.  The CPU baseline decodes the same
-slot's code blocks with the oracle (binary32 mode) on the host.
+slot's code blocks, and configuration 1 (8 CBs BG1 Zc=384), with the oracle
+(binary32 mode, built -O3 -march=native on the host) on pinned host threads.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -37,6 +38,11 @@ LDS_BYTES_PER_EDGE = 4
 # LDS bandwidth: 256 CUs x 128 B per clock x 2.4 GHz (nominal)
 LDS_PEAK_TBS = 256 * 128 * 2.4e9 / 1e12
 ITERS = 10
+
+
+def progress(msg):
+    """One progress line on stderr (the JSON result stays alone on stdout)."""
+    print("bench: " + msg, file=sys.stderr, flush=True)
 
 
 def parse():
@@ -187,46 +193,95 @@ def serving_section(dec, serve_ms, config3_gbps=None, inflight=3, loads=(0.6, 0.
     return out
 
 
-def cpu_baseline(tbs, budget_s):
-    """Oracle (binary32 layered decoder, C) over the slot's code blocks, on
-    the host cores, one thread per core up to 16.  Bounded: as many whole
-    passes over the slot as fit in the CPU-work budget (at least one)."""
+def native_oracle():
+    """The oracle built -O3 -march=native for the host this runs on (make
+    -C oracle native, into a temporary directory): its path and the flags,
+    or (None, reason) when the build fails."""
+    import subprocess
+    import tempfile
+    d = tempfile.mkdtemp(prefix="nr_oracle_native_")
+    r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "native", "NATIVE_DIR=" + d],
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        return None, (r.stderr or r.stdout)[-300:]
+    return os.path.join(d, "libnr_oracle.so"), "gcc -O3 -march=native -ffp-contract=off"
+
+
+def host_cpu():
+    """(lscpu model name, logical CPUs this process may use)."""
+    import subprocess
+    model = ""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        model = next((ln.split(":", 1)[1].strip() for ln in out.splitlines() if ln.startswith("Model name")), "")
+    except (OSError, subprocess.SubprocessError):
+        pass
+    if not model:
+        try:
+            with open("/proc/cpuinfo") as f:
+                model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+        except OSError:
+            pass
+    return model, len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(tbs, c1, c1_gpu, budget_s):
+    """The oracle (binary32 layered normalized min-sum, C, built -O3
+    -march=native on this host) on the host cores, threads pinned one per
+    logical CPU, 2 untimed warm-up passes then >= 20 timed ones, median.
+
+    value: the slot's code blocks (all TBs' code blocks from one queue) on
+    min(16, CPUs) threads, in decoded info-bit Gbps.  config1: 1 TB x 8
+    CBs BG1 Zc=384 on 8 threads and on 1 thread, gated on its hard decisions
+    being bit-identical to the GPU's binary32 plain-form decode (c1.out)."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    ncores = min(16, os.cpu_count() or 1)
-    jobs = []
+    path, flags = native_oracle()
+    if path is None:
+        O.lib()
+        flags = "portable build (-O3 -march=x86-64-v2); native build failed: " + flags
+    else:
+        O.use_library(path)
+    model, nproc = host_cpu()
+    ncores = min(16, nproc)
+
+    cfgs, llrs = [], []
     for tb in tbs:
-        cfg = O.dec_cfg(tb.bg, tb.Z, tb.mb, n_iter=ITERS, alpha=0.8125, clamp=32.0, mode=O.FP32)
-        llr = tb.llr.float().cpu().numpy()[:, :tb.num_llrs].copy()
-        jobs.append((cfg, llr, tb.ncb * tb.info_bits_per_cb))
-    bits_per_pass = sum(j[2] for j in jobs)
-    passes, t_total = 0, 0.0
-    while True:
-        t0 = time.perf_counter()
-        for cfg, llr, _ in jobs:
-            O.decode_cbs(cfg, llr, nthreads=ncores)
-        t_total += time.perf_counter() - t0
-        passes += 1
-        if t_total * ncores >= budget_s or passes >= 50:
-            break
-    gbps = bits_per_pass * passes / t_total / 1e9
-    # one pass on one core, for the per-core rate
-    t0 = time.perf_counter()
-    for cfg, llr, _ in jobs:
-        O.decode_cbs(cfg, llr, nthreads=1)
-    gbps_1 = bits_per_pass / (time.perf_counter() - t0) / 1e9
-    model = ""
-    try:
-        with open("/proc/cpuinfo") as f:
-            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
-    except OSError:
-        pass
-    return {"value": round(gbps, 4), "unit": "Gbps", "cores": ncores, "kind": "port",
-            "value_1_core": round(gbps_1, 5), "cpu_model": model,
-            "sample": "%d pass(es) over the slot's %d code blocks (%d info bits/pass), oracle binary32 "
-                      "layered NMS, 10 iterations, one thread per core" %
-                      (passes, sum(tb.ncb for tb in tbs), bits_per_pass)}
+        cfgs.append(O.dec_cfg(tb.bg, tb.Z, tb.mb, n_iter=ITERS, alpha=0.8125, clamp=32.0, mode=O.FP32))
+        llrs.append(tb.llr.float().cpu().numpy()[:, :tb.num_llrs].copy())
+    bits_per_pass = sum(tb.ncb * tb.info_bits_per_cb for tb in tbs)
+    t_one, _ = O.time_slot(cfgs, llrs, ncores, warmup=1, reps=1)
+    reps = int(min(200, max(20, budget_s / max(1e-6, t_one[0] * ncores))))
+    times, _ = O.time_slot(cfgs, llrs, ncores, warmup=2, reps=reps)
+    t_med = float(np.median(times))
+    times_1, _ = O.time_slot(cfgs, llrs, 1, warmup=1, reps=3)
+    out = {"value": round(bits_per_pass / t_med / 1e9, 4), "unit": "Gbps", "cores": ncores, "kind": "port",
+           "value_1_core": round(bits_per_pass / float(np.median(times_1)) / 1e9, 5),
+           "cpu_model": model, "nproc": nproc, "build": flags,
+           "sample": "%d timed passes (median; 2 warm-up) over the slot's %d code blocks (%d info bits/pass), "
+                     "oracle binary32 layered NMS, 10 iterations, %d pinned threads; 1-core: median of 3 passes"
+                     % (reps, sum(tb.ncb for tb in tbs), bits_per_pass, ncores)}
+    if c1 is not None:
+        c1cfg = O.dec_cfg(1, 384, 46, n_iter=ITERS, alpha=0.8125, clamp=32.0, mode=O.FP32)
+        c1llr = c1.llr.float().cpu().numpy()[:, :c1.num_llrs].copy()
+        t8, hard8 = O.time_decode(c1cfg, c1llr, 8, warmup=2, reps=20)
+        t1, hard1 = O.time_decode(c1cfg, c1llr, 1, warmup=2, reps=20)
+        gpu_words = c1.out.cpu().numpy().view(np.uint32)[:, :hard8.shape[1]]
+        identical = bool((hard8 == gpu_words).all() and (hard1 == gpu_words).all())
+        g8 = 8 * 8448 / float(np.median(t8)) / 1e9
+        g1 = 8 * 8448 / float(np.median(t1)) / 1e9
+        out["config1"] = {"gbps_8_threads": round(g8, 5), "gbps_1_thread": round(g1, 5),
+                          "median_ms_8_threads": round(float(np.median(t8)) * 1e3, 3),
+                          "median_ms_1_thread": round(float(np.median(t1)) * 1e3, 3),
+                          "gpu_plain_gbps": c1_gpu["gbps"] if c1_gpu else None,
+                          "gpu_over_cpu_8_threads": round(c1_gpu["gbps"] / g8, 1) if c1_gpu else None,
+                          "hard_decisions_identical_to_gpu": identical,
+                          "sample": "1 TB x 8 CBs BG1 Zc=384 (67584 info bits), binary32, 10 iterations; "
+                                    "2 warm-up + 20 timed decodes each, median"}
+        if not identical:
+            raise SystemExit("bench.py: CPU and GPU binary32 decodes of configuration 1 differ")
+    return out
 
 
 def oracle_slot_check(tbs):
@@ -402,20 +457,25 @@ def main():
                 "lds": lds_fraction(edge_updates, batched_ms),
                 "plain_kernel": {"achieved": round(plain_tops, 3), "avg_launch_us": round(kernel_ms * 1e3 / len(descs), 2)}}
 
-    config1 = None
+    progress("main slot done: batched %.3f ms, plain %.3f ms" % (batched_ms, plain_ms))
+    config1, c1 = None, None
     if "config1" in sections:
-        config1 = config1_section(dec, stream)
+        config1, c1 = config1_section(dec, stream)
     config3 = None
     if "config3" in sections:
+        progress("config1 done")
         config3 = config3_section(dec, stream, a.steps, with_oracle=not a.no_cpu)
     throughput = None
     if "throughput" in sections:
+        progress("config3 done")
         throughput = throughput_section(dec, stream, a.steps)
     fused = None
     if "fused" in sections:
+        progress("throughput done")
         fused = fused_slot_section(dec, a.steps)
     serving = None
     if "serving" in sections:
+        progress("fused slot done")
         c3g = config3["batched_gbps"] if config3 else None
         c3c = config3["code_blocks"] / (config3["batched_ms"] * 1e-3) if config3 else None
         serving = serving_section(dec, a.serve_ms, c3g, loads=[float(x) for x in a.serve_loads.split(",")],
@@ -425,7 +485,9 @@ def main():
         batched.pop("templates")
         serving["batched_launches_for_comparison"] = batched
 
-    cpu = None if a.no_cpu else cpu_baseline(tbs, a.cpu_seconds)
+    if not a.no_cpu:
+        progress("GPU sections done; CPU baseline")
+    cpu = None if a.no_cpu else cpu_baseline(tbs, c1, config1, a.cpu_seconds)
     return line_of(a, locals())
 
 
@@ -451,7 +513,7 @@ def config1_section(dec, stream):
     torch.cuda.synchronize()
     c1_us = e0.elapsed_time(e1) * 1e3 / n_c1
     c1_gbps = 8 * 8448 / (c1_us * 1e-6) / 1e9
-    return {"us_per_tb": round(c1_us, 2), "gbps": round(c1_gbps, 4)}
+    return {"us_per_tb": round(c1_us, 2), "gbps": round(c1_gbps, 4)}, c1
 
 
 def config3_section(dec, stream, steps, with_oracle=True):

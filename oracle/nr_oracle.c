@@ -21,6 +21,7 @@
 #include <sched.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include "../aerial-cuda-accelerated-ran_amd/csrc/nr_bg_tables.h"
 
@@ -695,6 +696,23 @@ int nr_orc_crc_desegment(const uint32_t* cb_words, int C, int Kp,
  * reads llr + i*llr_stride and writes hard + i*hard_stride; block i goes to
  * thread i % nthreads.
  * ==================================================================== */
+/* the calling thread on the tid-th logical CPU this process may use */
+static void pin_to(int tid)
+{
+    cpu_set_t allowed, one;
+    if (sched_getaffinity(0, sizeof(allowed), &allowed) != 0) return;
+    int n = 0;
+    for (int c = 0; c < CPU_SETSIZE; ++c) {
+        if (!CPU_ISSET(c, &allowed)) continue;
+        if (n++ == tid) {
+            CPU_ZERO(&one);
+            CPU_SET(c, &one);
+            pthread_setaffinity_np(pthread_self(), sizeof(one), &one);
+            return;
+        }
+    }
+}
+
 typedef struct {
     const nr_orc_dec_cfg* cfg;
     const float* llr;
@@ -708,21 +726,7 @@ typedef struct {
 static void* par_worker(void* p)
 {
     par_job_t* j = (par_job_t*)p;
-    if (j->pin) { /* thread t on logical CPU t of the ones this process may use */
-        cpu_set_t allowed, one;
-        if (sched_getaffinity(0, sizeof(allowed), &allowed) == 0) {
-            int n = 0;
-            for (int c = 0; c < CPU_SETSIZE; ++c) {
-                if (!CPU_ISSET(c, &allowed)) continue;
-                if (n++ == j->tid) {
-                    CPU_ZERO(&one);
-                    CPU_SET(c, &one);
-                    pthread_setaffinity_np(pthread_self(), sizeof(one), &one);
-                    break;
-                }
-            }
-        }
-    }
+    if (j->pin) pin_to(j->tid);
     for (int i = j->tid; i < j->ncb; i += j->nthreads) {
         const int it = nr_orc_decode_cb(j->cfg, j->llr + (size_t)i * j->llr_stride,
                                         j->hard ? j->hard + (size_t)i * j->hard_stride : NULL, NULL);
@@ -750,4 +754,95 @@ int nr_orc_decode_cbs(const nr_orc_dec_cfg* cfg, const float* llr, size_t llr_st
                       uint32_t* hard, size_t hard_stride, int nthreads)
 {
     return nr_orc_decode_cbs_it(cfg, llr, llr_stride, ncb, hard, hard_stride, NULL, nthreads, 0);
+}
+
+/* ======================================================================
+ * a slot of transport blocks (one decoder configuration each) on several
+ * host threads: the code blocks of every TB form one queue, handed out one
+ * at a time (an atomic counter), so that threads stay busy across TBs of
+ * different sizes.
+ * ==================================================================== */
+typedef struct {
+    const nr_orc_dec_cfg* cfgs;
+    const float* const* llrs;
+    const size_t* llr_strides;
+    const int* first; /* ntb + 1 prefix of code-block counts */
+    uint32_t* const* hards;
+    const size_t* hard_strides;
+    int ntb;
+    int next; /* atomic */
+} slot_queue_t;
+
+typedef struct {
+    slot_queue_t* q;
+    int tid, pin;
+} slot_job_t;
+
+static void* slot_worker(void* p)
+{
+    slot_job_t* j = (slot_job_t*)p;
+    slot_queue_t* q = j->q;
+    if (j->pin) pin_to(j->tid);
+    const int total = q->first[q->ntb];
+    int t = 0;
+    for (;;) {
+        const int i = __atomic_fetch_add(&q->next, 1, __ATOMIC_RELAXED);
+        if (i >= total) break;
+        while (q->first[t + 1] <= i) ++t; /* i only grows within a thread */
+        const int r = i - q->first[t];
+        nr_orc_decode_cb(&q->cfgs[t], q->llrs[t] + (size_t)r * q->llr_strides[t],
+                         q->hards[t] ? q->hards[t] + (size_t)r * q->hard_strides[t] : NULL, NULL);
+    }
+    return NULL;
+}
+
+int nr_orc_time_slot(const nr_orc_dec_cfg* cfgs, const float* const* llrs, const size_t* llr_strides,
+                     const int* ncbs, int ntb, uint32_t* const* hards, const size_t* hard_strides, int nthreads,
+                     int warmup, int reps, double* times)
+{
+    if (ntb < 1) return -1;
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    int* first = (int*)malloc(sizeof(int) * (size_t)(ntb + 1));
+    if (!first) return -1;
+    first[0] = 0;
+    for (int t = 0; t < ntb; ++t) first[t + 1] = first[t] + ncbs[t];
+    slot_queue_t q = {cfgs, llrs, llr_strides, first, hards, hard_strides, ntb, 0};
+    pthread_t th[256];
+    slot_job_t jobs[256];
+    int rc = 0;
+    for (int i = 0; i < warmup + reps && rc == 0; ++i) {
+        struct timespec a, b;
+        q.next = 0;
+        clock_gettime(CLOCK_MONOTONIC, &a);
+        int started = 0;
+        for (; started < nthreads; ++started) {
+            jobs[started] = (slot_job_t){&q, started, 1};
+            if (pthread_create(&th[started], NULL, slot_worker, &jobs[started]) != 0) {
+                rc = -1;
+                break;
+            }
+        }
+        for (int t = 0; t < started; ++t) pthread_join(th[t], NULL);
+        clock_gettime(CLOCK_MONOTONIC, &b);
+        if (i >= warmup) times[i - warmup] = (double)(b.tv_sec - a.tv_sec) + 1e-9 * (double)(b.tv_nsec - a.tv_nsec);
+    }
+    free(first);
+    return rc;
+}
+
+/* CPU-baseline timing: warmup + reps decodes of the ncb code blocks on
+ * nthreads pinned threads; seconds of each timed decode (monotonic clock)
+ * into times[reps].  Returns 0, or -1 if a thread could not start. */
+int nr_orc_time_decode(const nr_orc_dec_cfg* cfg, const float* llr, size_t llr_stride, int ncb, uint32_t* hard,
+                       size_t hard_stride, int nthreads, int warmup, int reps, double* times)
+{
+    for (int i = 0; i < warmup + reps; ++i) {
+        struct timespec a, b;
+        clock_gettime(CLOCK_MONOTONIC, &a);
+        if (nr_orc_decode_cbs_it(cfg, llr, llr_stride, ncb, hard, hard_stride, NULL, nthreads, 1) != 0) return -1;
+        clock_gettime(CLOCK_MONOTONIC, &b);
+        if (i >= warmup) times[i - warmup] = (double)(b.tv_sec - a.tv_sec) + 1e-9 * (double)(b.tv_nsec - a.tv_nsec);
+    }
+    return 0;
 }

@@ -102,6 +102,19 @@ int nr_orc_decode_cbs(const nr_orc_dec_cfg* cfg, const float* llr, size_t llr_st
 int nr_orc_decode_cbs_it(const nr_orc_dec_cfg* cfg, const float* llr, size_t llr_stride, int ncb,
                          uint32_t* hard, size_t hard_stride, int* iters, int nthreads, int pin);
 
+/* CPU-baseline timing of a slot: TB t (configuration cfgs[t], ncbs[t] code
+ * blocks at llrs[t] + r*llr_strides[t], hard decisions to hards[t] (or
+ * NULL)); all code blocks on nthreads pinned threads from one queue;
+ * warmup + reps passes, seconds of each timed pass into times[reps] */
+int nr_orc_time_slot(const nr_orc_dec_cfg* cfgs, const float* const* llrs, const size_t* llr_strides,
+                     const int* ncbs, int ntb, uint32_t* const* hards, const size_t* hard_strides, int nthreads,
+                     int warmup, int reps, double* times);
+
+/* CPU-baseline timing: warmup + reps decodes of the ncb code blocks on
+ * nthreads pinned threads, seconds of each timed decode into times[reps] */
+int nr_orc_time_decode(const nr_orc_dec_cfg* cfg, const float* llr, size_t llr_stride, int ncb, uint32_t* hard,
+                       size_t hard_stride, int nthreads, int warmup, int reps, double* times);
+
 /* one check-node update (compute_cC2V.m + update_C2V.m) on d inputs t[] in
  * the given mode; c2v[j] = sign * (j == argmin ? min2 : min1), unnormalised */
 void nr_orc_check_node(int d, const float* t, float* c2v, int mode);

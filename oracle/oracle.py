@@ -82,8 +82,56 @@ def lib():
         L.nr_orc_derate_match_fp16.argtypes = [P(RmCfg), C.c_void_p, C.c_void_p]
         L.nr_orc_k0.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
         L.nr_orc_crc_desegment.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.nr_orc_time_decode.argtypes = [P(DecCfg), C.c_void_p, C.c_size_t, C.c_int, C.c_void_p, C.c_size_t,
+                                         C.c_int, C.c_int, C.c_int, C.c_void_p]
+        L.nr_orc_time_slot.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                       C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]
         _lib = L
     return _lib
+
+
+def use_library(path):
+    """Load the oracle from another build of the same sources (the CPU
+    baseline's -march=native build); later calls go to it."""
+    global _lib, LIB_PATH
+    _lib = None
+    LIB_PATH = path
+    return lib()
+
+
+def time_decode(cfg, llr2d, nthreads, warmup=2, reps=20):
+    """Seconds of each of reps pinned decodes of the rows of llr2d (after
+    warmup untimed ones) and the last decode's hard words."""
+    llr2d = np.ascontiguousarray(llr2d, np.float32)
+    ncb = llr2d.shape[0]
+    nw = (cfg.kb_out * cfg.Z + 31) // 32
+    hard = np.zeros((ncb, nw), np.uint32)
+    times = np.zeros(reps, np.float64)
+    rc = lib().nr_orc_time_decode(C.byref(cfg), _p(llr2d), llr2d.shape[1], ncb, _p(hard), nw, nthreads, warmup,
+                                  reps, _p(times))
+    assert rc == 0
+    return times, hard
+
+
+def time_slot(cfgs, llrs, nthreads, warmup=2, reps=20):
+    """Seconds of each of reps pinned passes over a slot: TB t's code blocks
+    are the rows of llrs[t], decoded with cfgs[t]; all code blocks share one
+    queue over nthreads threads.  Also returns each TB's hard words."""
+    n = len(cfgs)
+    llrs = [np.ascontiguousarray(x, np.float32) for x in llrs]
+    hards = [np.zeros((x.shape[0], (c.kb_out * c.Z + 31) // 32), np.uint32) for c, x in zip(cfgs, llrs)]
+    cfg_arr = (DecCfg * n)(*cfgs)
+    llr_p = (C.c_void_p * n)(*[x.ctypes.data for x in llrs])
+    llr_s = (C.c_size_t * n)(*[x.shape[1] for x in llrs])
+    ncbs = (C.c_int * n)(*[x.shape[0] for x in llrs])
+    hard_p = (C.c_void_p * n)(*[h.ctypes.data for h in hards])
+    hard_s = (C.c_size_t * n)(*[h.shape[1] for h in hards])
+    times = np.zeros(reps, np.float64)
+    rc = lib().nr_orc_time_slot(C.cast(cfg_arr, C.c_void_p), C.cast(llr_p, C.c_void_p), C.cast(llr_s, C.c_void_p),
+                                C.cast(ncbs, C.c_void_p), n, C.cast(hard_p, C.c_void_p),
+                                C.cast(hard_s, C.c_void_p), nthreads, warmup, reps, _p(times))
+    assert rc == 0
+    return times, hards
 
 
 def _p(a):

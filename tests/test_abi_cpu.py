@@ -45,6 +45,18 @@ def test_library_exports_every_declared_symbol(L):
     assert not missing, missing
 
 
+def test_api_phases_carry_roctx_ranges(L):
+    """The library calls roctx (range push/pop around the API's phases, the
+    counterpart of the reference's NVTX PUSH_RANGE/POP_RANGE) and links the
+    roctx library, so that rocprofv3 --marker-trace shows the ranges."""
+    import subprocess
+    path = L.lib()._name
+    undef = subprocess.run(["nm", "-D", "--undefined-only", path], capture_output=True, text=True, check=True).stdout
+    assert "roctxRangePushA" in undef and "roctxRangePop" in undef
+    needed = subprocess.run(["readelf", "-d", path], capture_output=True, text=True, check=True).stdout
+    assert "librocprofiler-sdk-roctx" in needed
+
+
 def test_every_declared_symbol_has_a_python_signature():
     from aerial_amd import _lib
     missing = [n for n in header_entry_points() if n not in _lib.SIGNATURES]

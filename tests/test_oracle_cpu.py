@@ -215,6 +215,22 @@ def test_early_termination_stops_on_valid_codeword():
         assert np.array_equal(hard, h_full)  # stopping == running exactly `it` iterations
 
 
+def test_timed_decodes_equal_single_block_decodes():
+    """The CPU baseline's timing entry points (pinned threads, one queue over
+    the code blocks of several TBs) return what one-block decodes return."""
+    a, _ = V.make_cbs(1, 64, 6, 3, seed=21, snr_db=1.0)
+    b, _ = V.make_cbs(2, 96, 10, 5, seed=22, snr_db=0.5)
+    ca, cb = O.dec_cfg(1, 64, 6), O.dec_cfg(2, 96, 10)
+    want_a = np.stack([O.decode_cb(ca, x)[0] for x in a])
+    want_b = np.stack([O.decode_cb(cb, x)[0] for x in b])
+    times, hard = O.time_decode(cb, b, nthreads=3, warmup=1, reps=2)
+    assert times.shape == (2,) and (times > 0).all()
+    assert np.array_equal(hard, want_b)
+    times, hards = O.time_slot([ca, cb], [a, b], nthreads=4, warmup=0, reps=2)
+    assert (times > 0).all()
+    assert np.array_equal(hards[0], want_a) and np.array_equal(hards[1], want_b)
+
+
 def test_binary16_rounding_matches_ieee():
     rng = np.random.RandomState(0)
     xs = np.concatenate([rng.standard_normal(4000) * 10 ** rng.uniform(-8, 5, 4000),
