@@ -85,15 +85,16 @@ template <int BG> constexpr uint64_t kBarrierMask = barrier_mask<bg_traits<BG>>(
 // time.  Each row goes to one group: the rows of a barrier segment
 // alternate between the groups, and a segment's first row goes to the
 // group holding fewer edges so far (the per-lane message state is then
-// about half the graph's).  Within its group a row's edges take
-// consecutive message slots from edge_slot[r] and its compressed row state
-// the slot row_slot[r].  Rows r < NR only.
+// about half the graph's).  Within its group a row's binary16 messages take
+// consecutive slots from pair_slot[r] (rows padded to even degree) and its
+// compressed binary32 row state the slot row_slot[r].  Rows r < NR only.
 template <int BG, int NR> struct pair_plan {
     int group[64];
-    int edge_slot[64];
     int row_slot[64];
+    int pair_slot[64];  // binary16: first message slot of the row (even; rows padded to even degree)
     int edges[2];
     int rows[2];
+    int padded_edges[2];
 };
 template <int BG, int NR> constexpr pair_plan<BG, NR> make_pair_plan()
 {
@@ -104,15 +105,26 @@ template <int BG, int NR> constexpr pair_plan<BG, NR> make_pair_plan()
         const bool first = (r == 0) || ((kBarrierMask<BG> >> r) & 1ull);
         const int g = first ? (p.edges[1] < p.edges[0] ? 1 : 0) : 1 - prev;
         p.group[r] = g;
-        p.edge_slot[r] = p.edges[g];
         p.row_slot[r] = p.rows[g];
+        p.pair_slot[r] = p.padded_edges[g];
         p.edges[g] += BGT::row_start(r + 1) - BGT::row_start(r);
+        p.padded_edges[g] += (BGT::row_start(r + 1) - BGT::row_start(r) + 1) & ~1;
         p.rows[g] += 1;
         prev = g;
     }
     return p;
 }
 template <int BG, int NR> constexpr pair_plan<BG, NR> kPairPlan = make_pair_plan<BG, NR>();
+
+// binary16 message slots: row r's edges from padded_row_start(r) on, every
+// row's degree rounded up to even, so that each pair of a row's edges is one
+// 32-bit register of packed halves.
+template <int BG> constexpr int padded_row_start(int r)
+{
+    int s = 0;
+    for (int i = 0; i < r; ++i) s += (bg_traits<BG>::row_start(i + 1) - bg_traits<BG>::row_start(i) + 1) & ~1;
+    return s;
+}
 
 __device__ __forceinline__ uint32_t f2u(float x) { return __builtin_bit_cast(uint32_t, x); }
 __device__ __forceinline__ float u2f(uint32_t x) { return __builtin_bit_cast(float, x); }
@@ -238,51 +250,103 @@ __device__ __forceinline__ void process_row(float*, BP __restrict__ base, shift_
 }
 
 // ---------------------------------------------------------------------------
-// binary16 rows: the normalised C2V message of every edge kept explicitly,
-// two edges per VGPR (BG1: 158 registers).  Per edge: address 3, subtract
-// 1, min/min2/parity 3, new message + write-back 5 VALU ops; no argmin.
+// binary16 rows, two edges per 32-bit register (packed halves): the
+// normalised C2V message of every edge kept explicitly (edges 2k, 2k+1 of a
+// row in one register; a row of odd degree ends with half a pair whose
+// upper half is a dummy edge at +inf).  Per pair of edges:
+//   t   = APP - C2V                                     (packed subtract)
+//   min1/min2 running over |t|: min2 = min(min2, max(min1, |t|)),
+//   min1 = min(min1, |t|), two independent sets of accumulators on long rows
+//   parity ^= t (sign bits)
+// then per row the two halves (and accumulator sets) are merged, both halves
+// of min1/min2 holding the row's values, and per pair
+//   g     = min(|t|, min2)            (= min1 exactly for the argmin edge,
+//                                      min2 for every other edge)
+//   other = (min1 + min2) - g         (16-bit integer arithmetic on the bit
+//                                      patterns: the magnitude not taken)
+//   C2V   = other * (+-alpha)         (alpha's sign = sign(t) ^ parity)
+//   APP   = t + C2V
+// which is the extrinsic min-sum update with the first-index rule on ties
+// (tied minima give min1 == min2) and the same binary16 roundings as
+// alpha * min1 / alpha * min2 with the sign attached: the product's
+// magnitude does not depend on the sign under round-to-nearest.
+// No argmin tracking, no compare/select per edge.
 // ---------------------------------------------------------------------------
-// S0: message slot of the row's first edge (its edge index, or its slot
-// within a row-pair lane group).
-template <int BG, int R, int NW, int S0 = bg_traits<BG>::row_start(R), bool LM = false,
-          typename BP = unsigned char*>
+typedef uint16_t u16x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ half2_t u2h2(uint32_t x) { return __builtin_bit_cast(half2_t, x); }
+__device__ __forceinline__ uint32_t h22u(half2_t x) { return __builtin_bit_cast(uint32_t, x); }
+__device__ __forceinline__ half2_t hmin2(half2_t a, half2_t b) { return __builtin_elementwise_min(a, b); }
+__device__ __forceinline__ half2_t hmax2(half2_t a, half2_t b) { return __builtin_elementwise_max(a, b); }
+
+// S0: message slot of the row's first edge (even: padded_row_start(R), or
+// the row's pair_slot within a row-pair lane group).
+template <int BG, int R, int NW, int S0 = padded_row_start<BG>(R), bool LM = false, typename BP = unsigned char*>
 __device__ __forceinline__ void process_row(_Float16*, BP __restrict__ base, shift_ptr sh, uint32_t zb,
                                             uint32_t Zb, _Float16 alpha, half2_t (&st)[NW], uint32_t Pb = 0)
 {
     using BGT = bg_traits<BG>;
     constexpr int E0 = BGT::row_start(R);
     constexpr int D = BGT::row_start(R + 1) - E0;
-    _Float16 t[D];
+    constexpr int P = (D + 1) / 2;
+    static_assert(S0 % 2 == 0, "binary16 message slots of a row start at an even slot");
+    constexpr int W0 = S0 / 2;
+    constexpr bool TWO = P >= 4;  // two accumulator sets: shorter dependency chains
+    constexpr uint32_t SIGNS = 0x80008000u;
     uint32_t off[D];
-    static_for<D>([&](auto J) {
-        constexpr int j = decltype(J)::value;
-        constexpr int e = E0 + j;
-        constexpr int sl = S0 + j;
-        off[j] = edge_off<BG, e, _Float16, LM>(sh, zb, Zb, Pb);
-        const _Float16 old = (sl & 1) ? st[sl >> 1].y : st[sl >> 1].x;
-        t[j] = lds_ld<_Float16>(base, off[j]) - old;
-    });
-    _Float16 m1 = (_Float16)__builtin_huge_valf(), m2 = (_Float16)__builtin_huge_valf();
+    half2_t t[P];
+    const half2_t inf2 = u2h2(0x7C007C00u);
+    half2_t m1a = inf2, m2a = inf2, m1b = inf2, m2b = inf2;
     uint32_t par = 0;
-    static_for<D>([&](auto J) {
-        constexpr int j = decltype(J)::value;
-        const _Float16 a = __builtin_fabsf16(t[j]);
-        m2 = __builtin_fminf16(m2, __builtin_fmaxf16(m1, a));  // as in the binary32 row
-        m1 = __builtin_fminf16(m1, a);
-        par ^= h2u(t[j]);
+    static_for<P>([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        constexpr int j0 = 2 * k, j1 = 2 * k + 1;
+        off[j0] = edge_off<BG, E0 + j0, _Float16, LM>(sh, zb, Zb, Pb);
+        uint32_t v = lds_ld<uint16_t>(base, off[j0]);
+        if constexpr (j1 < D) {
+            off[j1] = edge_off<BG, E0 + j1, _Float16, LM>(sh, zb, Zb, Pb);
+            v |= (uint32_t)lds_ld<uint16_t>(base, off[j1]) << 16;
+        } else {
+            v |= 0x7C000000u;  // the dummy edge: +inf, sign bit clear
+        }
+        t[k] = u2h2(v) - st[W0 + k];
+        const half2_t a = __builtin_elementwise_abs(t[k]);
+        if (k < (TWO ? 2 : 1)) {  // a set's first pair: min1 = |t|, min2 = +inf
+            if (TWO && (k & 1))
+                m1b = a;
+            else
+                m1a = a;
+        } else if (TWO && (k & 1)) {
+            m2b = hmin2(m2b, hmax2(m1b, a));
+            m1b = hmin2(m1b, a);
+        } else {
+            m2a = hmin2(m2a, hmax2(m1a, a));
+            m1a = hmin2(m1a, a);
+        }
+        par ^= h22u(t[k]);
     });
-    par &= 0x8000u;
-    const _Float16 a1 = alpha * m1, a2 = alpha * m2;
-    static_for<D>([&](auto J) {
-        constexpr int j = decltype(J)::value;
-        constexpr int sl = S0 + j;
-        const _Float16 mag = (__builtin_fabsf16(t[j]) == m1) ? a2 : a1;
-        const _Float16 c = u2h((uint16_t)(h2u(mag) | ((h2u(t[j]) ^ par) & 0x8000u)));
-        lds_st<_Float16>(base, off[j], (_Float16)(t[j] + c));
-        if (sl & 1)
-            st[sl >> 1].y = c;
-        else
-            st[sl >> 1].x = c;
+    if constexpr (TWO) {
+        const half2_t m1 = hmin2(m1a, m1b);
+        m2a = hmin2(hmin2(m2a, m2b), hmax2(m1a, m1b));
+        m1a = m1;
+    }
+    // both halves: the row's min1 / min2, and the parity in bits 15 and 31
+    const half2_t m1s = m1a.yx, m2s = m2a.yx;
+    const half2_t M1 = hmin2(m1a, m1s);
+    const half2_t M2 = hmin2(hmin2(m2a, m2s), hmax2(m1a, m1s));
+    const uint32_t px = par ^ __builtin_amdgcn_alignbit(par, par, 16);
+    const uint32_t Q = h22u(half2_t{alpha, alpha}) ^ (px & SIGNS);
+    const u16x2_t S = __builtin_bit_cast(u16x2_t, M1) + __builtin_bit_cast(u16x2_t, M2);
+    static_for<P>([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        constexpr int j0 = 2 * k, j1 = 2 * k + 1;
+        const half2_t g = hmin2(__builtin_elementwise_abs(t[k]), M2);
+        const half2_t other = __builtin_bit_cast(half2_t, (u16x2_t)(S - __builtin_bit_cast(u16x2_t, g)));
+        const half2_t sa = u2h2(Q ^ (h22u(t[k]) & SIGNS));
+        const half2_t c = other * sa;
+        const uint32_t nv = h22u(t[k] + c);
+        lds_st<uint16_t>(base, off[j0], (uint16_t)nv);
+        if constexpr (j1 < D) lds_st<uint16_t>(base, off[j1], (uint16_t)(nv >> 16));
+        st[W0 + k] = c;
     });
 }
 
@@ -298,8 +362,12 @@ __device__ __forceinline__ uint32_t row_syndrome(BP __restrict__ base, shift_ptr
     static_for<D>([&](auto J) {
         constexpr int j = decltype(J)::value;
         const uint32_t off = edge_off<BG, E0 + j, T, LM>(sh, zb, Zb, Pb);
-        par ^= (uint32_t)(lds_ld<T>(base, off) < (T)0);
+        if constexpr (std::is_same<T, _Float16>::value)
+            par ^= (uint32_t)lds_ld<uint16_t>(base, off);  // sign bits (APP values are never -0)
+        else
+            par ^= (uint32_t)(lds_ld<T>(base, off) < (T)0);
     });
+    if constexpr (std::is_same<T, _Float16>::value) par >>= 15;
     return par;
 }
 
@@ -382,8 +450,11 @@ __device__ __forceinline__ int run_iterations(T* __restrict__ app, const cb_job&
 
     // C2V state: compressed per row (binary32) or explicit per edge (binary16)
     constexpr int NRS = (PAIR == 2) ? (PP.rows[0] > PP.rows[1] ? PP.rows[0] : PP.rows[1]) : NR;
-    constexpr int NEDGE = (PAIR == 2) ? (PP.edges[0] > PP.edges[1] ? PP.edges[0] : PP.edges[1]) : BGT::row_start(NR);
-    constexpr int NW = (NEDGE + 1) / 2;
+    // binary16 message slots: every row padded to an even number of edges
+    constexpr int NEDGE = (PAIR == 2) ? (PP.padded_edges[0] > PP.padded_edges[1] ? PP.padded_edges[0]
+                                                                                  : PP.padded_edges[1])
+                                      : padded_row_start<BG>(NR);
+    constexpr int NW = NEDGE / 2;
     float st_n1[std::is_same<T, float>::value ? NRS : 1], st_n2[std::is_same<T, float>::value ? NRS : 1];
     uint32_t st_meta[std::is_same<T, float>::value ? NRS : 1];
     half2_t st_h[std::is_same<T, float>::value ? 1 : NW];
@@ -448,7 +519,7 @@ __device__ __forceinline__ int run_iterations(T* __restrict__ app, const cb_job&
                 asm volatile("" : "+s"(shr), "+s"(Zr));
                 if (active && (PAIR == 1 || lgrp == (uint32_t)PP.group[R])) {
                     constexpr int RS = (PAIR == 2) ? PP.row_slot[R] : R;
-                    constexpr int S0 = (PAIR == 2) ? PP.edge_slot[R] : BGT::row_start(R);
+                    constexpr int S0 = (PAIR == 2) ? PP.pair_slot[R] : padded_row_start<BG>(R);
                     if constexpr (std::is_same<T, float>::value)
                         process_row<BG, R, NRS, RS, LM>(app, lane_base, shr, zb, Zr, alpha, st_n1, st_n2, st_meta, Pb);
                     else
