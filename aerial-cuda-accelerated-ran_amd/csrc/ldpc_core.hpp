@@ -8,6 +8,8 @@ namespace nrldpc {
 
 // Row capacity of the high-code-rate kernel variants (mb <= 8).
 constexpr int SMALL_MAXR = 8;
+// global loads a thread keeps in flight while loading a code block's LLRs
+constexpr int LLR_LOADS = 8;
 
 // Reduced shifts of every (BG, Z) in device memory, constant-initialised in
 // the code object.  Read through an address-space-4 pointer so that the
@@ -393,10 +395,24 @@ __device__ __forceinline__ void load_llrs_nosync(T* __restrict__ app, const TIn*
 {
     const T clampv = (T)clamp;
     const bool do_clamp = clamp > 0.0f;
-    for (int n = threadIdx.x; n < nval; n += blockDim.x) {
-        T v = llr_in<TIn, T>::cvt(src[n]);
-        if (do_clamp) v = num<T>::vmax(num<T>::vmin(v, clampv), -clampv);
-        app[n] = (v == (T)0) ? (T)0 : v;
+    const int step = (int)blockDim.x;
+    // LLR_LOADS loads in flight per thread (an index past the end reloads
+    // the thread's first value, whose store is skipped)
+    for (int n0 = threadIdx.x; n0 < nval; n0 += LLR_LOADS * step) {
+        TIn raw[LLR_LOADS];
+#pragma unroll
+        for (int u = 0; u < LLR_LOADS; ++u) {
+            const int n = n0 + u * step;
+            raw[u] = src[n < nval ? n : n0];
+        }
+#pragma unroll
+        for (int u = 0; u < LLR_LOADS; ++u) {
+            const int n = n0 + u * step;
+            if (n >= nval) break;
+            T v = llr_in<TIn, T>::cvt(raw[u]);
+            if (do_clamp) v = num<T>::vmax(num<T>::vmin(v, clampv), -clampv);
+            app[n] = (v == (T)0) ? (T)0 : v;
+        }
     }
 }
 template <typename T, typename TIn>
@@ -684,10 +700,21 @@ __device__ __forceinline__ int decode_lm(T* __restrict__ app, const cb_job& job,
         if (t < per * Z) {
             const int c0 = t / Z;
             const int z = t - c0 * Z;
-            for (int col = c0; col < ncol; col += per) {
-                T v = llr_in<TIn, T>::cvt(src[col * Z + z]);
-                if (do_clamp) v = num<T>::vmax(num<T>::vmin(v, clampv), -clampv);
-                lds_at<T>(lds, (uint32_t)z * pitch + (uint32_t)col * (uint32_t)sizeof(T)) = (v == (T)0) ? (T)0 : v;
+            for (int col0 = c0; col0 < ncol; col0 += LLR_LOADS * per) {
+                TIn raw[LLR_LOADS];  // loads in flight (a column past the end reloads column c0)
+#pragma unroll
+                for (int u = 0; u < LLR_LOADS; ++u) {
+                    const int col = col0 + u * per;
+                    raw[u] = src[(col < ncol ? col : c0) * Z + z];
+                }
+#pragma unroll
+                for (int u = 0; u < LLR_LOADS; ++u) {
+                    const int col = col0 + u * per;
+                    if (col >= ncol) break;
+                    T v = llr_in<TIn, T>::cvt(raw[u]);
+                    if (do_clamp) v = num<T>::vmax(num<T>::vmin(v, clampv), -clampv);
+                    lds_at<T>(lds, (uint32_t)z * pitch + (uint32_t)col * (uint32_t)sizeof(T)) = (v == (T)0) ? (T)0 : v;
+                }
             }
         }
         __syncthreads();
