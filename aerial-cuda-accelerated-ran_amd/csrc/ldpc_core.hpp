@@ -277,8 +277,12 @@ __device__ __forceinline__ void process_row(float*, BP __restrict__ base, shift_
 typedef uint16_t u16x2_t __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ half2_t u2h2(uint32_t x) { return __builtin_bit_cast(half2_t, x); }
 __device__ __forceinline__ uint32_t h22u(half2_t x) { return __builtin_bit_cast(uint32_t, x); }
-__device__ __forceinline__ half2_t hmin2(half2_t a, half2_t b) { return __builtin_elementwise_min(a, b); }
-__device__ __forceinline__ half2_t hmax2(half2_t a, half2_t b) { return __builtin_elementwise_max(a, b); }
+__device__ __forceinline__ u16x2_t u2m(uint32_t x) { return __builtin_bit_cast(u16x2_t, x); }
+// Magnitudes (|t|, min1, min2: never NaN, never negative) compared as their
+// 16-bit patterns, which order like the values: integer min/max need no
+// NaN canonicalisation of their operands and take swapped halves directly.
+__device__ __forceinline__ u16x2_t umin2(u16x2_t a, u16x2_t b) { return __builtin_elementwise_min(a, b); }
+__device__ __forceinline__ u16x2_t umax2(u16x2_t a, u16x2_t b) { return __builtin_elementwise_max(a, b); }
 
 // S0: message slot of the row's first edge (even: padded_row_start(R), or
 // the row's pair_slot within a row-pair lane group).
@@ -292,12 +296,12 @@ __device__ __forceinline__ void process_row(_Float16*, BP __restrict__ base, shi
     constexpr int P = (D + 1) / 2;
     static_assert(S0 % 2 == 0, "binary16 message slots of a row start at an even slot");
     constexpr int W0 = S0 / 2;
-    constexpr bool TWO = P >= 4;  // two accumulator sets: shorter dependency chains
-    constexpr uint32_t SIGNS = 0x80008000u;
+    constexpr bool TWO = P >= 6;  // two accumulator sets on long rows: shorter dependency chains
+    constexpr int NSET = TWO ? 2 : 1;
+    constexpr uint32_t SIGNS = 0x80008000u, MAGS = 0x7FFF7FFFu;
     uint32_t off[D];
     half2_t t[P];
-    const half2_t inf2 = u2h2(0x7C007C00u);
-    half2_t m1a = inf2, m2a = inf2, m1b = inf2, m2b = inf2;
+    u16x2_t m1a, m2a, m1b, m2b;
     uint32_t par = 0;
     static_for<P>([&](auto K) {
         constexpr int k = decltype(K)::value;
@@ -311,38 +315,46 @@ __device__ __forceinline__ void process_row(_Float16*, BP __restrict__ base, shi
             v |= 0x7C000000u;  // the dummy edge: +inf, sign bit clear
         }
         t[k] = u2h2(v) - st[W0 + k];
-        const half2_t a = __builtin_elementwise_abs(t[k]);
-        if (k < (TWO ? 2 : 1)) {  // a set's first pair: min1 = |t|, min2 = +inf
+        const u16x2_t a = u2m(h22u(t[k]) & MAGS);
+        if (k < NSET) {  // a set's first pair: min1 = |t|
             if (TWO && (k & 1))
                 m1b = a;
             else
                 m1a = a;
+        } else if (k < 2 * NSET) {  // its second: min2 = max(min1, |t|)
+            if (TWO && (k & 1)) {
+                m2b = umax2(m1b, a);
+                m1b = umin2(m1b, a);
+            } else {
+                m2a = umax2(m1a, a);
+                m1a = umin2(m1a, a);
+            }
         } else if (TWO && (k & 1)) {
-            m2b = hmin2(m2b, hmax2(m1b, a));
-            m1b = hmin2(m1b, a);
+            m2b = umin2(m2b, umax2(m1b, a));
+            m1b = umin2(m1b, a);
         } else {
-            m2a = hmin2(m2a, hmax2(m1a, a));
-            m1a = hmin2(m1a, a);
+            m2a = umin2(m2a, umax2(m1a, a));
+            m1a = umin2(m1a, a);
         }
         par ^= h22u(t[k]);
     });
     if constexpr (TWO) {
-        const half2_t m1 = hmin2(m1a, m1b);
-        m2a = hmin2(hmin2(m2a, m2b), hmax2(m1a, m1b));
+        const u16x2_t m1 = umin2(m1a, m1b);
+        m2a = umin2(umin2(m2a, m2b), umax2(m1a, m1b));
         m1a = m1;
     }
     // both halves: the row's min1 / min2, and the parity in bits 15 and 31
-    const half2_t m1s = m1a.yx, m2s = m2a.yx;
-    const half2_t M1 = hmin2(m1a, m1s);
-    const half2_t M2 = hmin2(hmin2(m2a, m2s), hmax2(m1a, m1s));
+    const u16x2_t m1s = m1a.yx, m2s = m2a.yx;
+    const u16x2_t M1 = umin2(m1a, m1s);
+    const u16x2_t M2 = umin2(umin2(m2a, m2s), umax2(m1a, m1s));
     const uint32_t px = par ^ __builtin_amdgcn_alignbit(par, par, 16);
     const uint32_t Q = h22u(half2_t{alpha, alpha}) ^ (px & SIGNS);
-    const u16x2_t S = __builtin_bit_cast(u16x2_t, M1) + __builtin_bit_cast(u16x2_t, M2);
+    const u16x2_t S = M1 + M2;
     static_for<P>([&](auto K) {
         constexpr int k = decltype(K)::value;
         constexpr int j0 = 2 * k, j1 = 2 * k + 1;
-        const half2_t g = hmin2(__builtin_elementwise_abs(t[k]), M2);
-        const half2_t other = __builtin_bit_cast(half2_t, (u16x2_t)(S - __builtin_bit_cast(u16x2_t, g)));
+        const u16x2_t g = umin2(u2m(h22u(t[k]) & MAGS), M2);
+        const half2_t other = __builtin_bit_cast(half2_t, (u16x2_t)(S - g));
         const half2_t sa = u2h2(Q ^ (h22u(t[k]) & SIGNS));
         const half2_t c = other * sa;
         const uint32_t nv = h22u(t[k] + c);
