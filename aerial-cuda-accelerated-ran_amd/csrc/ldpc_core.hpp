@@ -10,6 +10,9 @@ namespace nrldpc {
 constexpr int SMALL_MAXR = 8;
 // global loads a thread keeps in flight while loading a code block's LLRs
 constexpr int LLR_LOADS = 8;
+#ifndef TWO_SET_PAIRS
+#define TWO_SET_PAIRS 6
+#endif
 
 // Reduced shifts of every (BG, Z) in device memory, constant-initialised in
 // the code object.  Read through an address-space-4 pointer so that the
@@ -296,11 +299,12 @@ __device__ __forceinline__ void process_row(_Float16*, BP __restrict__ base, shi
     constexpr int P = (D + 1) / 2;
     static_assert(S0 % 2 == 0, "binary16 message slots of a row start at an even slot");
     constexpr int W0 = S0 / 2;
-    constexpr bool TWO = P >= 6;  // two accumulator sets on long rows: shorter dependency chains
+    constexpr bool TWO = P >= TWO_SET_PAIRS;  // two accumulator sets on long rows: shorter dependency chains
     constexpr int NSET = TWO ? 2 : 1;
     constexpr uint32_t SIGNS = 0x80008000u, MAGS = 0x7FFF7FFFu;
     uint32_t off[D];
     half2_t t[P];
+    u16x2_t mag[P];
     u16x2_t m1a, m2a, m1b, m2b;
     uint32_t par = 0;
     static_for<P>([&](auto K) {
@@ -316,6 +320,7 @@ __device__ __forceinline__ void process_row(_Float16*, BP __restrict__ base, shi
         }
         t[k] = u2h2(v) - st[W0 + k];
         const u16x2_t a = u2m(h22u(t[k]) & MAGS);
+        mag[k] = a;
         if (k < NSET) {  // a set's first pair: min1 = |t|
             if (TWO && (k & 1))
                 m1b = a;
@@ -353,7 +358,7 @@ __device__ __forceinline__ void process_row(_Float16*, BP __restrict__ base, shi
     static_for<P>([&](auto K) {
         constexpr int k = decltype(K)::value;
         constexpr int j0 = 2 * k, j1 = 2 * k + 1;
-        const u16x2_t g = umin2(u2m(h22u(t[k]) & MAGS), M2);
+        const u16x2_t g = umin2(mag[k], M2);
         const half2_t other = __builtin_bit_cast(half2_t, (u16x2_t)(S - g));
         const half2_t sa = u2h2(Q ^ (h22u(t[k]) & SIGNS));
         const half2_t c = other * sa;
