@@ -82,13 +82,17 @@ template <int L, uint32_t G>
 __device__ __forceinline__ uint32_t cmulmod(uint32_t a, uint32_t b)
 {
     constexpr uint32_t mask = (1u << L) - 1u;
+    // bit i of b and bit L-1 of r as all-ones / all-zero masks (sign-extending
+    // field extracts); bits of r above L-1 are left to fall out of the mask
+    // at the end, so that each step is a shift and two masked XORs
     uint32_t r = 0;
 #pragma unroll
     for (int i = L - 1; i >= 0; --i) {
-        r = ((r << 1) & mask) ^ ((r >> (L - 1)) ? G : 0u);
-        r ^= ((b >> i) & 1u) ? a : 0u;
+        const uint32_t top = (uint32_t)((int32_t)(r << (31 - (L - 1))) >> 31);
+        const uint32_t bit = (uint32_t)((int32_t)(b << (31 - i)) >> 31);
+        r = (r << 1) ^ (top & G) ^ (bit & a);
     }
-    return r;
+    return r & mask;
 }
 
 __device__ __forceinline__ uint32_t rbit8x4(uint32_t w)

@@ -180,6 +180,29 @@ template <> struct slot_vec<float> {
     typedef float type __attribute__((ext_vector_type(8)));
 };
 
+// The QM bits of symbol j combined into the buffer: their positions are
+// distinct (E <= L), so all QM reads are issued before the QM writes.
+template <int QM, typename TI, bool DESCR, typename V>
+__device__ __forceinline__ void combine_symbol(_Float16* __restrict__ cbuf, const V& v, const rm_geometry& g, int j,
+                                               float scale, uint32_t bits)
+{
+    int o[QM];
+    _Float16 old[QM];
+#pragma unroll
+    for (int k = 0; k < QM; ++k) {
+        int c = g.c0 + k * g.EQ + j;
+        if (c >= g.L) c -= g.L;
+        o[k] = (c < g.Kd) ? c : c + g.F;
+        old[k] = cbuf[o[k]];
+    }
+#pragma unroll
+    for (int k = 0; k < QM; ++k) {
+        _Float16 x = rm_num<_Float16>::clamp((_Float16)(llr_as_float(v[k]) * scale));
+        if (DESCR && ((bits >> k) & 1u)) x = -x;
+        cbuf[o[k]] = rm_num<_Float16>::clamp(old[k] + x);
+    }
+}
+
 // Combining when no circular-buffer position repeats within this
 // transmission (E <= L), the layers map one to one onto the buffer's
 // layers and the LLRs are not UCI-multiplexed: thread t takes whole symbols
@@ -214,13 +237,20 @@ __device__ __forceinline__ void combine_symbols(_Float16* __restrict__ cbuf, con
                 const uint64_t w2 = ((uint64_t)seq[(n0 >> 5) + 1] << 32) | seq[n0 >> 5];
                 bits = (uint32_t)(w2 >> (n0 & 31));
             }
-            for (int k = 0; k < g.Qm; ++k) {
-                int c = g.c0 + k * g.EQ + j;
-                if (c >= g.L) c -= g.L;
-                const int o = (c < g.Kd) ? c : c + g.F;
-                _Float16 x = rm_num<_Float16>::clamp((_Float16)(llr_as_float(v[u][k]) * scale));
-                if (DESCR && ((bits >> k) & 1u)) x = -x;
-                cbuf[o] = rm_num<_Float16>::clamp(cbuf[o] + x);
+            switch (g.Qm) {  // (uniform) modulation order as a constant
+            case 8: combine_symbol<8, TI, DESCR>(cbuf, v[u], g, j, scale, bits); break;
+            case 6: combine_symbol<6, TI, DESCR>(cbuf, v[u], g, j, scale, bits); break;
+            case 4: combine_symbol<4, TI, DESCR>(cbuf, v[u], g, j, scale, bits); break;
+            case 2: combine_symbol<2, TI, DESCR>(cbuf, v[u], g, j, scale, bits); break;
+            default:
+                for (int k = 0; k < g.Qm; ++k) {
+                    int c = g.c0 + k * g.EQ + j;
+                    if (c >= g.L) c -= g.L;
+                    const int o = (c < g.Kd) ? c : c + g.F;
+                    _Float16 x = rm_num<_Float16>::clamp((_Float16)(llr_as_float(v[u][k]) * scale));
+                    if (DESCR && ((bits >> k) & 1u)) x = -x;
+                    cbuf[o] = rm_num<_Float16>::clamp(cbuf[o] + x);
+                }
             }
         }
     }
