@@ -82,6 +82,7 @@ __device__ __forceinline__ cw_location locate_cw(const cuphyLDPCDecodeDesc_t& d,
 }
 
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+typedef uint16_t u16x2_t __attribute__((ext_vector_type(2)));
 
 template <int BG> constexpr uint64_t kBarrierMask = barrier_mask<bg_traits<BG>>();
 
@@ -184,17 +185,34 @@ __host__ __device__ constexpr uint32_t lm_pitch_bytes(int ncol, int elem)
 // add, subtract, min and one three-operand add per edge.
 // ---------------------------------------------------------------------------
 
+// Packed rotation (PK): when every rotated offset is below 2^16 (2 * Zb <=
+// 65536), the lane keeps zb and zb - Zb as the two 16-bit halves of one
+// register (lane_rot_base); one packed add of the shift to both halves and
+// one 16-bit unsigned min of the halves give rot_index: two instructions
+// instead of three.  (zb - Zb wraps to at least 65536 - Zb > any offset.)
+__device__ __forceinline__ uint32_t lane_rot_base(uint32_t zb, uint32_t Zb)
+{
+    return (zb & 0xFFFFu) | (((zb - Zb) & 0xFFFFu) << 16);
+}
+__device__ __forceinline__ uint32_t rot_index_packed(uint32_t zz, uint32_t sb)
+{
+    const u16x2_t pq = __builtin_bit_cast(u16x2_t, zz) + (u16x2_t){(uint16_t)sb, (uint16_t)sb};
+    return (uint32_t)__builtin_elementwise_min(pq.x, pq.y);
+}
+
 // Byte offset of edge E's APP value for the lane.  Column-major (LM false):
 // zb, Zb = lane and lifting size x sizeof(T).  Lane-major (LM true): zb, Zb
-// = lane and lifting size x pitch Pb.
-template <int BG, int E, typename T, bool LM>
+// = lane and lifting size x pitch Pb.  PK: zb is lane_rot_base(zb, Zb).
+template <int BG, int E, typename T, bool LM, bool PK = false>
 __device__ __forceinline__ uint32_t edge_off(shift_ptr sh, uint32_t zb, uint32_t Zb, uint32_t Pb)
 {
     using BGT = bg_traits<BG>;
+    const uint32_t sb = sh[E] * (LM ? Pb : (uint32_t)sizeof(T));
+    const uint32_t rot = PK ? rot_index_packed(zb, sb) : rot_index(zb, sb, Zb);
     if constexpr (LM)
-        return rot_index(zb, sh[E] * Pb, Zb) + (uint32_t)BGT::col(E) * (uint32_t)sizeof(T);
+        return rot + (uint32_t)BGT::col(E) * (uint32_t)sizeof(T);
     else
-        return (uint32_t)BGT::col(E) * Zb + rot_index(zb, sh[E] * (uint32_t)sizeof(T), Zb);
+        return (uint32_t)BGT::col(E) * Zb + rot;
 }
 
 // ---------------------------------------------------------------------------
@@ -204,7 +222,7 @@ __device__ __forceinline__ uint32_t edge_off(shift_ptr sh, uint32_t zb, uint32_t
 // subtract 1, min/min2/parity 3, new message + write-back 6 VALU ops.
 // ---------------------------------------------------------------------------
 // RS: the row's state slot (R, or its slot within a row-pair lane group).
-template <int BG, int R, int NR, int RS = R, bool LM = false, typename BP = unsigned char*>
+template <int BG, int R, int NR, int RS = R, bool LM = false, typename BP = unsigned char*, bool PK = false>
 __device__ __forceinline__ void process_row(float*, BP __restrict__ base, shift_ptr sh, uint32_t zb,
                                             uint32_t Zb, float alpha, float (&st_n1)[NR], float (&st_n2)[NR],
                                             uint32_t (&st_meta)[NR], uint32_t Pb = 0)
@@ -219,7 +237,7 @@ __device__ __forceinline__ void process_row(float*, BP __restrict__ base, shift_
     const uint32_t oidx = om >> 27;
     static_for<D>([&](auto J) {
         constexpr int j = decltype(J)::value;
-        off[j] = edge_off<BG, E0 + j, float, LM>(sh, zb, Zb, Pb);
+        off[j] = edge_off<BG, E0 + j, float, LM, PK>(sh, zb, Zb, Pb);
         const float mag = (oidx == (uint32_t)j) ? on2 : on1;
         const float old = u2f(f2u(mag) | ((om << (32 - D + j)) & 0x80000000u));
         t[j] = lds_ld<float>(base, off[j]) - old;
@@ -277,7 +295,6 @@ __device__ __forceinline__ void process_row(float*, BP __restrict__ base, shift_
 // magnitude does not depend on the sign under round-to-nearest.
 // No argmin tracking, no compare/select per edge.
 // ---------------------------------------------------------------------------
-typedef uint16_t u16x2_t __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ half2_t u2h2(uint32_t x) { return __builtin_bit_cast(half2_t, x); }
 __device__ __forceinline__ uint32_t h22u(half2_t x) { return __builtin_bit_cast(uint32_t, x); }
 __device__ __forceinline__ u16x2_t u2m(uint32_t x) { return __builtin_bit_cast(u16x2_t, x); }
@@ -289,7 +306,8 @@ __device__ __forceinline__ u16x2_t umax2(u16x2_t a, u16x2_t b) { return __builti
 
 // S0: message slot of the row's first edge (even: padded_row_start(R), or
 // the row's pair_slot within a row-pair lane group).
-template <int BG, int R, int NW, int S0 = padded_row_start<BG>(R), bool LM = false, typename BP = unsigned char*>
+template <int BG, int R, int NW, int S0 = padded_row_start<BG>(R), bool LM = false, typename BP = unsigned char*,
+          bool PK = false>
 __device__ __forceinline__ void process_row(_Float16*, BP __restrict__ base, shift_ptr sh, uint32_t zb,
                                             uint32_t Zb, _Float16 alpha, half2_t (&st)[NW], uint32_t Pb = 0)
 {
@@ -310,10 +328,10 @@ __device__ __forceinline__ void process_row(_Float16*, BP __restrict__ base, shi
     static_for<P>([&](auto K) {
         constexpr int k = decltype(K)::value;
         constexpr int j0 = 2 * k, j1 = 2 * k + 1;
-        off[j0] = edge_off<BG, E0 + j0, _Float16, LM>(sh, zb, Zb, Pb);
+        off[j0] = edge_off<BG, E0 + j0, _Float16, LM, PK>(sh, zb, Zb, Pb);
         uint32_t v = lds_ld<uint16_t>(base, off[j0]);
         if constexpr (j1 < D) {
-            off[j1] = edge_off<BG, E0 + j1, _Float16, LM>(sh, zb, Zb, Pb);
+            off[j1] = edge_off<BG, E0 + j1, _Float16, LM, PK>(sh, zb, Zb, Pb);
             v |= (uint32_t)lds_ld<uint16_t>(base, off[j1]) << 16;
         } else {
             v |= 0x7C000000u;  // the dummy edge: +inf, sign bit clear
@@ -370,7 +388,7 @@ __device__ __forceinline__ void process_row(_Float16*, BP __restrict__ base, shi
 }
 
 // Parity of check row R, lane z: hard bit = APP < 0 (syndrome_check.m:38-40).
-template <int BG, int R, typename T, bool LM = false, typename BP = unsigned char*>
+template <int BG, int R, typename T, bool LM = false, typename BP = unsigned char*, bool PK = false>
 __device__ __forceinline__ uint32_t row_syndrome(BP __restrict__ base, shift_ptr sh, uint32_t zb, uint32_t Zb,
                                                  uint32_t Pb = 0)
 {
@@ -380,7 +398,7 @@ __device__ __forceinline__ uint32_t row_syndrome(BP __restrict__ base, shift_ptr
     uint32_t par = 0;
     static_for<D>([&](auto J) {
         constexpr int j = decltype(J)::value;
-        const uint32_t off = edge_off<BG, E0 + j, T, LM>(sh, zb, Zb, Pb);
+        const uint32_t off = edge_off<BG, E0 + j, T, LM, PK>(sh, zb, Zb, Pb);
         if constexpr (std::is_same<T, _Float16>::value)
             par ^= (uint32_t)lds_ld<uint16_t>(base, off);  // sign bits (APP values are never -0)
         else
@@ -528,6 +546,10 @@ __device__ __forceinline__ int run_iterations(T* __restrict__ app, const cb_job&
     }();
     const uint32_t Pb = LM ? lm_pitch_bytes(BGT::NSYS + mb, (int)sizeof(T)) : (uint32_t)sizeof(T);
     const uint32_t Zb = Z * Pb;
+    // packed 16-bit rotations where 2 * Zb <= 65536 for every Z <= 384:
+    // column-major (Zb = Z * sizeof(T)) and lane-major binary16 with at most
+    // 8 block rows (pitch <= 60 bytes)
+    constexpr bool PK = !LM || (sizeof(T) == 2 && NR <= SMALL_MAXR);
 
     shift_ptr sh = (shift_ptr)(const void*)&g_shift_tables.v[BG - 1][z_index((int)Z)][0];
     int it = 0;
@@ -538,7 +560,7 @@ __device__ __forceinline__ int run_iterations(T* __restrict__ app, const cb_job&
         // per row: the 316 (BG1) per-edge shifts and addresses are reloaded
         // and recomputed where used instead of being hoisted out of the
         // loops into (spilled) registers.
-        uint32_t zb = zlane * Pb;
+        uint32_t zb = PK ? lane_rot_base(zlane * Pb, Zb) : zlane * Pb;
         asm volatile("" : "+v"(zb));
         static_for<NR>([&](auto Rc) {
             constexpr int R = decltype(Rc)::value;
@@ -554,9 +576,11 @@ __device__ __forceinline__ int run_iterations(T* __restrict__ app, const cb_job&
                     constexpr int RS = (PAIR == 2) ? PP.row_slot[R] : R;
                     constexpr int S0 = (PAIR == 2) ? PP.pair_slot[R] : padded_row_start<BG>(R);
                     if constexpr (std::is_same<T, float>::value)
-                        process_row<BG, R, NRS, RS, LM>(app, lane_base, shr, zb, Zr, alpha, st_n1, st_n2, st_meta, Pb);
+                        process_row<BG, R, NRS, RS, LM, decltype(lane_base), PK>(app, lane_base, shr, zb, Zr, alpha,
+                                                                                 st_n1, st_n2, st_meta, Pb);
                     else
-                        process_row<BG, R, NW, S0, LM>(app, lane_base, shr, zb, Zr, alpha, st_h, Pb);
+                        process_row<BG, R, NW, S0, LM, decltype(lane_base), PK>(app, lane_base, shr, zb, Zr, alpha,
+                                                                                st_h, Pb);
                 }
                 // barrier before the next row only where it shares a column
                 // with the rows since the last barrier, and at the iteration end
@@ -575,7 +599,7 @@ __device__ __forceinline__ int run_iterations(T* __restrict__ app, const cb_job&
                     uint32_t Zr = Zb;
                     asm volatile("" : "+s"(shr), "+s"(Zr));
                     if (active && (PAIR == 1 || lgrp == (uint32_t)PP.group[R]))
-                        fail |= row_syndrome<BG, R, T, LM>(lane_base, shr, zb, Zr, Pb);
+                        fail |= row_syndrome<BG, R, T, LM, decltype(lane_base), PK>(lane_base, shr, zb, Zr, Pb);
                 }
             });
             if constexpr (LM) {
