@@ -178,6 +178,41 @@ __host__ __device__ constexpr uint32_t lm_pitch_bytes(int ncol, int elem)
     return ((((uint32_t)(ncol * elem) + 3u) >> 2) | 1u) << 2;
 }
 
+// Split lane-major layout (binary16 kernels holding more than SMALL_MAXR
+// block rows): columns [0, S) in a lane-major array at LDS address 0 and
+// columns [S, ncol) in a second one at LM_HALF, both with the pitch of the
+// larger part, S = 34 (BG1) / 26 (BG2).  Z * pitch then stays at most 26,112
+// bytes, so every rotated offset fits 16 bits (packed rotation), and an
+// edge's column term -- 2c, or LM_HALF + 2(c - S) -- is still a constant
+// that the LDS instruction takes as its offset.
+constexpr uint32_t LM_HALF = 32768;
+__host__ __device__ constexpr int lm_split_col(int BG) { return BG == 1 ? 34 : 26; }
+__host__ __device__ constexpr bool lm_split_layout(int elem, int maxr) { return elem == 2 && maxr > SMALL_MAXR; }
+__host__ __device__ constexpr uint32_t lm_pitch(int BG, int ncol, int elem, bool split)
+{
+    if (!split) return lm_pitch_bytes(ncol, elem);
+    const int S = lm_split_col(BG);
+    const int n0 = ncol < S ? ncol : S, n1 = ncol - S;
+    return lm_pitch_bytes(n0 > n1 ? n0 : n1, elem);
+}
+// Byte offset of column c in a lane's row (add z * pitch for lane z).
+__host__ __device__ constexpr uint32_t lm_col_offset(int BG, int c, int elem, bool split)
+{
+    return (split && c >= lm_split_col(BG)) ? LM_HALF + (uint32_t)(c - lm_split_col(BG)) * (uint32_t)elem
+                                            : (uint32_t)c * (uint32_t)elem;
+}
+// LDS bytes of a code block's lane-major APP values.
+__host__ __device__ constexpr uint32_t lm_app_bytes(int BG, int Z, int ncol, int elem, bool split)
+{
+    return (split && ncol > lm_split_col(BG)) ? LM_HALF + (uint32_t)Z * lm_pitch(BG, ncol, elem, true)
+                                              : (uint32_t)Z * lm_pitch(BG, ncol, elem, split);
+}
+// Largest of those over Z <= 384 and every mb (launch sizing).
+__host__ __device__ constexpr uint32_t lm_app_bytes_max(int elem, bool split)
+{
+    return split ? LM_HALF + 384u * lm_pitch_bytes(34, elem) : 384u * lm_pitch_bytes(68, elem);
+}
+
 // ---------------------------------------------------------------------------
 // Row addressing: lane z of a code block whose APP values start at byte
 // `base` reads APP[col][(z + s) mod Z] at base + col*Zb + rot_index(zb, sb, Zb),
@@ -203,14 +238,14 @@ __device__ __forceinline__ uint32_t rot_index_packed(uint32_t zz, uint32_t sb)
 // Byte offset of edge E's APP value for the lane.  Column-major (LM false):
 // zb, Zb = lane and lifting size x sizeof(T).  Lane-major (LM true): zb, Zb
 // = lane and lifting size x pitch Pb.  PK: zb is lane_rot_base(zb, Zb).
-template <int BG, int E, typename T, bool LM, bool PK = false>
+template <int BG, int E, typename T, bool LM, bool PK = false, bool SPL = false>
 __device__ __forceinline__ uint32_t edge_off(shift_ptr sh, uint32_t zb, uint32_t Zb, uint32_t Pb)
 {
     using BGT = bg_traits<BG>;
     const uint32_t sb = sh[E] * (LM ? Pb : (uint32_t)sizeof(T));
     const uint32_t rot = PK ? rot_index_packed(zb, sb) : rot_index(zb, sb, Zb);
     if constexpr (LM)
-        return rot + (uint32_t)BGT::col(E) * (uint32_t)sizeof(T);
+        return rot + lm_col_offset(BG, BGT::col(E), (int)sizeof(T), SPL);
     else
         return (uint32_t)BGT::col(E) * Zb + rot;
 }
@@ -222,7 +257,8 @@ __device__ __forceinline__ uint32_t edge_off(shift_ptr sh, uint32_t zb, uint32_t
 // subtract 1, min/min2/parity 3, new message + write-back 6 VALU ops.
 // ---------------------------------------------------------------------------
 // RS: the row's state slot (R, or its slot within a row-pair lane group).
-template <int BG, int R, int NR, int RS = R, bool LM = false, typename BP = unsigned char*, bool PK = false>
+template <int BG, int R, int NR, int RS = R, bool LM = false, typename BP = unsigned char*, bool PK = false,
+          bool SPL = false>
 __device__ __forceinline__ void process_row(float*, BP __restrict__ base, shift_ptr sh, uint32_t zb,
                                             uint32_t Zb, float alpha, float (&st_n1)[NR], float (&st_n2)[NR],
                                             uint32_t (&st_meta)[NR], uint32_t Pb = 0)
@@ -237,7 +273,7 @@ __device__ __forceinline__ void process_row(float*, BP __restrict__ base, shift_
     const uint32_t oidx = om >> 27;
     static_for<D>([&](auto J) {
         constexpr int j = decltype(J)::value;
-        off[j] = edge_off<BG, E0 + j, float, LM, PK>(sh, zb, Zb, Pb);
+        off[j] = edge_off<BG, E0 + j, float, LM, PK, SPL>(sh, zb, Zb, Pb);
         const float mag = (oidx == (uint32_t)j) ? on2 : on1;
         const float old = u2f(f2u(mag) | ((om << (32 - D + j)) & 0x80000000u));
         t[j] = lds_ld<float>(base, off[j]) - old;
@@ -307,7 +343,7 @@ __device__ __forceinline__ u16x2_t umax2(u16x2_t a, u16x2_t b) { return __builti
 // S0: message slot of the row's first edge (even: padded_row_start(R), or
 // the row's pair_slot within a row-pair lane group).
 template <int BG, int R, int NW, int S0 = padded_row_start<BG>(R), bool LM = false, typename BP = unsigned char*,
-          bool PK = false>
+          bool PK = false, bool SPL = false>
 __device__ __forceinline__ void process_row(_Float16*, BP __restrict__ base, shift_ptr sh, uint32_t zb,
                                             uint32_t Zb, _Float16 alpha, half2_t (&st)[NW], uint32_t Pb = 0)
 {
@@ -328,10 +364,10 @@ __device__ __forceinline__ void process_row(_Float16*, BP __restrict__ base, shi
     static_for<P>([&](auto K) {
         constexpr int k = decltype(K)::value;
         constexpr int j0 = 2 * k, j1 = 2 * k + 1;
-        off[j0] = edge_off<BG, E0 + j0, _Float16, LM, PK>(sh, zb, Zb, Pb);
+        off[j0] = edge_off<BG, E0 + j0, _Float16, LM, PK, SPL>(sh, zb, Zb, Pb);
         uint32_t v = lds_ld<uint16_t>(base, off[j0]);
         if constexpr (j1 < D) {
-            off[j1] = edge_off<BG, E0 + j1, _Float16, LM, PK>(sh, zb, Zb, Pb);
+            off[j1] = edge_off<BG, E0 + j1, _Float16, LM, PK, SPL>(sh, zb, Zb, Pb);
             v |= (uint32_t)lds_ld<uint16_t>(base, off[j1]) << 16;
         } else {
             v |= 0x7C000000u;  // the dummy edge: +inf, sign bit clear
@@ -388,7 +424,8 @@ __device__ __forceinline__ void process_row(_Float16*, BP __restrict__ base, shi
 }
 
 // Parity of check row R, lane z: hard bit = APP < 0 (syndrome_check.m:38-40).
-template <int BG, int R, typename T, bool LM = false, typename BP = unsigned char*, bool PK = false>
+template <int BG, int R, typename T, bool LM = false, typename BP = unsigned char*, bool PK = false,
+          bool SPL = false>
 __device__ __forceinline__ uint32_t row_syndrome(BP __restrict__ base, shift_ptr sh, uint32_t zb, uint32_t Zb,
                                                  uint32_t Pb = 0)
 {
@@ -398,7 +435,7 @@ __device__ __forceinline__ uint32_t row_syndrome(BP __restrict__ base, shift_ptr
     uint32_t par = 0;
     static_for<D>([&](auto J) {
         constexpr int j = decltype(J)::value;
-        const uint32_t off = edge_off<BG, E0 + j, T, LM, PK>(sh, zb, Zb, Pb);
+        const uint32_t off = edge_off<BG, E0 + j, T, LM, PK, SPL>(sh, zb, Zb, Pb);
         if constexpr (std::is_same<T, _Float16>::value)
             par ^= (uint32_t)lds_ld<uint16_t>(base, off);  // sign bits (APP values are never -0)
         else
@@ -544,12 +581,14 @@ __device__ __forceinline__ int run_iterations(T* __restrict__ app, const cb_job&
         else
             return cm_base;
     }();
-    const uint32_t Pb = LM ? lm_pitch_bytes(BGT::NSYS + mb, (int)sizeof(T)) : (uint32_t)sizeof(T);
+    // lane-major binary16 with more than SMALL_MAXR rows: split layout
+    constexpr bool SPL = LM && lm_split_layout((int)sizeof(T), NR);
+    const uint32_t Pb = LM ? lm_pitch(BG, BGT::NSYS + mb, (int)sizeof(T), SPL) : (uint32_t)sizeof(T);
     const uint32_t Zb = Z * Pb;
     // packed 16-bit rotations where 2 * Zb <= 65536 for every Z <= 384:
-    // column-major (Zb = Z * sizeof(T)) and lane-major binary16 with at most
-    // 8 block rows (pitch <= 60 bytes)
-    constexpr bool PK = !LM || (sizeof(T) == 2 && NR <= SMALL_MAXR);
+    // column-major (Zb = Z * sizeof(T)), lane-major binary16 with at most 8
+    // block rows (pitch <= 60 bytes) and the split layout (pitch <= 68 bytes)
+    constexpr bool PK = !LM || sizeof(T) == 2;
 
     shift_ptr sh = (shift_ptr)(const void*)&g_shift_tables.v[BG - 1][z_index((int)Z)][0];
     int it = 0;
@@ -576,11 +615,12 @@ __device__ __forceinline__ int run_iterations(T* __restrict__ app, const cb_job&
                     constexpr int RS = (PAIR == 2) ? PP.row_slot[R] : R;
                     constexpr int S0 = (PAIR == 2) ? PP.pair_slot[R] : padded_row_start<BG>(R);
                     if constexpr (std::is_same<T, float>::value)
-                        process_row<BG, R, NRS, RS, LM, decltype(lane_base), PK>(app, lane_base, shr, zb, Zr, alpha,
-                                                                                 st_n1, st_n2, st_meta, Pb);
+                        process_row<BG, R, NRS, RS, LM, decltype(lane_base), PK, SPL>(app, lane_base, shr, zb, Zr,
+                                                                                      alpha, st_n1, st_n2, st_meta,
+                                                                                      Pb);
                     else
-                        process_row<BG, R, NW, S0, LM, decltype(lane_base), PK>(app, lane_base, shr, zb, Zr, alpha,
-                                                                                st_h, Pb);
+                        process_row<BG, R, NW, S0, LM, decltype(lane_base), PK, SPL>(app, lane_base, shr, zb, Zr,
+                                                                                     alpha, st_h, Pb);
                 }
                 // barrier before the next row only where it shares a column
                 // with the rows since the last barrier, and at the iteration end
@@ -599,7 +639,8 @@ __device__ __forceinline__ int run_iterations(T* __restrict__ app, const cb_job&
                     uint32_t Zr = Zb;
                     asm volatile("" : "+s"(shr), "+s"(Zr));
                     if (active && (PAIR == 1 || lgrp == (uint32_t)PP.group[R]))
-                        fail |= row_syndrome<BG, R, T, LM, decltype(lane_base), PK>(lane_base, shr, zb, Zr, Pb);
+                        fail |= row_syndrome<BG, R, T, LM, decltype(lane_base), PK, SPL>(lane_base, shr, zb, Zr,
+                                                                                         Pb);
                 }
             });
             if constexpr (LM) {
@@ -728,7 +769,8 @@ __device__ __forceinline__ int decode_lm(T* __restrict__ app, const cb_job& job,
 {
     const int Z = job.Z;
     const int ncol = bg_traits<BG>::NSYS + job.mb;
-    const uint32_t pitch = lm_pitch_bytes(ncol, (int)sizeof(T));
+    constexpr bool SPL = lm_split_layout((int)sizeof(T), MAXR);
+    const uint32_t pitch = lm_pitch(BG, ncol, (int)sizeof(T), SPL);
     unsigned char* const lds = reinterpret_cast<unsigned char*>(app);
     // LLRs: thread t takes lane z = t mod Z of columns t / Z, t / Z + per, ...
     // (per = blockDim.x / Z >= 1 column streams; coalesced over z)
@@ -754,7 +796,8 @@ __device__ __forceinline__ int decode_lm(T* __restrict__ app, const cb_job& job,
                     if (col >= ncol) break;
                     T v = llr_in<TIn, T>::cvt(raw[u]);
                     if (do_clamp) v = num<T>::vmax(num<T>::vmin(v, clampv), -clampv);
-                    lds_at<T>(lds, (uint32_t)z * pitch + (uint32_t)col * (uint32_t)sizeof(T)) = (v == (T)0) ? (T)0 : v;
+                    lds_at<T>(lds, (uint32_t)z * pitch + lm_col_offset(BG, col, (int)sizeof(T), SPL)) =
+                        (v == (T)0) ? (T)0 : v;
                 }
             }
         }
@@ -774,7 +817,7 @@ __device__ __forceinline__ int decode_lm(T* __restrict__ app, const cb_job& job,
         int col = n / Z, z = n - col * Z;
         for (int c = tid >> 6; c * 64 < nbits; c += nwaves) {
             const bool bit =
-                (n < nbits) && (lds_at<T>(lds, (uint32_t)z * pitch + (uint32_t)col * (uint32_t)sizeof(T)) <= (T)0);
+                (n < nbits) && (lds_at<T>(lds, (uint32_t)z * pitch + lm_col_offset(BG, col, (int)sizeof(T), SPL)) <= (T)0);
             const uint64_t m = __ballot(bit);
             if (lane == 0) {
                 b.out[2 * c] = (uint32_t)m;
@@ -793,7 +836,7 @@ __device__ __forceinline__ int decode_lm(T* __restrict__ app, const cb_job& job,
         int n = (int)threadIdx.x;
         int col = n / Z, z = n - col * Z;
         for (; n < nbits; n += step) {
-            b.soft[n] = (_Float16)lds_at<T>(lds, (uint32_t)z * pitch + (uint32_t)col * (uint32_t)sizeof(T));
+            b.soft[n] = (_Float16)lds_at<T>(lds, (uint32_t)z * pitch + lm_col_offset(BG, col, (int)sizeof(T), SPL));
             z += step;
             while (z >= Z) {
                 z -= Z;

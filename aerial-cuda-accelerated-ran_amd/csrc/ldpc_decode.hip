@@ -67,9 +67,10 @@ __global__ __launch_bounds__(768) void ldpc_layered_tb_pair(cuphyLDPCDecodeDesc_
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const cb_job job = tb_job<TIn>(desc);
-    // the vote word after the lane-major APP rows (Z * pitch bytes, a multiple of 4)
+    // the vote word after the lane-major APP values (a multiple of 4 bytes)
     uint32_t* flag = reinterpret_cast<uint32_t*>(
-        lds_raw + (size_t)job.Z * lm_pitch_bytes(bg_traits<BG>::NSYS + job.mb, (int)sizeof(T)));
+        lds_raw + lm_app_bytes(BG, job.Z, bg_traits<BG>::NSYS + job.mb, (int)sizeof(T),
+                               lm_split_layout((int)sizeof(T), 46)));
     decode_lm<BG, T, TIn, 46, 2>(reinterpret_cast<T*>(lds_raw), job, tb_buffers<TIn>(desc, (int)blockIdx.x), flag);
 }
 
@@ -186,8 +187,9 @@ __device__ __forceinline__ void persistent_lm(cuphyHipLDPCWorkItem_t* items, con
             const cb_job job = item_job(w);
             uint32_t st = work_item_reasons(w, MAXR);
             if (st == 0) {
-                // lane-major APP values: Z rows of lm_pitch_bytes, below the last 16 bytes
-                const uint32_t need = (uint32_t)job.Z * lm_pitch_bytes(((w.BG == 1) ? 22 : 10) + job.mb, (int)sizeof(T));
+                // lane-major APP values, below the last 16 bytes
+                const uint32_t need = lm_app_bytes(w.BG, job.Z, ((w.BG == 1) ? 22 : 10) + job.mb, (int)sizeof(T),
+                                                   lm_split_layout((int)sizeof(T), MAXR));
                 if (need + 16u > lds_bytes || PAIR * ((job.Z + 63) & ~63) > (int)blockDim.x) st = CUPHY_HIP_ITEM_NO_FIT;
             }
             if (threadIdx.x == 0) w.status = st;
@@ -355,8 +357,9 @@ cuphyStatus_t fill_launch(const cuphyLDPCDecodeDesc_t& desc, launch_geometry& g)
     }
     const int nsys = (c.BG == 1) ? 22 : 10;
     const size_t stride = (size_t)(((nsys + c.num_parity_nodes) * c.Z + 7) & ~7);
-    // row-pair kernels keep the APP values lane-major: Z rows of lm_pitch_bytes
-    g.lds_bytes = pair ? (size_t)c.Z * lm_pitch_bytes(nsys + c.num_parity_nodes, fp16 ? 2 : 4) + 16u
+    // row-pair kernels keep the APP values lane-major (binary16: split layout)
+    g.lds_bytes = pair ? (size_t)lm_app_bytes(c.BG, c.Z, nsys + c.num_parity_nodes, fp16 ? 2 : 4,
+                                              lm_split_layout(fp16 ? 2 : 4, 46)) + 16u
                        : (size_t)P * stride * (fp16 ? 2u : 4u);
     g.block = pair ? dim3((unsigned)(2 * ((c.Z + 63) / 64 * 64))) : dim3((unsigned)((P * c.Z + 63) / 64 * 64));
     g.grid = dim3((unsigned)((desc_codewords(desc) + P - 1) / P));
@@ -435,8 +438,12 @@ cuphyStatus_t decode_batch(cuphyHipLDPCWorkItem_t* d_items, const uint32_t* d_gr
                                         : select_persistent_kernel<46>(llr_type, f16);
     const unsigned block = (unsigned)block_threads;
     // lane-major: APP rows padded to an odd number of words, at most 7 bytes
-    // per lane beyond the values (Z <= 384), and 16 bytes of queue slot and vote
-    const size_t lds = (size_t)max_values_per_group * elem + (lm ? 8u * 384u : 0u);
+    // per lane beyond the values (Z <= 384), and 16 bytes of queue slot and
+    // vote; the split layout of the binary16 46-row kernels needs its
+    // second array at LM_HALF
+    size_t lds = (size_t)max_values_per_group * elem + (lm ? 8u * 384u : 0u);
+    if (lm && lm_split_layout((int)elem, (pair || max_mb > SMALL_MAXR) ? 46 : SMALL_MAXR))
+        lds = std::max(lds, (size_t)lm_app_bytes_max((int)elem, true) + 16u);
     cuphyStatus_t s = lm ? check_lm_kernel(func) : CUPHY_STATUS_SUCCESS;
     if (s == CUPHY_STATUS_SUCCESS) s = ensure_lds_attr(func, lds);
     if (s != CUPHY_STATUS_SUCCESS) return s;
