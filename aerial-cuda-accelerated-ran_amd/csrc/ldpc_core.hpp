@@ -523,7 +523,7 @@ template <> __device__ __forceinline__ _Float16 job_alpha<_Float16>(const cb_job
 // group 0 and group 1; rows of one barrier segment run at the same time,
 // and each lane holds the message state of its group's rows only.
 // LM: lane-major APP values from LDS address 0 (lm_pitch_bytes; count must
-// be 1, cb_flags one LDS word for the early-termination vote).
+// be 1, cb_flags two LDS words for the early-termination votes).
 template <int BG, typename T, int MAXR, int PAIR = 1, bool LM = (PAIR == 2)>
 __device__ __forceinline__ int run_iterations(T* __restrict__ app, const cb_job& job, int count = 1, int stride = 0,
                                               uint32_t* cb_flags = nullptr, uint32_t* cb_iters = nullptr)
@@ -629,32 +629,51 @@ __device__ __forceinline__ int run_iterations(T* __restrict__ app, const cb_job&
             }
         });
         if (job.early_term) {
-            uint32_t fail = 0;
-            static_for<NR>([&](auto Rc) {
-                constexpr int R = decltype(Rc)::value;
-                int mbr = mb;
-                asm volatile("" : "+s"(mbr));
-                if (R < mbr) {
-                    shift_ptr shr = sh;
-                    uint32_t Zr = Zb;
-                    asm volatile("" : "+s"(shr), "+s"(Zr));
-                    if (active && (PAIR == 1 || lgrp == (uint32_t)PP.group[R]))
-                        fail |= row_syndrome<BG, R, T, LM, decltype(lane_base), PK, SPL>(lane_base, shr, zb, Zr,
-                                                                                         Pb);
+            // parity of block rows [r0, r1) over this lane's check rows
+            const auto syndrome = [&](auto r0c, auto r1c) {
+                constexpr int r0 = decltype(r0c)::value, r1 = decltype(r1c)::value;
+                uint32_t fail = 0;
+                static_for<NR>([&](auto Rc) {
+                    constexpr int R = decltype(Rc)::value;
+                    if constexpr (R >= r0 && R < r1) {
+                        int mbr = mb;
+                        asm volatile("" : "+s"(mbr));
+                        if (R < mbr) {
+                            shift_ptr shr = sh;
+                            uint32_t Zr = Zb;
+                            asm volatile("" : "+s"(shr), "+s"(Zr));
+                            if (active && (PAIR == 1 || lgrp == (uint32_t)PP.group[R]))
+                                fail |= row_syndrome<BG, R, T, LM, decltype(lane_base), PK, SPL>(lane_base, shr, zb,
+                                                                                                 Zr, Pb);
+                        }
+                    }
+                });
+                return fail;
+            };
+            // OR of x over the workgroup (lane-major kernels: through the LDS
+            // word cb_flags[w], w = 0 / 1 for the two votes of an iteration;
+            // they hold no static LDS, so that their APP values start at LDS
+            // address 0)
+            const auto vote = [&](uint32_t x, int w) -> bool {
+                if constexpr (LM) {
+                    if (tid == 0u) cb_flags[w] = 0u;
+                    __syncthreads();
+                    if (x) cb_flags[w] = 1u;
+                    __syncthreads();
+                    return __builtin_amdgcn_readfirstlane(cb_flags[w]) != 0u;
+                } else {
+                    return __syncthreads_or((int)x) != 0;
                 }
-            });
-            if constexpr (LM) {
-                // OR over the workgroup through the LDS word cb_flags[0]
-                // (the lane-major kernels hold no static LDS, so that their
-                // APP values start at LDS address 0)
-                if (tid == 0u) cb_flags[0] = 0u;
-                __syncthreads();
-                if (fail) cb_flags[0] = 1u;
-                __syncthreads();
-                if (__builtin_amdgcn_readfirstlane(cb_flags[0]) == 0u) break;
-            } else if (count == 1) {
-                if (!__syncthreads_or((int)fail)) break;
+            };
+            if (LM || count == 1) {
+                // block row 0 first: a code block that has not converged
+                // almost always fails it, so the other rows are checked only
+                // when it passes (the stopping rule is unchanged: stop when
+                // every row passes)
+                if (vote(syndrome(std::integral_constant<int, 0>{}, std::integral_constant<int, 1>{}), 0)) continue;
+                if (!vote(syndrome(std::integral_constant<int, 1>{}, std::integral_constant<int, NR>{}), 1)) break;
             } else {
+                const uint32_t fail = syndrome(std::integral_constant<int, 0>{}, std::integral_constant<int, NR>{});
                 // per code block: any failing check row keeps it iterating
                 if (tid < (uint32_t)count) cb_flags[tid] = 0u;
                 __syncthreads();
