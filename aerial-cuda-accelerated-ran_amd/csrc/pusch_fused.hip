@@ -87,17 +87,21 @@ __device__ __forceinline__ void cb_and_share_crc(const fused_lds& sh, const uint
     const int t = threadIdx.x;
     uint32_t rc = 0, rs = 0;
     if (4 * t < nb_cb) {
+        // the two remainders as independent straight-line chains (a shift
+        // of 0 bytes multiplies by x^0 = 1), so that their table lookups
+        // and products interleave
         const uint32_t w = rbit8x4(sh.words[t]);
+        const uint32_t* TA = sh.crc_t[K_CRC24A];
         rc = crc_word<L>(w, T);
-        if (rc && nb_cb > 4 * t + 4) rc = cmulmod<L, G>(rc, S[nb_cb - 4 * t - 4]);
-        if (4 * t < db) {  // payload bytes [4t, min(4t + 4, db)) for the share
-            const int nb = min(4, db - 4 * t);
-            const uint32_t* TA = sh.crc_t[K_CRC24A];
-            uint32_t reg = 0;
-            for (int b = 0; b < nb; ++b) reg = ((reg << 8) & 0xFFFFFFu) ^ TA[((reg >> 16) ^ (w >> (8 * b))) & 0xFFu];
-            rs = reg;
-            if (rs && db > 4 * t + nb) rs = cmulmod<24, CRC24A_P.g>(rs, sh.shift[K_CRC24A][db - 4 * t - nb]);
+        if (4 * t + 4 <= db) {
+            rs = crc_word<24>(w, TA);
+        } else if (4 * t < db) {  // the payload's last, partial word: bytes [4t, db)
+            for (int b = 0; b < db - 4 * t; ++b)
+                rs = ((rs << 8) & 0xFFFFFFu) ^ TA[((rs >> 16) ^ (w >> (8 * b))) & 0xFFu];
         }
+        const int nb = max(0, min(4, db - 4 * t));
+        rc = cmulmod<L, G>(rc, S[nb_cb - 4 * t - 4]);
+        rs = cmulmod<24, CRC24A_P.g>(rs, sh.shift[K_CRC24A][max(0, db - 4 * t - nb)]);
     }
     for (int off = 32; off > 0; off >>= 1) {
         rc ^= __shfl_xor(rc, off, 64);
@@ -478,9 +482,12 @@ __device__ __forceinline__ uint32_t fused_group(_Float16* __restrict__ hb, fused
         // bytes after them), the TB zero-padded to whole words (CRC linearity)
         const uint32_t total = (uint32_t)((C * data_bytes + 3) / 4 * 4);
         const uint32_t after = total - (uint32_t)(cb_in_tb + 1) * (uint32_t)data_bytes;
-        if (part && after)
-            part = cmulmod<24, CRC24A_P.g>(part, cmulmod<24, CRC24A_P.g>(sh.shift[K_CRC24A][after & 1023u],
-                                                                         g_crc_shift.tb_hi[(after >> 10) & 255u]));
+        // uniform operands: the two products run on the scalar unit
+        part = (uint32_t)__builtin_amdgcn_readfirstlane((int)part);
+        if (part && after) {
+            const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)sh.shift[K_CRC24A][after & 1023u]);
+            part = cmulmod<24, CRC24A_P.g>(part, cmulmod<24, CRC24A_P.g>(lo, g_crc_shift.tb_hi[(after >> 10) & 255u]));
+        }
         acc ^= part;
         if (c + 1 < count) __syncthreads();  // the words in LDS are the next CB's
     }
