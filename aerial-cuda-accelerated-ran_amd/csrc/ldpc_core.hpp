@@ -414,7 +414,8 @@ __device__ __forceinline__ void process_row(_Float16*, BP __restrict__ base, shi
         constexpr int j0 = 2 * k, j1 = 2 * k + 1;
         const u16x2_t g = umin2(mag[k], M2);
         const half2_t other = __builtin_bit_cast(half2_t, (u16x2_t)(S - g));
-        const half2_t sa = u2h2(Q ^ (h22u(t[k]) & SIGNS));
+        // Q ^ (t & SIGNS) as one three-input bitwise instruction (table 0x6C: B ^ (A & C))
+        const half2_t sa = u2h2(__builtin_amdgcn_bitop3_b32(h22u(t[k]), Q, SIGNS, (unsigned char)0x6C));
         const half2_t c = other * sa;
         const uint32_t nv = h22u(t[k] + c);
         lds_st<uint16_t>(base, off[j0], (uint16_t)nv);
