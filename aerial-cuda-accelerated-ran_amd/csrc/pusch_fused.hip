@@ -379,7 +379,8 @@ __device__ __forceinline__ void fused_combine(_Float16* __restrict__ hb, uint32_
 template <typename TI, int MAXR, bool GROUPED>
 __device__ __forceinline__ uint32_t fused_group(_Float16* __restrict__ hb, fused_lds& sh,
                                                 const cuphyHipPuschSlotTb_t& job, const PerTbParams& p, int cb0,
-                                                int count_in, int lds_values)
+                                                int count_in, int lds_values, uint32_t* __restrict__ tbw,
+                                                uint32_t& xor_prev)
 {
     const int count = GROUPED ? count_in : 1;
     rm_geometry g0 = rm_geometry_of(p, cb0);
@@ -469,7 +470,6 @@ __device__ __forceinline__ uint32_t fused_group(_Float16* __restrict__ hb, fused
         else
             cb_and_share_crc<24, CRC24A_P.g>(sh, sh.crc_t[kind], sh.shift[kind], 4 * ((Kp + 31) / 32), data_bytes,
                                              sh.red[0], sh.red[1], crc, part);
-        for (int b = tid; b < data_bytes; b += nthr) job.tb_bytes[(size_t)r * data_bytes + b] = (uint8_t)byte_of(b);
         if (tid == 0) {
             job.cb_crc[r] = crc;
             if (cb_in_tb == 0) {
@@ -477,18 +477,28 @@ __device__ __forceinline__ uint32_t fused_group(_Float16* __restrict__ hb, fused
                 for (int b = bytes; b < (bytes + 3) / 4 * 4; ++b) job.tb_bytes[b] = 0;
             }
         }
-        if (C == 1) return crc;
-        // TB CRC = XOR over CBs of crc(payload bytes of CB c) * x^(8 * TB
-        // bytes after them), the TB zero-padded to whole words (CRC linearity)
-        const uint32_t total = (uint32_t)((C * data_bytes + 3) / 4 * 4);
-        const uint32_t after = total - (uint32_t)(cb_in_tb + 1) * (uint32_t)data_bytes;
-        // uniform operands: the two products run on the scalar unit
-        part = (uint32_t)__builtin_amdgcn_readfirstlane((int)part);
-        if (part && after) {
-            const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)sh.shift[K_CRC24A][after & 1023u]);
-            part = cmulmod<24, CRC24A_P.g>(part, cmulmod<24, CRC24A_P.g>(lo, g_crc_shift.tb_hi[(after >> 10) & 255u]));
+        if (C > 1) {
+            // TB CRC = XOR over CBs of crc(payload bytes of CB c) * x^(8 * TB
+            // bytes after them), the TB zero-padded to whole words (CRC linearity)
+            const uint32_t total = (uint32_t)((C * data_bytes + 3) / 4 * 4);
+            const uint32_t after = total - (uint32_t)(cb_in_tb + 1) * (uint32_t)data_bytes;
+            // uniform operands: the two products run on the scalar unit
+            part = (uint32_t)__builtin_amdgcn_readfirstlane((int)part);
+            if (part && after) {
+                const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)sh.shift[K_CRC24A][after & 1023u]);
+                part = cmulmod<24, CRC24A_P.g>(part,
+                                               cmulmod<24, CRC24A_P.g>(lo, g_crc_shift.tb_hi[(after >> 10) & 255u]));
+            }
+            acc ^= part;
+            // the group's share goes into the TB's accumulator now, so that
+            // the atomic's round trip overlaps the byte stores below
+            // (complete_cb raises the TB's count once it has returned)
+            if (c + 1 == count && tid == 0)
+                xor_prev = __hip_atomic_fetch_xor(tbw + 2u * job.share_offset, acc, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT);
         }
-        acc ^= part;
+        for (int b = tid; b < data_bytes; b += nthr) job.tb_bytes[(size_t)r * data_bytes + b] = (uint8_t)byte_of(b);
+        if (C == 1) return crc;
         if (c + 1 < count) __syncthreads();  // the words in LDS are the next CB's
     }
     return acc;
@@ -498,9 +508,10 @@ __device__ __forceinline__ uint32_t fused_group(_Float16* __restrict__ hb, fused
 // 2 * share_offset, count of finished CBs at 2 * share_offset + 1.  Called
 // by one thread once the whole workgroup has finished a group of n CBs of
 // the TB (all of their outputs issued).  v: the XOR of the group's shares
-// (TB_ACC_ERROR: rejected group).
+// (TB_ACC_ERROR: rejected group); for C > 1 and an accepted group, the
+// shares are already in the accumulator and xor_prev is its old value.
 __device__ __forceinline__ void complete_cb(const cuphyHipPuschSlotTb_t& job, uint32_t C, uint32_t v, uint32_t n,
-                                            uint32_t* __restrict__ tbw)
+                                            uint32_t* __restrict__ tbw, uint32_t xor_prev)
 {
     const bool notify = (job.flags & CUPHY_HIP_SLOT_TB_NOTIFY) != 0;
     // every output of this CB out of the XCD's L2 before it is counted
@@ -510,9 +521,11 @@ __device__ __forceinline__ void complete_cb(const cuphyHipPuschSlotTb_t& job, ui
         tb_crc = (v == TB_ACC_ERROR) ? FUSED_CB_ERROR : v;
     } else {
         uint32_t* acc = tbw + 2u * job.share_offset;
+        // an accepted group's share was XORed in by fused_group (its old
+        // value is xor_prev); a rejected group flags the accumulator here
         uint32_t prev = (v == TB_ACC_ERROR)
                             ? __hip_atomic_fetch_or(acc, 0x80000000u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                            : __hip_atomic_fetch_xor(acc, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            : xor_prev;
         // the share is in the accumulator (its old value returned) before
         // the count is raised
         asm volatile("" : : "v"(prev) : "memory");
@@ -562,9 +575,10 @@ void pusch_slot_fused(const cuphyHipPuschSlotTb_t* __restrict__ tbs, int n_tbs,
             const PerTbParams& p = *job.prm;
             const uint32_t C = (uint32_t)__builtin_amdgcn_readfirstlane((int)p.num_CBs);
             if ((uint32_t)(cb + n) <= C) {
-                const uint32_t part = fused_group<TI, MAXR, GROUPED>(hb, sh, job, p, cb, n, lds_values);
+                uint32_t xprev = 0;
+                const uint32_t part = fused_group<TI, MAXR, GROUPED>(hb, sh, job, p, cb, n, lds_values, tbw, xprev);
                 __syncthreads();  // every output of the group issued by every thread
-                if (threadIdx.x == 0) complete_cb(job, C, part, (uint32_t)n, tbw);
+                if (threadIdx.x == 0) complete_cb(job, C, part, (uint32_t)n, tbw, xprev);
             }
         }
         __syncthreads();
@@ -688,9 +702,10 @@ __global__ __launch_bounds__(384) void pusch_serve_resident(const serve_ring_hdr
             __syncthreads();
             const uint32_t C = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_prm.num_CBs);
             if ((uint32_t)(cb + n) <= C) {
-                const uint32_t part = fused_group<TI, 46, true>(hb, sh, s_job, s_prm, cb, n, lds_values);
+                uint32_t xprev = 0;
+                const uint32_t part = fused_group<TI, 46, true>(hb, sh, s_job, s_prm, cb, n, lds_values, tbw, xprev);
                 __syncthreads();  // every output of the group issued by every thread
-                if (tid == 0) complete_cb(s_job, C, part, (uint32_t)n, tbw);
+                if (tid == 0) complete_cb(s_job, C, part, (uint32_t)n, tbw, xprev);
             }
         }
         __syncthreads();
