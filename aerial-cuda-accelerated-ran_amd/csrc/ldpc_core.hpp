@@ -222,28 +222,35 @@ __host__ __device__ constexpr uint32_t lm_app_bytes_max(int elem, bool split)
 
 // Packed rotation (PK): when every rotated offset is below 2^16 (2 * Zb <=
 // 65536), the lane keeps zb and zb - Zb as the two 16-bit halves of one
-// register (lane_rot_base); one packed add of the shift to both halves and
-// one 16-bit unsigned min of the halves give rot_index: two instructions
-// instead of three.  (zb - Zb wraps to at least 65536 - Zb > any offset.)
+// register (lane_rot_base); adding the shift to both halves and one 16-bit
+// unsigned min of the halves give rot_index: two instructions instead of
+// three.  (zb - Zb wraps to at least 65536 - Zb > any offset.)  The add is
+// a plain 32-bit add of sb * 0x10001 (two-operand encoding, twice the issue
+// rate of a packed add): the low half zb + sb < 2 * Zb <= 65536 never
+// carries into the high half, and the high half's carry leaves bit 31.
 __device__ __forceinline__ uint32_t lane_rot_base(uint32_t zb, uint32_t Zb)
 {
     return (zb & 0xFFFFu) | (((zb - Zb) & 0xFFFFu) << 16);
 }
-__device__ __forceinline__ uint32_t rot_index_packed(uint32_t zz, uint32_t sb)
+__device__ __forceinline__ uint32_t rot_index_packed(uint32_t zz, uint32_t sbb)
 {
-    const u16x2_t pq = __builtin_bit_cast(u16x2_t, zz) + (u16x2_t){(uint16_t)sb, (uint16_t)sb};
-    return (uint32_t)__builtin_elementwise_min(pq.x, pq.y);
+    const uint32_t pq = zz + sbb;  // sbb = sb * 0x10001
+    return (uint32_t)__builtin_elementwise_min((uint16_t)pq, (uint16_t)(pq >> 16));
 }
 
 // Byte offset of edge E's APP value for the lane.  Column-major (LM false):
 // zb, Zb = lane and lifting size x sizeof(T).  Lane-major (LM true): zb, Zb
-// = lane and lifting size x pitch Pb.  PK: zb is lane_rot_base(zb, Zb).
+// = lane and lifting size x pitch Pb.  PK: zb is lane_rot_base(zp, Zb), zp
+// the plain lane offset (otherwise zp == zb).  An edge whose shift is 0 in
+// every lifting set (the identity circulants of the parity columns) needs
+// no rotation.
 template <int BG, int E, typename T, bool LM, bool PK = false, bool SPL = false>
-__device__ __forceinline__ uint32_t edge_off(shift_ptr sh, uint32_t zb, uint32_t Zb, uint32_t Pb)
+__device__ __forceinline__ uint32_t edge_off(shift_ptr sh, uint32_t zb, uint32_t zp, uint32_t Zb, uint32_t Pb)
 {
     using BGT = bg_traits<BG>;
-    const uint32_t sb = sh[E] * (LM ? Pb : (uint32_t)sizeof(T));
-    const uint32_t rot = PK ? rot_index_packed(zb, sb) : rot_index(zb, sb, Zb);
+    constexpr bool ID = BGT::identity_edge(E);
+    const uint32_t sb = ID ? 0u : sh[E] * ((LM ? Pb : (uint32_t)sizeof(T)) * (PK ? 0x10001u : 1u));
+    const uint32_t rot = ID ? zp : PK ? rot_index_packed(zb, sb) : rot_index(zb, sb, Zb);
     if constexpr (LM)
         return rot + lm_col_offset(BG, BGT::col(E), (int)sizeof(T), SPL);
     else
@@ -260,7 +267,7 @@ __device__ __forceinline__ uint32_t edge_off(shift_ptr sh, uint32_t zb, uint32_t
 template <int BG, int R, int NR, int RS = R, bool LM = false, typename BP = unsigned char*, bool PK = false,
           bool SPL = false>
 __device__ __forceinline__ void process_row(float*, BP __restrict__ base, shift_ptr sh, uint32_t zb,
-                                            uint32_t Zb, float alpha, float (&st_n1)[NR], float (&st_n2)[NR],
+                                            uint32_t zp, uint32_t Zb, float alpha, float (&st_n1)[NR], float (&st_n2)[NR],
                                             uint32_t (&st_meta)[NR], uint32_t Pb = 0)
 {
     using BGT = bg_traits<BG>;
@@ -273,7 +280,7 @@ __device__ __forceinline__ void process_row(float*, BP __restrict__ base, shift_
     const uint32_t oidx = om >> 27;
     static_for<D>([&](auto J) {
         constexpr int j = decltype(J)::value;
-        off[j] = edge_off<BG, E0 + j, float, LM, PK, SPL>(sh, zb, Zb, Pb);
+        off[j] = edge_off<BG, E0 + j, float, LM, PK, SPL>(sh, zb, zp, Zb, Pb);
         const float mag = (oidx == (uint32_t)j) ? on2 : on1;
         const float old = u2f(f2u(mag) | ((om << (32 - D + j)) & 0x80000000u));
         t[j] = lds_ld<float>(base, off[j]) - old;
@@ -345,7 +352,7 @@ __device__ __forceinline__ u16x2_t umax2(u16x2_t a, u16x2_t b) { return __builti
 template <int BG, int R, int NW, int S0 = padded_row_start<BG>(R), bool LM = false, typename BP = unsigned char*,
           bool PK = false, bool SPL = false>
 __device__ __forceinline__ void process_row(_Float16*, BP __restrict__ base, shift_ptr sh, uint32_t zb,
-                                            uint32_t Zb, _Float16 alpha, half2_t (&st)[NW], uint32_t Pb = 0)
+                                            uint32_t zp, uint32_t Zb, _Float16 alpha, half2_t (&st)[NW], uint32_t Pb = 0)
 {
     using BGT = bg_traits<BG>;
     constexpr int E0 = BGT::row_start(R);
@@ -364,10 +371,10 @@ __device__ __forceinline__ void process_row(_Float16*, BP __restrict__ base, shi
     static_for<P>([&](auto K) {
         constexpr int k = decltype(K)::value;
         constexpr int j0 = 2 * k, j1 = 2 * k + 1;
-        off[j0] = edge_off<BG, E0 + j0, _Float16, LM, PK, SPL>(sh, zb, Zb, Pb);
+        off[j0] = edge_off<BG, E0 + j0, _Float16, LM, PK, SPL>(sh, zb, zp, Zb, Pb);
         uint32_t v = lds_ld<uint16_t>(base, off[j0]);
         if constexpr (j1 < D) {
-            off[j1] = edge_off<BG, E0 + j1, _Float16, LM, PK, SPL>(sh, zb, Zb, Pb);
+            off[j1] = edge_off<BG, E0 + j1, _Float16, LM, PK, SPL>(sh, zb, zp, Zb, Pb);
             v |= (uint32_t)lds_ld<uint16_t>(base, off[j1]) << 16;
         } else {
             v |= 0x7C000000u;  // the dummy edge: +inf, sign bit clear
@@ -427,8 +434,8 @@ __device__ __forceinline__ void process_row(_Float16*, BP __restrict__ base, shi
 // Parity of check row R, lane z: hard bit = APP < 0 (syndrome_check.m:38-40).
 template <int BG, int R, typename T, bool LM = false, typename BP = unsigned char*, bool PK = false,
           bool SPL = false>
-__device__ __forceinline__ uint32_t row_syndrome(BP __restrict__ base, shift_ptr sh, uint32_t zb, uint32_t Zb,
-                                                 uint32_t Pb = 0)
+__device__ __forceinline__ uint32_t row_syndrome(BP __restrict__ base, shift_ptr sh, uint32_t zb, uint32_t zp,
+                                                 uint32_t Zb, uint32_t Pb = 0)
 {
     using BGT = bg_traits<BG>;
     constexpr int E0 = BGT::row_start(R);
@@ -436,7 +443,7 @@ __device__ __forceinline__ uint32_t row_syndrome(BP __restrict__ base, shift_ptr
     uint32_t par = 0;
     static_for<D>([&](auto J) {
         constexpr int j = decltype(J)::value;
-        const uint32_t off = edge_off<BG, E0 + j, T, LM, PK, SPL>(sh, zb, Zb, Pb);
+        const uint32_t off = edge_off<BG, E0 + j, T, LM, PK, SPL>(sh, zb, zp, Zb, Pb);
         if constexpr (std::is_same<T, _Float16>::value)
             par ^= (uint32_t)lds_ld<uint16_t>(base, off);  // sign bits (APP values are never -0)
         else
@@ -600,8 +607,9 @@ __device__ __forceinline__ int run_iterations(T* __restrict__ app, const cb_job&
         // per row: the 316 (BG1) per-edge shifts and addresses are reloaded
         // and recomputed where used instead of being hoisted out of the
         // loops into (spilled) registers.
-        uint32_t zb = PK ? lane_rot_base(zlane * Pb, Zb) : zlane * Pb;
-        asm volatile("" : "+v"(zb));
+        uint32_t zp = zlane * Pb;
+        uint32_t zb = PK ? lane_rot_base(zp, Zb) : zp;
+        asm volatile("" : "+v"(zb), "+v"(zp));
         static_for<NR>([&](auto Rc) {
             constexpr int R = decltype(Rc)::value;
             // opaque row count: the R < mb tests stay scalar compares at
@@ -616,11 +624,11 @@ __device__ __forceinline__ int run_iterations(T* __restrict__ app, const cb_job&
                     constexpr int RS = (PAIR == 2) ? PP.row_slot[R] : R;
                     constexpr int S0 = (PAIR == 2) ? PP.pair_slot[R] : padded_row_start<BG>(R);
                     if constexpr (std::is_same<T, float>::value)
-                        process_row<BG, R, NRS, RS, LM, decltype(lane_base), PK, SPL>(app, lane_base, shr, zb, Zr,
+                        process_row<BG, R, NRS, RS, LM, decltype(lane_base), PK, SPL>(app, lane_base, shr, zb, zp, Zr,
                                                                                       alpha, st_n1, st_n2, st_meta,
                                                                                       Pb);
                     else
-                        process_row<BG, R, NW, S0, LM, decltype(lane_base), PK, SPL>(app, lane_base, shr, zb, Zr,
+                        process_row<BG, R, NW, S0, LM, decltype(lane_base), PK, SPL>(app, lane_base, shr, zb, zp, Zr,
                                                                                      alpha, st_h, Pb);
                 }
                 // barrier before the next row only where it shares a column
@@ -645,7 +653,7 @@ __device__ __forceinline__ int run_iterations(T* __restrict__ app, const cb_job&
                             asm volatile("" : "+s"(shr), "+s"(Zr));
                             if (active && (PAIR == 1 || lgrp == (uint32_t)PP.group[R]))
                                 fail |= row_syndrome<BG, R, T, LM, decltype(lane_base), PK, SPL>(lane_base, shr, zb,
-                                                                                                 Zr, Pb);
+                                                                                                 zp, Zr, Pb);
                         }
                     }
                 });

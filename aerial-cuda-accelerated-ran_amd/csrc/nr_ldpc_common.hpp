@@ -47,6 +47,13 @@ template <> struct bg_traits<1> {
     static constexpr int MAXDEG = 19;
     static constexpr int row_start(int r) { return nr_bg1_row_start[r]; }
     static constexpr int col(int e) { return nr_bg1_col[e]; }
+    // shift 0 in every lifting set: an identity circulant for every Z
+    static constexpr bool identity_edge(int e)
+    {
+        for (int i = 0; i < 8; ++i)
+            if (nr_bg1_shift[i][e] != 0) return false;
+        return true;
+    }
 };
 template <> struct bg_traits<2> {
     static constexpr int NROWS = NR_BG2_NUM_ROWS;  // 42
@@ -55,6 +62,12 @@ template <> struct bg_traits<2> {
     static constexpr int MAXDEG = 10;
     static constexpr int row_start(int r) { return nr_bg2_row_start[r]; }
     static constexpr int col(int e) { return nr_bg2_col[e]; }
+    static constexpr bool identity_edge(int e)
+    {
+        for (int i = 0; i < 8; ++i)
+            if (nr_bg2_shift[i][e] != 0) return false;
+        return true;
+    }
 };
 
 // TS 38.212 Table 5.3.2-1: lifting-set index of Z, -1 if Z is not a lifting size.
