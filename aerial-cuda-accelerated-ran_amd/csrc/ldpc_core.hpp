@@ -13,6 +13,9 @@ constexpr int LLR_LOADS = 8;
 #ifndef TWO_SET_PAIRS
 #define TWO_SET_PAIRS 6
 #endif
+#ifndef SDWA_PAIR
+#define SDWA_PAIR 1
+#endif
 
 // Reduced shifts of every (BG, Z) in device memory, constant-initialised in
 // the code object.  Read through an address-space-4 pointer so that the
@@ -347,6 +350,25 @@ __device__ __forceinline__ u16x2_t u2m(uint32_t x) { return __builtin_bit_cast(u
 __device__ __forceinline__ u16x2_t umin2(u16x2_t a, u16x2_t b) { return __builtin_elementwise_min(a, b); }
 __device__ __forceinline__ u16x2_t umax2(u16x2_t a, u16x2_t b) { return __builtin_elementwise_max(a, b); }
 
+// (v0 - m.lo, v1 - m.hi) in binary16, v0 and v1 two zero-extended 16-bit
+// LDS loads, without packing them first: one two-operand subtract into the
+// low half, one into the high half (SDWA destination select, low half
+// preserved) -- two full-rate instructions in place of a three-operand
+// pack and a packed subtract.  x - y is x + (-y) in IEEE arithmetic, so the
+// halves equal those of the packed subtract.  The wait state: a VALU result
+// written through a destination select must not be read by the next VALU
+// instruction.
+__device__ __forceinline__ half2_t sub_pair(uint32_t v0, uint32_t v1, half2_t m)
+{
+    uint32_t r;
+    asm("v_sub_f16_e32 %0, %1, %2\n\t"
+        "v_sub_f16_sdwa %0, %3, %2 dst_sel:WORD_1 dst_unused:UNUSED_PRESERVE src0_sel:WORD_0 src1_sel:WORD_1\n\t"
+        "s_nop 0"
+        : "=&v"(r)
+        : "v"(v0), "v"(h22u(m)), "v"(v1));
+    return u2h2(r);
+}
+
 // S0: message slot of the row's first edge (even: padded_row_start(R), or
 // the row's pair_slot within a row-pair lane group).
 template <int BG, int R, int NW, int S0 = padded_row_start<BG>(R), bool LM = false, typename BP = unsigned char*,
@@ -372,14 +394,17 @@ __device__ __forceinline__ void process_row(_Float16*, BP __restrict__ base, shi
         constexpr int k = decltype(K)::value;
         constexpr int j0 = 2 * k, j1 = 2 * k + 1;
         off[j0] = edge_off<BG, E0 + j0, _Float16, LM, PK, SPL>(sh, zb, zp, Zb, Pb);
-        uint32_t v = lds_ld<uint16_t>(base, off[j0]);
+        const uint32_t v0 = lds_ld<uint16_t>(base, off[j0]);
         if constexpr (j1 < D) {
             off[j1] = edge_off<BG, E0 + j1, _Float16, LM, PK, SPL>(sh, zb, zp, Zb, Pb);
-            v |= (uint32_t)lds_ld<uint16_t>(base, off[j1]) << 16;
+            const uint32_t v1 = lds_ld<uint16_t>(base, off[j1]);
+            if constexpr (SDWA_PAIR)
+                t[k] = sub_pair(v0, v1, st[W0 + k]);
+            else
+                t[k] = u2h2(v0 | (v1 << 16)) - st[W0 + k];
         } else {
-            v |= 0x7C000000u;  // the dummy edge: +inf, sign bit clear
+            t[k] = u2h2(v0 | 0x7C000000u) - st[W0 + k];  // the dummy edge: +inf, sign bit clear
         }
-        t[k] = u2h2(v) - st[W0 + k];
         const u16x2_t a = u2m(h22u(t[k]) & MAGS);
         mag[k] = a;
         if (k < NSET) {  // a set's first pair: min1 = |t|
