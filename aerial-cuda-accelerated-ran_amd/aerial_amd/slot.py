@@ -19,7 +19,8 @@ from .decoder import _stream_handle
 from .workloads import num_edges, nsys
 
 SLOT_CB_WORDS = L.MAX_WORDS_PER_CODE_BLOCK  # decoded-word stride per CB
-SMALL_MAXR = 8
+SMALL_MAXR = 8   # row capacities of the fused kernel's variants
+MID_MAXR = 16
 
 _LLR_TYPES = {torch.int8: L.CUPHY_R_8I, torch.float16: L.CUPHY_R_16F, torch.float32: L.CUPHY_R_32F}
 
@@ -120,10 +121,15 @@ class _Launch:
         self.workspace = None
 
 
+def row_class(mb):
+    """Kernel variant of a TB's CBs: 0 (mb <= 8), 1 (mb <= 16), 2 (all rows)."""
+    return 0 if mb <= SMALL_MAXR else 1 if mb <= MID_MAXR else 2
+
+
 class SlotDecoder:
-    """Fused decode of a list of SlotTb; one or two persistent launches on
-    two streams: the mb <= 8 register-lean variant when at least split_min
-    CBs (enough to fill the GPU) have mb <= 8, and the full variant."""
+    """Fused decode of a list of SlotTb; one persistent launch per kernel
+    variant (row class), each on its own stream: a class with fewer than
+    split_min CBs (too few to fill the GPU) joins the next larger one."""
 
     def __init__(self, decoder, split_min=256, concurrent=True, pack=True):
         self.dec = decoder
@@ -133,7 +139,7 @@ class SlotDecoder:
         self._lib = L.lib()
         self.launches = []
         self.graph = None
-        self.aux_stream = torch.cuda.Stream()
+        self.aux_streams = [torch.cuda.Stream(), torch.cuda.Stream()]
         self._norm_cache = {}
 
     def _alpha_bits(self, tb):
@@ -170,9 +176,17 @@ class SlotDecoder:
         psize = C.sizeof(L.PerTbParams)
         jobs = (L.PuschSlotTb * n)()
         share_offset = 0
-        n_small = sum(tb.params.num_CBs for tb in tbs if tb.mb <= SMALL_MAXR)
-        n_all = sum(tb.params.num_CBs for tb in tbs)
-        split = n_small >= self.split_min or n_small == n_all
+        counts = [0, 0, 0]
+        for tb in tbs:
+            counts[row_class(tb.mb)] += tb.params.num_CBs
+        last = max(c for c in range(3) if counts[c] > 0)
+        launch_class, pending = {}, []
+        for c in range(3):
+            pending.append(c)
+            if sum(counts[k] for k in pending) >= self.split_min or c >= last:
+                for k in pending:
+                    launch_class[k] = c
+                pending = []
         classes = {}
         for t, tb in enumerate(tbs):
             p = tb.params
@@ -196,7 +210,7 @@ class SlotDecoder:
             j.share_offset = share_offset
             share_offset += p.num_CBs
             cost = p.Zc * num_edges(p.bg, tb.mb) * max(1, tb.max_iterations)
-            key = (tb.mb <= SMALL_MAXR) if split else False
+            key = launch_class[row_class(tb.mb)]
             g = group_size(p, tb.mb, tb.descramble) if self.pack else 1
             for c in range(0, p.num_CBs, g):
                 n_cb = min(g, p.num_CBs - c)
@@ -207,7 +221,7 @@ class SlotDecoder:
         L.check("cuphyHipPuschSlotDecodeGetWorkspaceSize",
                 self._lib.cuphyHipPuschSlotDecodeGetWorkspaceSize(share_offset, C.byref(sz)))
         self.launches = []
-        for key in sorted(classes, key=lambda k: not k):
+        for key in sorted(classes):
             rows = classes[key]
             if order == "lpt":
                 rows.sort(key=lambda r: -r[0])
@@ -244,10 +258,14 @@ class SlotDecoder:
             for ln in self.launches:
                 self._launch_one(ln, main)
             return
-        self.aux_stream.wait_stream(main)
+        aux = self.aux_streams[:len(self.launches) - 1]
+        for s in aux:
+            s.wait_stream(main)
         self._launch_one(self.launches[0], main)
-        self._launch_one(self.launches[1], self.aux_stream)
-        main.wait_stream(self.aux_stream)
+        for ln, s in zip(self.launches[1:], aux):
+            self._launch_one(ln, s)
+        for s in aux:
+            main.wait_stream(s)
 
     def capture(self):
         """Capture one launch into a HIP graph.  The captured launch combines

@@ -6,8 +6,10 @@
 
 namespace nrldpc {
 
-// Row capacity of the high-code-rate kernel variants (mb <= 8).
+// Row capacity of the high-code-rate kernel variants (mb <= 8) and of the
+// fused slot kernel's middle variant (mb <= 16).
 constexpr int SMALL_MAXR = 8;
+constexpr int MID_MAXR = 16;
 // global loads a thread keeps in flight while loading a code block's LLRs
 constexpr int LLR_LOADS = 8;
 #ifndef TWO_SET_PAIRS

@@ -545,12 +545,14 @@ __device__ __forceinline__ void complete_cb(const cuphyHipPuschSlotTb_t& job, ui
 // each TB uses the pair at its first slot (TB-CRC accumulator and count of
 // finished CBs; see complete_cb).  The next item index is fetched while the current CB is
 // worked on.  The queue is returned to 0 by the last workgroup to leave.
-// The high-rate variant (MAXR <= 8) is held to 4 waves per SIMD (at most
-// 128 VGPRs): a 384-thread workgroup puts its 6 waves 2/2/1/1 on the CU's
-// SIMDs, so a second workgroup on the CU needs 4 wave slots on two SIMDs;
-// at its natural 157 VGPRs (3 slots) one workgroup per CU ran.
+// The high-rate variant (MAXR <= 8) and the middle one (MAXR <= 16) are
+// held to 4 waves per SIMD (at most 128 VGPRs): a 384-thread workgroup puts
+// its 6 waves 2/2/1/1 on the CU's SIMDs, so a second workgroup on the CU
+// needs 4 wave slots on two SIMDs; at its natural 157 VGPRs (3 slots) the
+// high-rate variant ran one workgroup per CU.  (The middle variant's BG1
+// rows beyond 8 spill a few registers; BG2 rows, half as long, do not.)
 template <typename TI, int MAXR, bool GROUPED>
-__global__ __launch_bounds__(384) __attribute__((amdgpu_waves_per_eu(MAXR <= SMALL_MAXR ? 4 : 1)))
+__global__ __launch_bounds__(384) __attribute__((amdgpu_waves_per_eu(MAXR <= MID_MAXR ? 4 : 1)))
 void pusch_slot_fused(const cuphyHipPuschSlotTb_t* __restrict__ tbs, int n_tbs,
                                                         const uint32_t* __restrict__ items, int n_items,
                                                         int lds_values, uint32_t* __restrict__ ws)
@@ -725,27 +727,32 @@ __global__ __launch_bounds__(256) void zero_words(uint32_t* __restrict__ p, int 
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
 }
 
+template <typename TI, bool GROUPED>
+void* fused_kernel_rows(int max_parity_nodes)
+{
+    return max_parity_nodes <= SMALL_MAXR ? (void*)&pusch_slot_fused<TI, SMALL_MAXR, GROUPED>
+           : max_parity_nodes <= MID_MAXR ? (void*)&pusch_slot_fused<TI, MID_MAXR, GROUPED>
+                                          : (void*)&pusch_slot_fused<TI, 46, GROUPED>;
+}
+
 template <bool GROUPED>
-void* fused_kernel_of(cuphyDataType_t llr_type, bool small)
+void* fused_kernel_of(cuphyDataType_t llr_type, int max_parity_nodes)
 {
     switch (llr_type) {
-    case CUPHY_R_8I:
-        return small ? (void*)&pusch_slot_fused<int8_t, SMALL_MAXR, GROUPED> : (void*)&pusch_slot_fused<int8_t, 46, GROUPED>;
-    case CUPHY_R_16F:
-        return small ? (void*)&pusch_slot_fused<_Float16, SMALL_MAXR, GROUPED>
-                     : (void*)&pusch_slot_fused<_Float16, 46, GROUPED>;
-    case CUPHY_R_32F:
-        return small ? (void*)&pusch_slot_fused<float, SMALL_MAXR, GROUPED> : (void*)&pusch_slot_fused<float, 46, GROUPED>;
+    case CUPHY_R_8I: return fused_kernel_rows<int8_t, GROUPED>(max_parity_nodes);
+    case CUPHY_R_16F: return fused_kernel_rows<_Float16, GROUPED>(max_parity_nodes);
+    case CUPHY_R_32F: return fused_kernel_rows<float, GROUPED>(max_parity_nodes);
     default: return nullptr;
     }
 }
 
-// Kernel of a launch: LLR type, mb class, and whether items hold groups
-// (the one-CB form keeps no group state: measurably faster per CB).
+// Kernel of a launch: LLR type, mb class (rows <= 8, <= 16, all), and
+// whether items hold groups (the one-CB form keeps no group state:
+// measurably faster per CB).
 void* fused_kernel(cuphyDataType_t llr_type, int max_parity_nodes, bool grouped)
 {
-    const bool small = max_parity_nodes <= SMALL_MAXR;
-    return grouped ? fused_kernel_of<true>(llr_type, small) : fused_kernel_of<false>(llr_type, small);
+    return grouped ? fused_kernel_of<true>(llr_type, max_parity_nodes)
+                   : fused_kernel_of<false>(llr_type, max_parity_nodes);
 }
 
 }  // namespace nrldpc

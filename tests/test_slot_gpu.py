@@ -111,8 +111,9 @@ class Case:
 def run_slot(dec, dtype, early_term, specs=SLOT, descramble=False):
     from aerial_amd.slot import SlotDecoder, SlotTb
     cases = [Case(k, s, dtype, descramble) for k, s in enumerate(specs)]
-    # int8: one launch of the full variant; otherwise the high-rate CBs run
-    # on the register-lean variant in a second launch
+    # int8: one launch of the full variant; otherwise one launch per row
+    # class (mb <= 8, mb <= 16 -- tx 1's BG1 Zc=104 and BG2 Zc=32 TBs -- and
+    # the full variant), on separate streams
     sd = SlotDecoder(dec, split_min=256 if dtype == "int8" else 1)
     for tx, (ndi, rv) in enumerate(TX):
         print("[case] slot %s tx %d ndi %d rv %d ET %d descr %d" % (dtype, tx, ndi, rv, early_term, descramble),
