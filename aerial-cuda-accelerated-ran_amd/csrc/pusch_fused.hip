@@ -74,6 +74,21 @@ __device__ __forceinline__ uint32_t crc_word(uint32_t w, const uint32_t* T)
     return reg;
 }
 
+// XOR of x over the wave, returned in every lane: within each row of 16
+// lanes by DPP (lane pairs, quads, half rows, rows: register-to-register,
+// no LDS round trip), then across the four rows by readlane.  Every lane
+// of the wave must be active (workgroups of whole waves, called in uniform
+// control flow).
+__device__ __forceinline__ uint32_t wave_xor(uint32_t x)
+{
+    x ^= (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+    x ^= (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
+    x ^= (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x141, 0xF, 0xF, false);  // row_half_mirror
+    x ^= (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x140, 0xF, 0xF, false);  // row_mirror
+    return (uint32_t)(__builtin_amdgcn_readlane((int)x, 0) ^ __builtin_amdgcn_readlane((int)x, 16) ^
+                      __builtin_amdgcn_readlane((int)x, 32) ^ __builtin_amdgcn_readlane((int)x, 48));
+}
+
 // The CB CRC (kind K of L bits over nb_cb = 4 * words bytes) and the CB's
 // TB-CRC24A share (over its db payload bytes, before the shift to its
 // place in the TB) in one pass: thread t takes word t, shifts its two
@@ -103,10 +118,8 @@ __device__ __forceinline__ void cb_and_share_crc(const fused_lds& sh, const uint
         rc = cmulmod<L, G>(rc, S[nb_cb - 4 * t - 4]);
         rs = cmulmod<24, CRC24A_P.g>(rs, sh.shift[K_CRC24A][max(0, db - 4 * t - nb)]);
     }
-    for (int off = 32; off > 0; off >>= 1) {
-        rc ^= __shfl_xor(rc, off, 64);
-        rs ^= __shfl_xor(rs, off, 64);
-    }
+    rc = wave_xor(rc);
+    rs = wave_xor(rs);
     if ((t & 63) == 0) {
         red0[t >> 6] = rc;
         red1[t >> 6] = rs;
