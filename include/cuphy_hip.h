@@ -501,8 +501,8 @@ cuphyStatus_t CUPHYWINAPI cuphyHipPuschSlotDecodeGetWorkspaceSize(int num_cbs, s
  * an item that does not fit is rejected); max_cbs_per_item: the largest
  * count of the items (1 runs the one-CB-per-workgroup kernel, which keeps
  * no group state; an item of several CBs is then rejected); max_parity_nodes: max mb
- * (<= 8 selects the register-lean high-code-rate variant; a CB with a
- * larger mb is rejected). */
+ * (<= 8 selects the register-lean high-code-rate variant, <= 16 the 16-row
+ * variant, otherwise the full one; a CB with a larger mb is rejected). */
 cuphyStatus_t CUPHYWINAPI cuphyHipPuschSlotDecode(cuphyLDPCDecoder_t decoder, const cuphyHipPuschSlotTb_t* d_tbs,
                                                   int num_tbs, const uint32_t* d_cb_items, int num_cb_items,
                                                   cuphyDataType_t llr_type, int max_lds_values, int max_parity_nodes,
