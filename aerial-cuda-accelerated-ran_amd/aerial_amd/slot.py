@@ -179,7 +179,7 @@ class SlotDecoder:
         counts = [0, 0, 0]
         for tb in tbs:
             counts[row_class(tb.mb)] += tb.params.num_CBs
-        last = max(c for c in range(3) if counts[c] > 0)
+        last = max((c for c in range(3) if counts[c] > 0), default=2)
         launch_class, pending = {}, []
         for c in range(3):
             pending.append(c)
