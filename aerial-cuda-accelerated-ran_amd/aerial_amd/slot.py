@@ -126,6 +126,21 @@ def row_class(mb):
     return 0 if mb <= SMALL_MAXR else 1 if mb <= MID_MAXR else 2
 
 
+def launch_classes(counts, split_min):
+    """Launch of each row class, given the CBs per class: a class joins the
+    next larger one until the CBs gathered reach split_min (enough to fill
+    the GPU) or no larger class has CBs.  Returns {row class: launch class}."""
+    last = max((c for c in range(3) if counts[c] > 0), default=2)
+    launch_class, pending = {}, []
+    for c in range(3):
+        pending.append(c)
+        if sum(counts[k] for k in pending) >= split_min or c >= last:
+            for k in pending:
+                launch_class[k] = c
+            pending = []
+    return launch_class
+
+
 class SlotDecoder:
     """Fused decode of a list of SlotTb; one persistent launch per kernel
     variant (row class), each on its own stream: a class with fewer than
@@ -179,14 +194,7 @@ class SlotDecoder:
         counts = [0, 0, 0]
         for tb in tbs:
             counts[row_class(tb.mb)] += tb.params.num_CBs
-        last = max((c for c in range(3) if counts[c] > 0), default=2)
-        launch_class, pending = {}, []
-        for c in range(3):
-            pending.append(c)
-            if sum(counts[k] for k in pending) >= self.split_min or c >= last:
-                for k in pending:
-                    launch_class[k] = c
-                pending = []
+        launch_class = launch_classes(counts, self.split_min)
         classes = {}
         for t, tb in enumerate(tbs):
             p = tb.params
